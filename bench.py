@@ -1,0 +1,200 @@
+"""Benchmark: 3D cone-beam distance-driven projection + backprojection on one MI355X.
+
+Workload (one "step"): A then B of the cone-flat geometry with a 256x256x64 volume at
+1 mm, a flat 512x64 detector at 1 mm pitch, SOD 500 mm, SDD 1000 mm and 720 views over
+2 pi, on a synthetic phantom of 16 seeded random ellipsoids (fp32), inputs resident in
+HBM.  value = voxel-views/s = Nx*Ny*Nz*V / (t_A + t_B).
+
+The JSON line also carries:
+  roofline      the dominant gather kernel's algorithmic FLOP rate, F = 4 * nnz per
+                launch (nnz = exact (voxel, view, cell) pairs with positive overlap,
+                tests/golden/nnz.json), over its HIP-event-timed average launch, against
+                the FP32 vector peak 157.3 TFLOP/s;
+  cpu_baseline  the reference's own CPU kernels (oracle/_ref, std::thread over all host
+                cores) timed on a bounded sample of the same workload (a subset of views,
+                scaled linearly in views) -- a reported baseline, not the target;
+  adjoint       the A + A^T pair (autograd use) in the same unit.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "3D cone DD proj+backproj voxel-views/s, batched+adjoint vs plain; % roofline"
+FP32_PEAK_TFLOPS = 157.3
+HBM_PEAK_GBS = 8000.0
+WORKLOAD = "cone_flat_256x256x64_512x64_720v"
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-views", type=int, default=36, help="views in the CPU baseline sample")
+    return ap.parse_args()
+
+
+def workload():
+    W, H, D, Nu, Nv, V = 256, 256, 64, 512, 64, 720
+    return dict(W=W, H=H, D=D, Nu=Nu, Nv=Nv, V=V, pix=1.0, det=1.0, sod=500.0, sdd=1000.0)
+
+
+def nnz_of(name):
+    path = os.path.join(ROOT, "tests", "golden", "nnz.json")
+    with open(path) as f:
+        return int(json.load(f)[name]["nnz"])
+
+
+def cpu_baseline(proj_opts, phantom, angles, nviews):
+    """Reference CPU kernels (oracle/_ref) on `nviews` evenly spaced views, A + B."""
+    import oracle
+
+    if not oracle.ref_available():
+        return None
+    V = angles.shape[0]
+    idx = np.linspace(0, V, nviews, endpoint=False).astype(np.int64)
+    sub = np.ascontiguousarray(angles[idx])
+    x = phantom[None]
+    oracle.ref_op3d("A", x, sub[:2], proj_opts)  # warm-up (thread start-up, page faults)
+    t0 = time.perf_counter()
+    y = oracle.ref_op3d("A", x, sub, proj_opts)
+    t1 = time.perf_counter()
+    oracle.ref_op3d("B", y, sub, proj_opts)
+    t2 = time.perf_counter()
+    vv = phantom.size * nviews
+    return {"value": vv / (t2 - t0), "unit": "voxel-views/s", "cores": os.cpu_count(), "kind": "reference",
+            "sample": f"A+B of the same volume over {nviews} of {V} views (evenly spaced), "
+                      f"scaled linearly in views; t_A={t1 - t0:.2f}s t_B={t2 - t1:.2f}s",
+            "threads": "std::thread x hardware_concurrency (reference cpu3d_common.h:17-22)"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build runs on a single MI355X and has no multi-GPU path "
+              f"(--gpus {args.gpus} requested)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    import xrecon_amd
+    from xrecon_amd import _backend as be
+    from xrecon_amd.phantom import random_ellipsoids
+
+    wl = workload()
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    proj = xrecon_amd.Projector3d((wl["W"], wl["H"], wl["D"]), (wl["Nu"], wl["Nv"]), wl["pix"], wl["det"],
+                                  wl["sod"], wl["sdd"], scan_type="flat")
+    angles = (torch.arange(wl["V"], dtype=torch.float32) * np.float32(2 * np.pi / wl["V"]))
+    phantom = random_ellipsoids((wl["W"], wl["H"], wl["D"]), (wl["pix"],) * 3, count=16, semi_axes=(5.0, 80.0),
+                                intensity=(0.1, 1.0), seed=1002)
+    x = torch.from_numpy(phantom).to(dev).view(1, 1, wl["D"], wl["H"], wl["W"])
+    plan = proj.plan(angles, dev)
+    stream = torch.cuda.current_stream(dev)
+    sid = stream.cuda_stream
+
+    A, B, AT = be.OP_PROJECTION, be.OP_BACKPROJECTION, be.OP_PROJECTION_T
+    sino = torch.empty((1, 1, wl["V"], wl["Nv"], wl["Nu"]), device=dev)
+    recon = torch.empty_like(x)
+    grad = torch.empty_like(x)
+    ws_a = torch.empty(plan.workspace_bytes(A, 1), dtype=torch.uint8, device=dev)
+    ws_b = torch.empty(plan.workspace_bytes(B, 1), dtype=torch.uint8, device=dev)
+
+    def step(ev=None):
+        # A: prefix pass, then the cell-gather kernel (events bracket the gather only)
+        plan.execute_into(A, x, sino, ws_a, sid, 0, 0, 1)
+        if ev: ev[0].record(stream)
+        plan.execute_into(A, x, sino, ws_a, sid, 0, 0, 2)
+        if ev: ev[1].record(stream)
+        plan.execute_into(B, sino, recon, ws_b, sid, 0, 0, 1)
+        if ev: ev[2].record(stream)
+        plan.execute_into(B, sino, recon, ws_b, sid, 0, 0, 2)
+        if ev: ev[3].record(stream)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+
+    K = args.steps
+    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(K)]
+    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    start.record(stream)
+    for k in range(K):
+        step(evs[k])
+    end.record(stream)
+    torch.cuda.synchronize()
+    total_ms = start.elapsed_time(end)
+    ms_step = total_ms / K
+    t_cell = sum(e[0].elapsed_time(e[1]) for e in evs) / K
+    t_vox = sum(e[2].elapsed_time(e[3]) for e in evs) / K
+
+    # adjoint pair A + A^T (the autograd pair), same unit, untimed-kernel breakdown
+    ws_at = torch.empty(plan.workspace_bytes(AT, 1), dtype=torch.uint8, device=dev)
+    for _ in range(2):
+        plan.execute_into(AT, sino, grad, ws_at, sid, 0, 0, 0)
+    torch.cuda.synchronize()
+    s2, e2 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    s2.record(stream)
+    for _ in range(K):
+        plan.execute_into(A, x, sino, ws_a, sid, 0, 0, 0)
+        plan.execute_into(AT, sino, grad, ws_at, sid, 0, 0, 0)
+    e2.record(stream)
+    torch.cuda.synchronize()
+    ms_adj = s2.elapsed_time(e2) / K
+
+    vox_views = wl["W"] * wl["H"] * wl["D"] * wl["V"]
+    value = vox_views / (ms_step * 1e-3)
+    nnz = nnz_of(WORKLOAD)
+    flops = 4.0 * nnz
+    dominant, t_dom = ("cell_gather(A)", t_cell) if t_cell >= t_vox else ("voxel_gather(B)", t_vox)
+    achieved = flops / (t_dom * 1e-3) / 1e12
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(list(proj._host_options), phantom, angles.numpy(), args.cpu_views)
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "voxel-views/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: 16 seeded random ellipsoids (semi-axes 5-80 mm, intensities 0.1-1.0, seed 1002)",
+        "config": {"workload": "cone-flat A+B: volume 256x256x64 @1mm, flat detector 512x64 @1mm, "
+                               "SOD 500 mm, SDD 1000 mm, 720 views over 2pi, B*C=1",
+                   "voxel_views_per_operator": vox_views, "nnz_per_operator": nnz},
+        "roofline": {"bound": "valu", "kernel": dominant, "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
+                     "note": "FP32 vector-ALU roof (packed FP32 peak); no dense contraction, so no MFMA; "
+                             "F = 4*nnz per launch"},
+        "kernels_ms": {"cell_gather_A": t_cell, "voxel_gather_B": t_vox,
+                       "prefix_passes": ms_step - t_cell - t_vox},
+        "adjoint": {"ms_per_pair": ms_adj, "value": vox_views / (ms_adj * 1e-3), "unit": "voxel-views/s",
+                    "pair": "A + A^T"},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,195 @@
+/*
+ * xrecon_amd — C-ABI of the MI355X (gfx950) distance-driven projector library
+ * (libxrecon_amd.so).  Plain C types only: host descriptors, raw device pointers,
+ * byte counts and an opaque hipStream_t passed as void*.  No torch types.
+ *
+ * Layouts are the reference's: volumes are [num_batch, Nz, Ny, Nx] float32 with x fastest
+ * (the reference's [B, C, D, H, W] folded to num_batch = B*C), sinograms are
+ * [num_batch, num_view, Nv, Nu] float32 (the reference's [B, C, V, Nv, Nu]).
+ *
+ * Every call returns 0 on success or a negative XR_E* code; xr_last_error() then
+ * returns a message (thread-local).  Nothing calls exit(), nothing synchronises the
+ * device, and the plan API never allocates on the execute path, so a sequence of
+ * xr_plan*_execute calls can be captured into a hipGraph.
+ */
+#ifndef XRECON_AMD_H
+#define XRECON_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Error codes. */
+#define XR_OK 0
+#define XR_EINVAL -1   /* bad argument (shape, geometry, null pointer, unknown op) */
+#define XR_EHIP -2     /* a HIP runtime call failed */
+#define XR_ENOMEM -3   /* allocation failed (plan creation / one-shot entry points only) */
+#define XR_EWORKSPACE -4 /* workspace smaller than xr_plan3d_workspace_bytes() */
+
+/* Geometry types: the reference's GeometryType (include/types.h:7-11). */
+#define XR_PARALLEL 0
+#define XR_FLAT 1
+#define XR_ARC 2
+
+/* Operators.  The reference binds one function per operator
+ * (src/api/torch_binding.cpp:407-523); here they are one enum. */
+#define XR_OP_PROJECTION 0            /* A     : volume -> sinogram  (forward3d_cuda)            */
+#define XR_OP_PROJECTION_T 1          /* A^T   : sinogram -> volume  (forward3d_t_cuda)          */
+#define XR_OP_BACKPROJECTION 2        /* B     : sinogram -> volume  (backward3d_cuda)           */
+#define XR_OP_BACKPROJECTION_W 3      /* B_W   : sinogram -> volume  (weighted_backward3d_cuda)  */
+#define XR_OP_BACKPROJECTION_T 4      /* B^T   : volume -> sinogram  (backward3d_t_cuda)         */
+#define XR_OP_BACKPROJECTION_W_T 5    /* B_W^T : volume -> sinogram  (weighted_backward3d_t_cuda)*/
+
+/* Volume descriptor: the reference's Volume3d (include/types.h:62-80).  Shifts in mm. */
+typedef struct xr_volume3d {
+    int nx, ny, nz;
+    float dx, dy, dz;
+    float shift_x, shift_y, shift_z;
+} xr_volume3d;
+
+/* Detector/scan descriptor: the reference's Geometry3d (include/types.h:82-119) without
+ * the helical fields.  du/shift_u follow the kernel convention of options[8]/[15]
+ * (xrecon/projector/projector3d.py:81-82): flat = physical mm, arc = radians,
+ * parallel = mm.  dv/shift_v are physical mm. */
+typedef struct xr_geometry3d {
+    int type;
+    int nu, nv;
+    float du, dv;
+    float shift_u, shift_v;
+    float sod, sdd;
+} xr_geometry3d;
+
+/* 2D descriptors: the reference's Volume2d / Geometry2d (include/types.h:35-60).  In 2D,
+ * du is already the effective pitch (xrecon/projector/geometry.py:106-109). */
+typedef struct xr_volume2d {
+    int nx, ny;
+    float dx, dy;
+    float shift_x, shift_y;
+} xr_volume2d;
+
+typedef struct xr_geometry2d {
+    int type;
+    int nu;
+    float du, shift_u;
+    float sod, sdd;
+} xr_geometry2d;
+
+typedef struct xr_plan3d xr_plan3d;
+
+/* Last error message of the calling thread ("" if none). */
+const char *xr_last_error(void);
+
+/* Library version string. */
+const char *xr_version(void);
+
+/* ---------------------------------------------------------------- plan API (3D) */
+
+/* Builds the per-view geometry tables in FP64 on the host from float32 host angles
+ * (the reference computes sinf/cosf per voxel per view on the device instead,
+ * include/xrmath.h:116-123), stores them as FP32 device tables on `device`, and
+ * returns an opaque plan.  Allocates device memory; not capturable. */
+int xr_plan3d_create(const xr_volume3d *vol, const xr_geometry3d *geo, const float *angles_host,
+                     int num_view, int device, xr_plan3d **out);
+
+/* Releases a plan and its device tables. */
+void xr_plan3d_destroy(xr_plan3d *plan);
+
+/* Device workspace, in bytes, that xr_plan3d_execute needs for `op` on `num_batch`
+ * volumes (0 is possible).  The workspace holds prefix-summed copies of the input. */
+size_t xr_plan3d_workspace_bytes(const xr_plan3d *plan, int op, int num_batch);
+
+/* Runs `op` on `num_batch` instances on `stream` (a hipStream_t; NULL = the legacy
+ * default stream).  `in`/`out` are device pointers in the layouts above; `out` is fully
+ * overwritten.  No allocation, no synchronisation: capturable into a hipGraph. */
+int xr_plan3d_execute(const xr_plan3d *plan, int op, const float *in, float *out, int num_batch,
+                      void *workspace, size_t workspace_bytes, void *stream);
+
+/* The two passes of xr_plan3d_execute, callable separately (both capturable):
+ * phase 1 builds the prefix tables of `in` in the workspace (out unused), phase 2 runs
+ * the gather from the workspace into `out` (in unused); phase 0 does both.  Used to
+ * time the gather kernels on their own, and to reuse one prefix pass for several
+ * gathers over the same input. */
+#define XR_PHASE_ALL 0
+#define XR_PHASE_PREFIX 1
+#define XR_PHASE_GATHER 2
+int xr_plan3d_execute_phase(const xr_plan3d *plan, int op, int phase, const float *in, float *out,
+                            int num_batch, void *workspace, size_t workspace_bytes, int tile_rows,
+                            int voxel_chunk, void *stream);
+
+/* Number of views / a copy of the plan's per-view flags (x-driven, keep-order), as
+ * decided on the host with the reference's FP32 rule (include/cpu3d_common.h:65, :87). */
+int xr_plan3d_num_view(const xr_plan3d *plan);
+int xr_plan3d_view_flags(const xr_plan3d *plan, int *xdriven, int *keep_order);
+
+/* ------------------------------------------ batched uneven instances (one launch) */
+
+/* Runs the same `op` over `num_instances` independent plans (uneven volumes, detectors
+ * and view counts) as persistent kernels whose work list spans every instance:
+ * one prefix pass, one gather pass.  in[i]/out[i] are the instances' device buffers,
+ * num_batch[i] their batch counts.  `workspace` must hold
+ * xr_batch3d_workspace_bytes().  `tile_rows` (1, 2, 4 or 8; 0 = default) is the
+ * detector-row register blocking of the cell kernels, `voxel_chunk` (4, 8 or 16;
+ * 0 = default) the z-chunk of the voxel kernels. */
+size_t xr_batch3d_workspace_bytes(xr_plan3d *const *plans, int num_instances, int op,
+                                  const int *num_batch);
+int xr_batch3d_execute(xr_plan3d *const *plans, int num_instances, int op, const float *const *in,
+                       float *const *out, const int *num_batch, void *workspace,
+                       size_t workspace_bytes, int tile_rows, int voxel_chunk, void *stream);
+
+/* Same as xr_plan3d_execute with explicit tiling parameters (see above); used by the
+ * autotuner. */
+int xr_plan3d_execute_tuned(const xr_plan3d *plan, int op, const float *in, float *out,
+                            int num_batch, void *workspace, size_t workspace_bytes, int tile_rows,
+                            int voxel_chunk, void *stream);
+
+/* ----------------------------------------------- one-shot entry points (3D, 2D) */
+/* Drop-in counterparts of the reference's C++ operator API (include/functions.h).
+ * They build a plan, allocate the workspace, run on `stream`, synchronise `stream`,
+ * and free everything: convenient, but they allocate and synchronise, so they are not
+ * capturable.  Unlike the reference, `angles_host` is a HOST pointer (the geometry
+ * tables are computed on the host in FP64). */
+
+/* include/functions.h:374-382  forward3d_cuda(image, proj, ang, vol, geo, num_batch, num_view) */
+int xr_forward3d(const float *image, float *proj, const float *angles_host, const xr_volume3d *vol,
+                 const xr_geometry3d *geo, int num_batch, int num_view, void *stream);
+/* include/functions.h:384-392  forward3d_t_cuda */
+int xr_forward3d_t(float *image, const float *proj, const float *angles_host, const xr_volume3d *vol,
+                   const xr_geometry3d *geo, int num_batch, int num_view, void *stream);
+/* include/functions.h:394-402  backward3d_cuda */
+int xr_backward3d(float *image, const float *proj, const float *angles_host, const xr_volume3d *vol,
+                  const xr_geometry3d *geo, int num_batch, int num_view, void *stream);
+/* include/functions.h:404-412  weighted_backward3d_cuda */
+int xr_weighted_backward3d(float *image, const float *proj, const float *angles_host,
+                           const xr_volume3d *vol, const xr_geometry3d *geo, int num_batch,
+                           int num_view, void *stream);
+/* include/functions.h:414-422  backward3d_t_cuda(proj, image, ...) */
+int xr_backward3d_t(float *proj, const float *image, const float *angles_host, const xr_volume3d *vol,
+                    const xr_geometry3d *geo, int num_batch, int num_view, void *stream);
+/* include/functions.h:424-432  weighted_backward3d_t_cuda(proj, image, ...) */
+int xr_weighted_backward3d_t(float *proj, const float *image, const float *angles_host,
+                             const xr_volume3d *vol, const xr_geometry3d *geo, int num_batch,
+                             int num_view, void *stream);
+
+/* 2D operators run as the 3D kernels on a one-slice volume and a one-row detector.
+ * include/functions.h:216-274  forward2d_cuda, backward2d_cuda, weighted_backward2d_cuda,
+ * forward2d_t_cuda, backward2d_t_cuda, weighted_backward2d_t_cuda.  Image [nb, Ny, Nx],
+ * sinogram [nb, num_view, Nu]. */
+int xr_op2d(int op, const float *in, float *out, const float *angles_host, const xr_volume2d *vol,
+            const xr_geometry2d *geo, int num_batch, int num_view, void *stream);
+
+/* Maps a 2D descriptor pair onto the equivalent one-slice 3D descriptors used by the
+ * plan API (so 2D operators can use plans, workspaces and graphs too).  The returned
+ * geometry type carries XR_GEOMETRY_2D, which tells xr_plan3d_create to apply the 2D
+ * conventions (effective pitch, no depth magnification). */
+#define XR_GEOMETRY_2D 0x100
+int xr_geometry2d_as_3d(const xr_volume2d *vol2, const xr_geometry2d *geo2, xr_volume3d *vol3,
+                        xr_geometry3d *geo3);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* XRECON_AMD_H */

@@ -1,0 +1,111 @@
+"""Shared test helpers: geometry cases, error norms, and the CPU emulator of the kernel
+bodies (tests/native/libxr_emu.so)."""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+import oracle
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+EMU_SO = os.path.join(HERE, "native", "libxr_emu.so")
+
+OPS = ("A", "AT", "B", "BW", "BT", "BWT")
+VOLUME_IN = ("A", "BT", "BWT")
+OP_ID = {"A": 0, "AT": 1, "B": 2, "BW": 3, "BT": 4, "BWT": 5}
+
+
+def opts3d(W, H, D, Nu, Nv, dx, dy, dz, du, dv, SOD, SDD, scan, shx=0.0, shy=0.0, shz=0.0, shu=0.0, shv=0.0):
+    """The reference's 18-entry 3D options vector (projector3d.py:98-107)."""
+    return [W, H, D, Nu, Nv, dx, dy, dz, du, dv, SOD, SDD, shx, shy, shz, shu, shv, scan]
+
+
+def opts2d(W, H, Nu, dx, dy, du, SOD, SDD, scan, shx=0.0, shy=0.0, shu=0.0):
+    """The reference's 12-entry 2D options vector (projector.py:106-110)."""
+    return [W, H, Nu, dx, dy, du, SOD, SDD, shx, shy, shu, scan]
+
+
+# Small but non-trivial geometries: non-square pixels, shifted volume and detector,
+# enough rows and columns to cross several tile boundaries.
+CASES3D = {
+    "flat": opts3d(32, 28, 10, 48, 12, 1.0, 1.1, 0.9, 1.0, 1.0, 500.0, 1000.0, 1, 0.3, -0.2, 0.1, 0.25, 0.1),
+    "arc": opts3d(32, 28, 10, 48, 12, 1.0, 1.1, 0.9, 1.0 / 1000.0, 1.0, 500.0, 1000.0, 2, 0.3, -0.2, 0.1,
+                  0.25 / 1000.0, 0.1),
+    "par": opts3d(24, 20, 6, 40, 8, 1.0, 1.2, 1.0, 0.9, 1.1, 0.0, 0.0, 0, 0.3, -0.2, 0.1, 0.2, 0.1),
+}
+
+
+def angles_with_ties(V: int, offset: float = 0.01) -> np.ndarray:
+    """V float32 angles over 2 pi with the 45-degree tie views inserted exactly as the
+    float32 grid k*2pi/8 produces them."""
+    a = (np.arange(V, dtype=np.float32) * np.float32(2 * np.pi / V) + np.float32(offset)).astype(np.float32)
+    for q, k in enumerate((1, 3, 5, 7)):
+        a[(q * V) // 4 + 1] = np.float32(k * 2 * np.pi / 8)
+    return a
+
+
+def uniform_angles(V: int) -> np.ndarray:
+    return (np.arange(V, dtype=np.float32) * np.float32(2 * np.pi / V)).astype(np.float32)
+
+
+def shapes3d(opts, nb, V):
+    W, H, D, Nu, Nv = (int(v) for v in opts[:5])
+    return (nb, D, H, W), (nb, V, Nv, Nu)
+
+
+def rel_l2(a, b) -> float:
+    a = np.asarray(a, dtype=np.float64).ravel()
+    b = np.asarray(b, dtype=np.float64).ravel()
+    nb = np.linalg.norm(b)
+    return float(np.linalg.norm(a - b) / (nb if nb > 0 else 1.0))
+
+
+_emu = None
+
+
+def emu_available() -> bool:
+    return os.path.exists(EMU_SO)
+
+
+def _load_emu():
+    global _emu
+    if _emu is None:
+        lib = ctypes.CDLL(EMU_SO)
+        lib.emu_op3d.restype = ctypes.c_int
+        lib.emu_op3d.argtypes = ([ctypes.c_int] + [ctypes.c_void_p] * 5 + [ctypes.c_int, ctypes.c_void_p,
+                                                                            ctypes.c_void_p] + [ctypes.c_int] * 4)
+        lib.emu_error.restype = ctypes.c_char_p
+        _emu = lib
+    return _emu
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def emu_op3d(op, x, angles, opts, R=0, KC=0, is2d=False, nthreads=None):
+    """The product's kernel bodies run on the CPU (float32 in/out)."""
+    lib = _load_emu()
+    vn, vf, gn, gf = oracle.split3d(opts)
+    if is2d:
+        gn = gn.copy()
+        gn[0] |= 0x100
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    angles = np.ascontiguousarray(angles, dtype=np.float32)
+    nb = x.shape[0]
+    vol, sino = shapes3d(opts, nb, angles.shape[0])
+    out = np.zeros(sino if op in VOLUME_IN else vol, dtype=np.float32)
+    rc = lib.emu_op3d(OP_ID[op], _p(vn), _p(vf), _p(gn), _p(gf), _p(angles), angles.shape[0], _p(x), _p(out), nb,
+                      R, KC, nthreads or min(8, os.cpu_count() or 1))
+    if rc != 0:
+        raise RuntimeError(lib.emu_error().decode())
+    return out
+
+
+def opts2d_as_3d(o2):
+    """2D options -> the one-slice 3D options the kernels use (with the 2D flag set by
+    emu_op3d(is2d=True)): W, H, 1, Nu, 1, dx, dy, 1, du, 1, SOD, SDD, shx, shy, 0, shu, 0, type."""
+    W, H, Nu, dx, dy, du, SOD, SDD, shx, shy, shu, scan = o2
+    return [W, H, 1, Nu, 1, dx, dy, 1.0, du, 1.0, SOD, SDD, shx, shy, 0.0, shu, 0.0, scan]

@@ -1,0 +1,151 @@
+// Test-only CPU emulator of the product kernels.
+//
+// Runs the exact per-thread bodies of the gfx950 kernels (xrecon_amd/csrc/dd_body.h),
+// with the product's own host plan builder (plan.cpp) and operator routing, one
+// "thread" at a time on the CPU.  The CPU test suite uses it to check the kernel logic
+// against the oracle without a GPU; the GPU suite checks the compiled kernels.  It is
+// not part of the product library and nothing in xrecon_amd loads it.
+#include <cstring>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../xrecon_amd/csrc/dd_body.h"
+
+using namespace xr;
+
+namespace {
+
+template <int OPK, int GEO, int R>
+void run_cell_R(const Plan3d &p, const float2 *WX, const float2 *WY, float *out, int nb, int nthreads)
+{
+    const CellArgs a = make_cell_args(p, nb);
+    const int ngroups = (a.nv + R - 1) / R;
+    const int items = nb * p.nview;
+    std::vector<std::thread> th;
+    for (int t = 0; t < nthreads; ++t) {
+        th.emplace_back([&, t]() {
+            for (int it = t; it < items; it += nthreads) {
+                const int v = it % p.nview, b = it / p.nview;
+                for (int g = 0; g < ngroups; ++g)
+                    for (int c = 0; c < a.nu; ++c)
+                        cell_body<OPK, GEO, R>(p.d_hdr, p.d_lineC, p.d_col, WX, WY, out, a, c, g * R, v, b);
+            }
+        });
+    }
+    for (auto &x : th) x.join();
+}
+
+template <int OPK, int GEO>
+void run_cell(const Plan3d &p, int R, const float2 *WX, const float2 *WY, float *out, int nb, int nt)
+{
+    if (R == 1) run_cell_R<OPK, GEO, 1>(p, WX, WY, out, nb, nt);
+    else if (R == 2) run_cell_R<OPK, GEO, 2>(p, WX, WY, out, nb, nt);
+    else if (R == 4) run_cell_R<OPK, GEO, 4>(p, WX, WY, out, nb, nt);
+    else run_cell_R<OPK, GEO, 8>(p, WX, WY, out, nb, nt);
+}
+
+template <int OPK, int GEO, int KC>
+void run_voxel_KC(const Plan3d &p, const float2 *Q, float *out, int nb, int nthreads)
+{
+    const VoxArgs a = make_vox_args(p, nb, KC);
+    const int items = nb * a.nchunk * a.ny;
+    std::vector<std::thread> th;
+    for (int t = 0; t < nthreads; ++t) {
+        th.emplace_back([&, t]() {
+            for (int it = t; it < items; it += nthreads) {
+                const int j = it % a.ny, rest = it / a.ny;
+                const int chunk = rest % a.nchunk, b = rest / a.nchunk;
+                for (int i = 0; i < a.nx; ++i) voxel_body<OPK, GEO, KC>(p.d_hdr, p.d_lineV, p.d_col, Q, out, a, i, j, chunk * KC, b);
+            }
+        });
+    }
+    for (auto &x : th) x.join();
+}
+
+template <int OPK, int GEO>
+void run_voxel(const Plan3d &p, int KC, const float2 *Q, float *out, int nb, int nt)
+{
+    if (KC == 1) run_voxel_KC<OPK, GEO, 1>(p, Q, out, nb, nt);
+    else if (KC == 4) run_voxel_KC<OPK, GEO, 4>(p, Q, out, nb, nt);
+    else if (KC == 8) run_voxel_KC<OPK, GEO, 8>(p, Q, out, nb, nt);
+    else run_voxel_KC<OPK, GEO, 16>(p, Q, out, nb, nt);
+}
+
+template <int GEO>
+void dispatch(const Plan3d &p, const Route &rt, int R, int KC, const float *in, float *out, int nb, int nt)
+{
+    if (rt.cell) {
+        const int64_t plane = (int64_t)p.vol.nx * p.vol.ny;
+        std::vector<float2> WX((size_t)nb * (p.vol.nz + 1) * plane), WY(WX.size());
+        for (int b = 0; b < nb; ++b)
+            for (int64_t e = 0; e < plane; ++e) vol_prefix_body(in, WX.data(), plane, p.vol.nz, b, e);
+        const int nx = p.vol.nx, ny = p.vol.ny;
+        for (int64_t q = 0; q < (int64_t)nb * (p.vol.nz + 1); ++q)
+            for (int j = 0; j < ny; ++j)
+                for (int i = 0; i < nx; ++i) WY[q * plane + (int64_t)i * ny + (ny - 1 - j)] = WX[q * plane + (int64_t)j * nx + i];
+        if (rt.kop == CELL_A) run_cell<CELL_A, GEO>(p, R, WX.data(), WY.data(), out, nb, nt);
+        else if (rt.kop == CELL_BT) run_cell<CELL_BT, GEO>(p, R, WX.data(), WY.data(), out, nb, nt);
+        else run_cell<CELL_BWT, GEO>(p, R, WX.data(), WY.data(), out, nb, nt);
+    } else {
+        const SinoArgs sa = make_sino_args(p, nb);
+        std::vector<float2> Q((size_t)nb * p.nview * (p.geo.nv + 1) * p.geo.nu);
+        for (int b = 0; b < nb; ++b)
+            for (int v = 0; v < p.nview; ++v)
+                for (int c = 0; c < p.geo.nu; ++c) {
+                    if (rt.kop == VOX_AT) sino_prefix_body<SINO_AT, GEO>(p.d_hdr, p.d_col, in, Q.data(), sa, b, v, c);
+                    else sino_prefix_body<SINO_PLAIN, GEO>(p.d_hdr, p.d_col, in, Q.data(), sa, b, v, c);
+                }
+        if (rt.kop == VOX_B) run_voxel<VOX_B, GEO>(p, KC, Q.data(), out, nb, nt);
+        else if (rt.kop == VOX_BW) run_voxel<VOX_BW, GEO>(p, KC, Q.data(), out, nb, nt);
+        else run_voxel<VOX_AT, GEO>(p, KC, Q.data(), out, nb, nt);
+    }
+}
+
+std::string g_err;
+
+}  // namespace
+
+extern "C" {
+
+const char *emu_error(void) { return g_err.c_str(); }
+
+// vn = {nx, ny, nz}, vf = {dx, dy, dz, shx, shy, shz}, gn = {type(|0x100 for 2D), nu, nv},
+// gf = {du, dv, shu, shv, sod, sdd}; tile_rows / voxel_chunk 0 = the library defaults
+// (8/4/2/1 by Nv, 16/8/4/1 by Nz).
+int emu_op3d(int op, const int *vn, const float *vf, const int *gn, const float *gf, const float *angles, int nview,
+             const float *in, float *out, int nbatch, int tile_rows, int voxel_chunk, int nthreads)
+{
+    Plan3d p;
+    p.vol = xr_volume3d{vn[0], vn[1], vn[2], vf[0], vf[1], vf[2], vf[3], vf[4], vf[5]};
+    p.geo = xr_geometry3d{gn[0] & 0xff, gn[1], gn[2], gf[0], gf[1], gf[2], gf[3], gf[4], gf[5]};
+    p.is2d = (gn[0] & 0x100) ? 1 : 0;
+    p.kind = p.geo.type;
+    p.nview = nview;
+    PlanTables t;
+    std::string err;
+    int rc = build_tables(p, angles, t, err);
+    if (rc) {
+        g_err = err;
+        return rc;
+    }
+    p.d_hdr = t.hdr.data();
+    p.d_lineC = t.lineC.data();
+    p.d_lineV = t.lineV.data();
+    p.d_col = t.col.data();
+    const Route rt = route(p, op);
+    if (rt.kop < 0) {
+        g_err = "unknown op";
+        return XR_EINVAL;
+    }
+    int R = tile_rows, KC = voxel_chunk;
+    if (R == 0) R = p.geo.nv >= 32 ? 8 : p.geo.nv >= 16 ? 4 : p.geo.nv >= 4 ? 2 : 1;
+    if (KC == 0) KC = p.vol.nz >= 64 ? 16 : p.vol.nz >= 8 ? 8 : p.vol.nz >= 4 ? 4 : 1;
+    if (nthreads < 1) nthreads = 1;
+    if (p.kind == GEO_FLAT) dispatch<GEO_FLAT>(p, rt, R, KC, in, out, nbatch, nthreads);
+    else if (p.kind == GEO_ARC) dispatch<GEO_ARC>(p, rt, R, KC, in, out, nbatch, nthreads);
+    else dispatch<GEO_PAR>(p, rt, R, KC, in, out, nbatch, nthreads);
+    return XR_OK;
+}
+
+}  // extern "C"

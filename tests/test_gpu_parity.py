@@ -1,0 +1,198 @@
+"""GPU parity: the gfx950 kernels (through the C-ABI and the torch extension) against
+the FP64 oracle and the reference CPU kernels, plus full-size properties.
+
+Tolerances: relative L2 <= 1e-5 against the FP64 oracle for every FP32 operator
+(north-star bar); <= 1e-4 against the reference's own FP32 CPU arc kernels, whose
+FP32 angle sums (tanf(beta + gamma)) deviate from FP64 by up to ~1e-4 at large beta
+(measured in tests/test_oracle.py).
+"""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from helpers import CASES3D, OPS, VOLUME_IN, angles_with_ties, emu_available, emu_op3d, rel_l2, shapes3d, \
+    uniform_angles
+
+pytestmark = pytest.mark.gpu
+
+OPID = {"A": 0, "AT": 1, "B": 2, "BW": 3, "BT": 4, "BWT": 5}
+
+
+def _native():
+    from xrecon_amd import _backend as be
+
+    return be.native()
+
+
+def gpu_op(op, x_np, angles, opts, dev, is2d=False):
+    C = _native()
+    plan = C.Plan([float(v) for v in opts], torch.from_numpy(np.asarray(angles, np.float32)), dev.index or 0, is2d)
+    x = torch.from_numpy(np.ascontiguousarray(x_np, np.float32)).to(dev)
+    nb = x.shape[0]
+    x5 = x.view(nb, 1, *x.shape[1:])
+    out = plan.execute(OPID[op], x5, torch.cuda.current_stream(dev).cuda_stream)
+    torch.cuda.synchronize()
+    return out.cpu().numpy().reshape(nb, *out.shape[2:])
+
+
+@pytest.mark.parametrize("geo", sorted(CASES3D))
+@pytest.mark.parametrize("op", OPS)
+def test_small_vs_oracle(cuda, geo, op):
+    opts = CASES3D[geo]
+    V = 40
+    ang = angles_with_ties(V)
+    rng = np.random.default_rng(7)
+    vol, sino = shapes3d(opts, 2, V)
+    x = rng.random(vol if op in VOLUME_IN else sino).astype(np.float32)
+    print(f"[gpu] small {geo} {op} shape={x.shape}", flush=True)
+    got = gpu_op(op, x, ang, opts, cuda)
+    want = oracle.op3d(op, x.astype(np.float64), ang, opts)
+    err = rel_l2(got, want)
+    assert err <= 1e-5, f"{geo} {op}: relL2 vs FP64 oracle {err:.3e}"
+    if emu_available():
+        assert rel_l2(got, emu_op3d(op, x, ang, opts)) <= 1e-5
+
+
+@pytest.mark.parametrize("geo", ["flat", "arc"])
+@pytest.mark.parametrize("op", ["A", "B", "BW", "AT", "BT"])
+def test_small_vs_reference_cpu(cuda, geo, op):
+    if not oracle.ref_available():
+        pytest.skip("reference CPU library not built")
+    opts = CASES3D[geo]
+    V = 24
+    ang = uniform_angles(V)
+    rng = np.random.default_rng(8)
+    vol, sino = shapes3d(opts, 1, V)
+    x = rng.random(vol if op in VOLUME_IN else sino).astype(np.float32)
+    got = gpu_op(op, x, ang, opts, cuda)
+    ref = oracle.ref_op3d(op, x, ang, opts)
+    tol = 2e-5 if geo == "flat" else 2e-4
+    assert rel_l2(got, ref) <= tol
+
+
+@pytest.mark.parametrize("tile_rows,voxel_chunk", [(1, 1), (2, 4), (4, 8), (8, 16)])
+def test_tilings_agree(cuda, tile_rows, voxel_chunk):
+    C = _native()
+    opts = CASES3D["flat"]
+    V = 20
+    ang = angles_with_ties(V)
+    plan = C.Plan([float(v) for v in opts], torch.from_numpy(ang), 0, False)
+    rng = np.random.default_rng(3)
+    vol, sino = shapes3d(opts, 1, V)
+    x = torch.from_numpy(rng.random(vol).astype(np.float32)).to(cuda).unsqueeze(1)
+    y = torch.from_numpy(rng.random(sino).astype(np.float32)).to(cuda).unsqueeze(1)
+    s = torch.cuda.current_stream().cuda_stream
+    outs = {}
+    for op, inp in ((0, x), (2, y)):
+        out = torch.empty(plan.execute(op, inp, s).shape, device=cuda)
+        ws = torch.empty(plan.workspace_bytes(op, 1), dtype=torch.uint8, device=cuda)
+        plan.execute_into(op, inp, out, ws, s, tile_rows, voxel_chunk, 0)
+        outs[op] = out.cpu().numpy()[:, 0]
+    want_a = oracle.op3d("A", x.cpu().numpy()[:, 0].astype(np.float64), ang, opts)
+    want_b = oracle.op3d("B", y.cpu().numpy()[:, 0].astype(np.float64), ang, opts)
+    assert rel_l2(outs[0], want_a) <= 1e-5
+    assert rel_l2(outs[2], want_b) <= 1e-5
+
+
+# ------------------------------------------------------------------ full size (cfg 2)
+
+CFG2 = [256, 256, 64, 512, 64, 1.0, 1.0, 1.0, 1.0, 1.0, 500.0, 1000.0, 0, 0, 0, 0, 0, 1]
+
+
+@pytest.fixture(scope="module")
+def cfg2(cuda):
+    from xrecon_amd.phantom import random_ellipsoids
+
+    V = 720
+    ang = uniform_angles(V)
+    ph = random_ellipsoids((256, 256, 64), (1.0, 1.0, 1.0), count=16, semi_axes=(5.0, 80.0), seed=1002)
+    C = _native()
+    plan = C.Plan([float(v) for v in CFG2], torch.from_numpy(ang), 0, False)
+    x = torch.from_numpy(ph).to(cuda).view(1, 1, 64, 256, 256)
+    return plan, ang, ph, x
+
+
+def test_cfg2_projection_sampled_views(cfg2, cuda):
+    plan, ang, ph, x = cfg2
+    print("[gpu] cfg2 A sampled views", flush=True)
+    y = plan.execute(0, x, torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    views = np.array([0, 1, 90, 179, 180, 270, 359, 450, 541, 719])  # includes the 45/135/225/315-degree ties
+    want = oracle.op3d("A", ph[None].astype(np.float64), np.ascontiguousarray(ang[views]), CFG2)
+    got = y.cpu().numpy()[0, 0][views]
+    assert rel_l2(got, want[0]) <= 1e-5
+
+
+def test_cfg2_backprojection_subset(cfg2, cuda):
+    plan, ang, ph, x = cfg2
+    views = np.array([0, 45, 90, 135, 180, 225, 270, 315, 360, 600])
+    sub = np.ascontiguousarray(ang[views])
+    C = _native()
+    psub = C.Plan([float(v) for v in CFG2], torch.from_numpy(sub), 0, False)
+    rng = np.random.default_rng(11)
+    y = rng.random((1, len(views), 64, 512)).astype(np.float32)
+    s = torch.cuda.current_stream().cuda_stream
+    for op in ("B", "BW"):
+        print(f"[gpu] cfg2 {op} on {len(views)} views", flush=True)
+        got = psub.execute(OPID[op], torch.from_numpy(y).to(cuda).unsqueeze(1), s).cpu().numpy()[:, 0]
+        want = oracle.op3d(op, y.astype(np.float64), sub, CFG2)
+        assert rel_l2(got, want) <= 1e-5, op
+
+
+@pytest.mark.parametrize("pair", [("A", "AT"), ("B", "BT"), ("BW", "BWT")])
+def test_cfg2_adjointness(cfg2, cuda, pair):
+    """<A x, y> = <x, A^T y> at full size (dot products in FP64 of FP32 results)."""
+    plan, ang, ph, x = cfg2
+    fwd, adj = pair
+    s = torch.cuda.current_stream().cuda_stream
+    g = torch.Generator(device=cuda).manual_seed(5)
+    vol = torch.rand((1, 1, 64, 256, 256), device=cuda, generator=g)
+    sino = torch.rand((1, 1, 720, 64, 512), device=cuda, generator=g)
+    if fwd == "A":
+        ax = plan.execute(OPID[fwd], vol, s)
+        aty = plan.execute(OPID[adj], sino, s)
+        lhs = (ax.double() * sino.double()).sum().item()
+        rhs = (vol.double() * aty.double()).sum().item()
+    else:  # B maps sinogram -> volume
+        by = plan.execute(OPID[fwd], sino, s)
+        btx = plan.execute(OPID[adj], vol, s)
+        lhs = (by.double() * vol.double()).sum().item()
+        rhs = (sino.double() * btx.double()).sum().item()
+    assert abs(lhs - rhs) / abs(rhs) <= 1e-5, (lhs, rhs)
+
+
+def test_cfg2_deterministic_and_linear(cfg2, cuda):
+    plan, ang, ph, x = cfg2
+    s = torch.cuda.current_stream().cuda_stream
+    y1 = plan.execute(0, x, s)
+    y2 = plan.execute(0, x, s)
+    assert torch.equal(y1, y2), "projection is not bitwise reproducible"
+    b1 = plan.execute(2, y1, s)
+    b2 = plan.execute(2, y1, s)
+    assert torch.equal(b1, b2), "backprojection is not bitwise reproducible"
+    z = torch.rand_like(x)
+    lin = plan.execute(0, 2.0 * x + z, s)
+    ref = 2.0 * y1 + plan.execute(0, z, s)
+    err = ((lin - ref).double().norm() / ref.double().norm()).item()
+    assert err <= 1e-6
+
+
+def test_batch_independence(cuda):
+    C = _native()
+    opts = CASES3D["flat"]
+    V = 16
+    ang = uniform_angles(V)
+    plan = C.Plan([float(v) for v in opts], torch.from_numpy(ang), 0, False)
+    vol, _ = shapes3d(opts, 3, V)
+    x = torch.rand(vol, device=cuda).view(3, 1, *vol[1:])
+    s = torch.cuda.current_stream().cuda_stream
+    yb = plan.execute(0, x, s)
+    for i in range(3):
+        yi = plan.execute(0, x[i:i + 1].contiguous(), s)
+        assert torch.equal(yb[i:i + 1], yi)
+    # B = 2, C = 2 folds into four instances
+    x4 = torch.rand((2, 2) + tuple(vol[1:]), device=cuda)
+    y4 = plan.execute(0, x4, s)
+    assert y4.shape == (2, 2, V, int(opts[4]), int(opts[3]))
+    assert torch.equal(y4[1, 0], plan.execute(0, x4[1:2, 0:1].contiguous(), s)[0, 0])

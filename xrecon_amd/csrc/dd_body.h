@@ -1,0 +1,366 @@
+// Per-thread bodies of the distance-driven kernels, written as __host__ __device__
+// functions of the thread's coordinates so that the exact same code runs in the gfx950
+// kernels (dd_kernels.hip) and, one "thread" at a time, in the CPU emulator that the
+// test suite uses to check kernel logic against the oracle without a GPU.
+//
+// See dd_kernels.hip for the design; the formulas cite the reference's files.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+
+#include "xr_internal.h"
+
+namespace xr {
+
+enum CellOp : int { CELL_A = 0, CELL_BT = 1, CELL_BWT = 2 };
+enum VoxOp : int { VOX_B = 0, VOX_BW = 1, VOX_AT = 2 };
+enum SinoMode : int { SINO_PLAIN = 0, SINO_AT = 1 };
+
+constexpr int kWave = 64;
+constexpr int kBlock = 256;
+constexpr int kWavesPerBlock = kBlock / kWave;
+
+struct CellArgs {
+    int nx, ny, nz, nu, nv, nview, maxline, nbatch;
+    float v0, dvv, zoff, sod;
+    float dx, dy, shx, shy;
+};
+
+struct VoxArgs {
+    int nx, ny, nz, nu, nv, nview, maxline, nbatch, nchunk;
+    float zb, dz, rv0, sod;
+    float dx, dy, shx, shy;
+};
+
+struct SinoArgs {
+    int nu, nv, nview, nbatch;
+    float v0, dvv;
+};
+
+// Operator -> kernel route.  Parallel-beam 3D backprojection is the transpose of the
+// projection and its weighted form equals the plain one (src/cpu/backward3d.cpp:101-135,
+// src/kernels/forward3d_parallel.cu:201-219); 2D parallel has its own backprojection
+// (src/cpu/backward2d.cpp:165-205) whose weighted form is again the plain one.
+struct Route {
+    bool cell;  // cell gather (writes sinogram) vs voxel gather (writes volume)
+    int kop;    // CellOp or VoxOp, -1 for an unknown operator
+};
+
+inline Route route(const Plan3d &p, int op)
+{
+    const bool par3d = p.kind == GEO_PAR && !p.is2d;
+    switch (op) {
+    case XR_OP_PROJECTION: return {true, CELL_A};
+    case XR_OP_PROJECTION_T: return {false, VOX_AT};
+    case XR_OP_BACKPROJECTION: return par3d ? Route{false, VOX_AT} : Route{false, VOX_B};
+    case XR_OP_BACKPROJECTION_W:
+        if (par3d) return {false, VOX_AT};
+        return p.kind == GEO_PAR ? Route{false, VOX_B} : Route{false, VOX_BW};
+    case XR_OP_BACKPROJECTION_T: return par3d ? Route{true, CELL_A} : Route{true, CELL_BT};
+    case XR_OP_BACKPROJECTION_W_T:
+        if (par3d) return {true, CELL_A};
+        return p.kind == GEO_PAR ? Route{true, CELL_BT} : Route{true, CELL_BWT};
+    }
+    return {true, -1};
+}
+
+inline CellArgs make_cell_args(const Plan3d &p, int nbatch)
+{
+    CellArgs a;
+    a.nx = p.vol.nx; a.ny = p.vol.ny; a.nz = p.vol.nz; a.nu = p.geo.nu; a.nv = p.geo.nv;
+    a.nview = p.nview; a.maxline = p.maxline; a.nbatch = nbatch;
+    a.v0 = p.v0; a.dvv = p.dvv; a.zoff = p.zoff; a.sod = p.sod;
+    a.dx = p.vol.dx; a.dy = p.vol.dy; a.shx = p.vol.shift_x; a.shy = p.vol.shift_y;
+    return a;
+}
+
+inline VoxArgs make_vox_args(const Plan3d &p, int nbatch, int KC)
+{
+    VoxArgs a;
+    a.nx = p.vol.nx; a.ny = p.vol.ny; a.nz = p.vol.nz; a.nu = p.geo.nu; a.nv = p.geo.nv;
+    a.nview = p.nview; a.maxline = p.maxline; a.nbatch = nbatch;
+    a.nchunk = (a.nz + KC - 1) / KC;
+    a.zb = p.zb; a.dz = p.dz; a.rv0 = p.rv0; a.sod = p.sod;
+    a.dx = p.vol.dx; a.dy = p.vol.dy; a.shx = p.vol.shift_x; a.shy = p.vol.shift_y;
+    return a;
+}
+
+inline SinoArgs make_sino_args(const Plan3d &p, int nbatch)
+{
+    SinoArgs a;
+    a.nu = p.geo.nu; a.nv = p.geo.nv; a.nview = p.nview; a.nbatch = nbatch;
+    a.v0 = p.v0; a.dvv = p.dvv;
+    return a;
+}
+
+// FDK distance weight U of a voxel centre p for a view with source s
+// (include/xrmath.h:76-86): flat SOD^2/(SOD - s.p/SOD)^2, arc 1/|s - p|^2.
+template <int GEO>
+__host__ __device__ inline float fdk_weight(float sx, float sy, float px, float py, float sod)
+{
+    if (GEO == GEO_FLAT) {
+        const float dd = (sx * px + sy * py) / sod;
+        const float den = sod - dd;
+        return (sod * sod) / (den * den);
+    } else if (GEO == GEO_ARC) {
+        const float ex = sx - px, ey = sy - py;
+        return 1.0f / (ex * ex + ey * ey);
+    }
+    return 1.0f;
+}
+
+// ---------------------------------------------------------------- prefix tables
+
+// z-prefix of one volume column (b, e = j*nx + i) in the x-driven line orientation:
+// W[b][k][j][i] = (J_k, f_k), J_k = sum_{k'<k} f[k'] - total/2 (FP64 accumulation).
+__host__ __device__ inline void vol_prefix_body(const float *__restrict__ f, float2 *__restrict__ W, int64_t plane,
+                                                int nz, int64_t b, int64_t e)
+{
+    const float *src = f + b * plane * nz + e;
+    float2 *dst = W + b * plane * (nz + 1) + e;
+    double total = 0.0;
+    for (int k = 0; k < nz; ++k) total += (double)src[(int64_t)k * plane];
+    double run = -0.5 * total;
+    for (int k = 0; k < nz; ++k) {
+        const float v = src[(int64_t)k * plane];
+        dst[(int64_t)k * plane] = make_float2((float)run, v);
+        run += (double)v;
+    }
+    dst[(int64_t)nz * plane] = make_float2((float)run, 0.0f);
+}
+
+// Forward-projection cell normalisation applied to a sinogram sample before the A^T
+// gather: Delta / (W_c cos theta_cr) (src/cpu/forward3d.cpp:330-344; the row height
+// cancels against the depth overlap, which the voxel gather measures in row units).
+template <int GEO>
+__host__ __device__ inline float at_scale(const ViewHdr &h, float scale0, float dbp, float sg, float v0, float dvv,
+                                          int r)
+{
+    if (GEO == GEO_PAR) return scale0;
+    const float hb = sg * (v0 + ((float)r + 0.5f) * dvv);
+    const float cosw = fabsf(h.S) / sqrtf(dbp * dbp + hb * hb + h.S * h.S);
+    return scale0 / cosw;
+}
+
+// Row-prefix of one sinogram column (b, v, internal column c), in internal column
+// order: Q[b][v][r][c] = (Q_r, q_r), q_r = p[b][v][r][col(c)] * s(c, r),
+// Q_r = sum_{r'<r} q - total/2, r = 0..Nv.
+template <int MODE, int GEO>
+__host__ __device__ inline void sino_prefix_body(const ViewHdr *__restrict__ hdr, const float4 *__restrict__ colt,
+                                                 const float *__restrict__ p, float2 *__restrict__ Q,
+                                                 const SinoArgs &a, int b, int v, int c)
+{
+    const ViewHdr h = hdr[v];
+    const int col = h.keep ? c : (a.nu - 1 - c);
+    const int64_t sv = (int64_t)b * a.nview + v;
+    const float *src = p + sv * a.nv * a.nu + col;
+    float2 *dst = Q + sv * (int64_t)(a.nv + 1) * a.nu + c;
+    float scale0 = 1.0f, dbp = 0.0f, sg = 1.0f;
+    if (MODE == SINO_AT) {
+        const float4 cc = colt[(int64_t)v * (a.nu + 1) + c];
+        const float dn = colt[(int64_t)v * (a.nu + 1) + c + 1].x;
+        scale0 = h.delta / cc.w;
+        dbp = 0.5f * (cc.x + dn) - h.P;
+        sg = cc.y;
+    }
+    double total = 0.0;
+    for (int r = 0; r < a.nv; ++r) {
+        float q = src[(int64_t)r * a.nu];
+        if (MODE == SINO_AT) q *= at_scale<GEO>(h, scale0, dbp, sg, a.v0, a.dvv, r);
+        total += (double)q;
+    }
+    double run = -0.5 * total;
+    for (int r = 0; r < a.nv; ++r) {
+        float q = src[(int64_t)r * a.nu];
+        if (MODE == SINO_AT) q *= at_scale<GEO>(h, scale0, dbp, sg, a.v0, a.dvv, r);
+        dst[(int64_t)r * a.nu] = make_float2((float)run, q);
+        run += (double)q;
+    }
+    dst[(int64_t)a.nv * a.nu] = make_float2((float)run, 0.0f);
+}
+
+// ---------------------------------------------------------------- cell gather
+
+// One thread: internal column c, rows r0..r0+R-1 of view v, batch b.
+template <int OPK, int GEO, int R>
+__host__ __device__ inline void cell_body(const ViewHdr *__restrict__ hdr, const float4 *__restrict__ lineC,
+                                          const float4 *__restrict__ colt, const float2 *__restrict__ WX,
+                                          const float2 *__restrict__ WY, float *__restrict__ out, const CellArgs &a,
+                                          int c, int r0, int v, int b)
+{
+    const ViewHdr h = hdr[v];
+    const float4 *ct = colt + (int64_t)v * (a.nu + 1);
+    const float4 cc = ct[c];
+    const float d0 = cc.x, d1 = ct[c + 1].x;
+    float hv[R + 1];
+    for (int q = 0; q <= R; ++q) hv[q] = cc.y * (a.v0 + (float)(r0 + q) * a.dvv);
+
+    const int nline = h.nline, nalong = h.nalong;
+    const int plane = nline * nalong;
+    const float2 *W = (h.xdrv ? WX : WY) + (int64_t)b * (int64_t)(a.nz + 1) * plane;
+    const float4 *lt = lineC + (int64_t)v * a.maxline;
+    const float fnz = (float)a.nz, fal = (float)nalong;
+
+    float acc[R];
+    for (int q = 0; q < R; ++q) acc[q] = 0.0f;
+
+    for (int L = 0; L < nline; ++L) {
+        const float4 lc = lt[L];  // {toff, invd, iz, lam}
+        // Column boundaries in pixel units along the line.
+        const float t0 = fmaf(d0, lc.y, lc.x);
+        const float t1 = fmaf(d1, lc.y, lc.x);
+        if (!(t1 > 0.0f && t0 < fal)) continue;
+        // Row boundaries in voxel units along z at this line's magnification.
+        int koff[R + 1];
+        float fr[R + 1];
+        for (int q = 0; q <= R; ++q) {
+            float z = fmaf(hv[q], lc.z, a.zoff);
+            z = fminf(fmaxf(z, 0.0f), fnz);
+            const int k = (int)z;
+            fr[q] = z - (float)k;
+            koff[q] = k * plane;
+        }
+        const int m0 = t0 > 0.0f ? (int)floorf(t0) : 0;
+        const int m1 = t1 < fal ? (int)ceilf(t1) : nalong;
+        const float2 *Wl = W + L * nalong;
+        for (int m = m0; m < m1; ++m) {
+            const float ou = fminf(t1, (float)(m + 1)) - fmaxf(t0, (float)m);
+            float w;
+            if (OPK == CELL_A) {
+                w = ou * lc.w;
+            } else if (OPK == CELL_BWT) {
+                float px, py;
+                if (h.xdrv) {
+                    px = ((float)m - 0.5f * a.nx + 0.5f) * a.dx + a.shx;
+                    py = (0.5f * a.ny - (float)L - 0.5f) * a.dy + a.shy;
+                } else {
+                    px = ((float)L - 0.5f * a.nx + 0.5f) * a.dx + a.shx;
+                    py = ((float)m - 0.5f * a.ny + 0.5f) * a.dy + a.shy;
+                }
+                w = ou * fdk_weight<GEO>(h.sx, h.sy, px, py, a.sod);
+            } else {
+                w = ou;
+            }
+            float J[R + 1];
+            for (int q = 0; q <= R; ++q) {
+                const float2 e = Wl[koff[q] + m];
+                J[q] = fmaf(fr[q], e.y, e.x);
+            }
+            for (int q = 0; q < R; ++q) acc[q] = fmaf(w, J[q + 1] - J[q], acc[q]);
+        }
+    }
+
+    const int col = h.keep ? c : (a.nu - 1 - c);
+    float *o = out + ((int64_t)b * a.nview + v) * a.nv * a.nu + col;
+    for (int q = 0; q < R; ++q) {
+        const int r = r0 + q;
+        if (r >= a.nv) break;
+        float s;
+        if (OPK == CELL_A) {
+            // Delta / (W_c H_cr cos theta_cr) with H = 1/kappa (src/cpu/forward3d.cpp:330-344)
+            s = h.delta * cc.z / cc.w;
+            if (GEO != GEO_PAR) {
+                const float dbp = 0.5f * (d0 + d1) - h.P;
+                const float hb = cc.y * (a.v0 + ((float)r + 0.5f) * a.dvv);
+                const float cosw = fabsf(h.S) / sqrtf(dbp * dbp + hb * hb + h.S * h.S);
+                s /= cosw;
+            }
+        } else {
+            s = h.colw;
+        }
+        o[(int64_t)r * a.nu] = acc[q] * s;
+    }
+}
+
+// ---------------------------------------------------------------- voxel gather
+
+__host__ __device__ inline float row_integral(const float2 *__restrict__ Qc, int nu, float fnv, float rho)
+{
+    rho = fminf(fmaxf(rho, 0.0f), fnv);
+    const int r = (int)rho;
+    const float2 e = Qc[(int64_t)r * nu];
+    return fmaf(rho - (float)r, e.y, e.x);
+}
+
+// One thread: voxel column (i, j), slices k0..k0+KC-1, batch b.
+template <int OPK, int GEO, int KC>
+__host__ __device__ inline void voxel_body(const ViewHdr *__restrict__ hdr, const float4 *__restrict__ lineV,
+                                           const float4 *__restrict__ colt, const float2 *__restrict__ Q,
+                                           float *__restrict__ out, const VoxArgs &a, int i, int j, int k0, int b)
+{
+    const float px = ((float)i - 0.5f * a.nx + 0.5f) * a.dx + a.shx;
+    const float py = (0.5f * a.ny - (float)j - 0.5f) * a.dy + a.shy;
+    const float fnv = (float)a.nv;
+    const int nu = a.nu;
+
+    float acc[KC];
+    for (int q = 0; q < KC; ++q) acc[q] = 0.0f;
+
+    for (int v = 0; v < a.nview; ++v) {
+        const ViewHdr h = hdr[v];
+        int L, m;
+        if (h.xdrv) { L = j; m = i; } else { L = i; m = a.ny - 1 - j; }
+        const float4 lv = lineV[(int64_t)v * a.maxline + L];  // {P0, Delta, zeta}
+        const float a0 = fmaf((float)m, lv.y, lv.x);
+        const float a1 = a0 + lv.y;
+        if (!(a1 > h.dlo && a0 < h.dhi)) continue;
+
+        float hB = 1.0f;
+        if (OPK != VOX_AT) {
+            // U * w / A_vox with A_vox = Delta * zeta * dz (include/cpu3d_common.h:80-85)
+            const float U = (OPK == VOX_BW) ? fdk_weight<GEO>(h.sx, h.sy, px, py, a.sod) : 1.0f;
+            hB = U * h.colw / (lv.y * lv.z * a.dz);
+        }
+
+        // First column whose right boundary exceeds a0: closed-form inverse, then
+        // corrected against the table so the walk is exact w.r.t. the FP32 boundaries.
+        const float T = (h.tp1 * a0 + h.tp0) / (h.tq1 * a0 + h.tq0);
+        const float tc = h.tk0 + h.tk1 * (GEO == GEO_ARC ? atanf(T) : T);
+        int c = (int)floorf(fminf(fmaxf(tc, 0.0f), (float)(nu - 1)));
+        const float4 *ct = colt + (int64_t)v * (nu + 1);
+        float dc = ct[c].x;
+        while (c > 0 && dc > a0) { --c; dc = ct[c].x; }
+        float dn = ct[c + 1].x;
+        while (c < nu - 1 && dn <= a0) { ++c; dc = dn; dn = ct[c + 1].x; }
+
+        const float zbz = lv.z * a.zb, zsz = lv.z * a.dz;
+        const float2 *Qv = Q + ((int64_t)b * a.nview + v) * (int64_t)(a.nv + 1) * nu;
+        while (c < nu && dc < a1) {
+            const float4 cc = ct[c];
+            dn = ct[c + 1].x;
+            const float ou = fminf(a1, dn) - fmaxf(a0, dc);
+            if (ou > 0.0f) {
+                // Row coordinate of slice boundary k: rho_k = (zeta z_k / sigma - v0) / dvv.
+                const float kap = cc.z;
+                const float step = zsz * kap;
+                const float base = fmaf(zbz, kap, -a.rv0);
+                const float rlo = fmaf((float)k0, step, base);
+                const float rhi = fmaf((float)(k0 + KC), step, base);
+                if (rhi > 0.0f && rlo < fnv) {
+                    const float w = (OPK == VOX_AT) ? ou : ou * hB / kap;
+                    const float2 *Qc = Qv + c;
+                    float Ip = row_integral(Qc, nu, fnv, rlo);
+                    for (int q = 0; q < KC; ++q) {
+                        const float In = row_integral(Qc, nu, fnv, fmaf((float)(k0 + q + 1), step, base));
+                        acc[q] = fmaf(w, In - Ip, acc[q]);
+                        Ip = In;
+                    }
+                }
+            }
+            ++c;
+            dc = dn;
+        }
+    }
+
+    float *o = out + (int64_t)b * a.nz * a.ny * a.nx + (int64_t)j * a.nx + i;
+    for (int q = 0; q < KC; ++q) {
+        const int k = k0 + q;
+        if (k >= a.nz) break;
+        o[(int64_t)k * a.ny * a.nx] = acc[q];
+    }
+}
+
+}  // namespace xr

@@ -1,0 +1,303 @@
+// Distance-driven projector kernels for gfx950 (MI355X).
+//
+// Two kernel families cover every operator of the reference's projector core
+// (src/kernels/forward3d_*.cu, backward3d_*.cu, forward2d*.cu, backward2d*.cu), and both
+// are gathers: every output element is written once by the thread that owns it, with no
+// atomics, so results are deterministic run to run.
+//
+//  * Cell gathers (A, B^T, B_W^T -> sinogram).  A thread owns one detector column and R
+//    consecutive detector rows of one view; a wave is 64 adjacent columns, so the
+//    volume reads of a wave are contiguous along the line.  The thread walks the
+//    volume's lines (image rows for x-driven views, image columns otherwise, read from a
+//    y/x-transposed copy so both orientations are coalesced); on each line the column's
+//    axis interval [d_c, d_c+1] covers a few pixels m, and each row's height interval
+//    covers a fraction of each pixel's z column.  That z overlap is read from a z-prefix
+//    table of the volume, J(q) = J[k] + (q - k) f[k], so one row costs one table lookup
+//    per boundary instead of a loop over the voxels it overlaps, and the R rows share
+//    the line's column overlaps and their common row boundaries.
+//
+//  * Voxel gathers (B, B_W, A^T -> volume).  A thread owns one voxel column (i, j) and a
+//    chunk of KC slices; a wave is 64 adjacent i.  For each view it finds the detector
+//    columns its footprint overlaps (closed-form inverse of the column map, corrected
+//    against the FP32 table), and for each such column reads the row overlaps of all KC
+//    voxels from a row-prefix table of the sinogram, I(rho) = Q[r] + (rho - r) p[r];
+//    consecutive voxels share their boundary lookups.
+//
+// The per-thread bodies live in dd_body.h.  Geometry comes from per-view FP32 tables
+// computed in FP64 on the host (plan.cpp); all overlap arithmetic is FP32, the prefix
+// sums are accumulated in FP64 and stored FP32.  The path is VALU-bound (no dense
+// contraction, so no MFMA).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+
+#include "dd_body.h"
+
+namespace xr {
+
+namespace {
+
+// Zero fill (a kernel rather than hipMemsetAsync so that it is safe inside captured graphs).
+__global__ __launch_bounds__(kBlock) void k_zero(float *__restrict__ p, int64_t n)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = 0.0f;
+}
+
+__global__ __launch_bounds__(kBlock) void k_vol_prefix(const float *__restrict__ f, float2 *__restrict__ W, int nx,
+                                                       int ny, int nz, int nbatch)
+{
+    const int64_t plane = (int64_t)nx * ny;
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= plane * nbatch) return;
+    const int64_t b = idx / plane;
+    vol_prefix_body(f, W, plane, nz, b, idx - b * plane);
+}
+
+// y-driven line orientation: WY[p][i][ny-1-j] = WX[p][j][i] for every plane p of every
+// batch item, an LDS-tiled transpose (32x32 tiles padded against bank conflicts).
+__global__ __launch_bounds__(kBlock) void k_plane_transpose(const float2 *__restrict__ WX, float2 *__restrict__ WY,
+                                                            int nx, int ny)
+{
+    __shared__ float2 tile[32][33];
+    const int64_t plane = (int64_t)nx * ny;
+    const float2 *src = WX + (int64_t)blockIdx.z * plane;
+    float2 *dst = WY + (int64_t)blockIdx.z * plane;
+    const int i0 = blockIdx.x * 32, j0 = blockIdx.y * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+    for (int r = ty; r < 32; r += 8) {
+        const int j = j0 + r, i = i0 + tx;
+        if (j < ny && i < nx) tile[r][tx] = src[(int64_t)j * nx + i];
+    }
+    __syncthreads();
+    for (int r = ty; r < 32; r += 8) {
+        const int i = i0 + r, j = j0 + tx;
+        if (i < nx && j < ny) dst[(int64_t)i * ny + (ny - 1 - j)] = tile[tx][r];
+    }
+}
+
+template <int MODE, int GEO>
+__global__ __launch_bounds__(kBlock) void k_sino_prefix(const ViewHdr *__restrict__ hdr, const float4 *__restrict__ colt,
+                                                        const float *__restrict__ p, float2 *__restrict__ Q, SinoArgs a)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= a.nu) return;
+    sino_prefix_body<MODE, GEO>(hdr, colt, p, Q, a, blockIdx.z, blockIdx.y, c);
+}
+
+template <int OPK, int GEO, int R>
+__global__ __launch_bounds__(kBlock) void k_cell(const ViewHdr *__restrict__ hdr, const float4 *__restrict__ lineC,
+                                                 const float4 *__restrict__ colt, const float2 *__restrict__ WX,
+                                                 const float2 *__restrict__ WY, float *__restrict__ out, CellArgs a)
+{
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = threadIdx.x / kWave;
+    const int c = blockIdx.x * kWave + lane;
+    const int r0 = (blockIdx.y * kWavesPerBlock + wave) * R;
+    const int v = blockIdx.z % a.nview;
+    const int b = blockIdx.z / a.nview;
+    if (c >= a.nu || r0 >= a.nv) return;
+    cell_body<OPK, GEO, R>(hdr, lineC, colt, WX, WY, out, a, c, r0, v, b);
+}
+
+template <int OPK, int GEO, int KC>
+__global__ __launch_bounds__(kBlock) void k_voxel(const ViewHdr *__restrict__ hdr, const float4 *__restrict__ lineV,
+                                                  const float4 *__restrict__ colt, const float2 *__restrict__ Q,
+                                                  float *__restrict__ out, VoxArgs a)
+{
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = threadIdx.x / kWave;
+    const int i = blockIdx.x * kWave + lane;
+    const int j = blockIdx.y * kWavesPerBlock + wave;
+    const int chunk = blockIdx.z % a.nchunk;
+    const int b = blockIdx.z / a.nchunk;
+    if (i >= a.nx || j >= a.ny) return;
+    voxel_body<OPK, GEO, KC>(hdr, lineV, colt, Q, out, a, i, j, chunk * KC, b);
+}
+
+inline int ceil_div(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }
+
+inline size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
+
+inline size_t vol_prefix_bytes(const Plan3d &p, int nbatch)
+{
+    return (size_t)nbatch * (p.vol.nz + 1) * (size_t)p.vol.nx * p.vol.ny * sizeof(float2);
+}
+
+inline size_t sino_prefix_bytes(const Plan3d &p, int nbatch)
+{
+    return (size_t)nbatch * p.nview * (size_t)(p.geo.nv + 1) * p.geo.nu * sizeof(float2);
+}
+
+int default_rows(const Plan3d &p)
+{
+    const int nv = p.geo.nv;
+    if (nv >= 32) return 8;
+    if (nv >= 16) return 4;
+    if (nv >= 4) return 2;
+    return 1;
+}
+
+int default_chunk(const Plan3d &p)
+{
+    const int nz = p.vol.nz;
+    if (nz >= 64) return 16;
+    if (nz >= 8) return 8;
+    if (nz >= 4) return 4;
+    return 1;
+}
+
+template <int OPK, int GEO>
+void launch_cell(const Plan3d &p, int R, const float2 *WX, const float2 *WY, float *out, int nbatch, hipStream_t s)
+{
+    const CellArgs a = make_cell_args(p, nbatch);
+    dim3 grid(ceil_div(a.nu, kWave), ceil_div(a.nv, R * kWavesPerBlock), nbatch * p.nview);
+    switch (R) {
+    case 1: hipLaunchKernelGGL((k_cell<OPK, GEO, 1>), grid, dim3(kBlock), 0, s, p.d_hdr, p.d_lineC, p.d_col, WX, WY, out, a); break;
+    case 2: hipLaunchKernelGGL((k_cell<OPK, GEO, 2>), grid, dim3(kBlock), 0, s, p.d_hdr, p.d_lineC, p.d_col, WX, WY, out, a); break;
+    case 4: hipLaunchKernelGGL((k_cell<OPK, GEO, 4>), grid, dim3(kBlock), 0, s, p.d_hdr, p.d_lineC, p.d_col, WX, WY, out, a); break;
+    default: hipLaunchKernelGGL((k_cell<OPK, GEO, 8>), grid, dim3(kBlock), 0, s, p.d_hdr, p.d_lineC, p.d_col, WX, WY, out, a); break;
+    }
+}
+
+template <int OPK, int GEO>
+void launch_voxel(const Plan3d &p, int KC, const float2 *Q, float *out, int nbatch, hipStream_t s)
+{
+    const VoxArgs a = make_vox_args(p, nbatch, KC);
+    dim3 grid(ceil_div(a.nx, kWave), ceil_div(a.ny, kWavesPerBlock), nbatch * a.nchunk);
+    switch (KC) {
+    case 1: hipLaunchKernelGGL((k_voxel<OPK, GEO, 1>), grid, dim3(kBlock), 0, s, p.d_hdr, p.d_lineV, p.d_col, Q, out, a); break;
+    case 4: hipLaunchKernelGGL((k_voxel<OPK, GEO, 4>), grid, dim3(kBlock), 0, s, p.d_hdr, p.d_lineV, p.d_col, Q, out, a); break;
+    case 8: hipLaunchKernelGGL((k_voxel<OPK, GEO, 8>), grid, dim3(kBlock), 0, s, p.d_hdr, p.d_lineV, p.d_col, Q, out, a); break;
+    default: hipLaunchKernelGGL((k_voxel<OPK, GEO, 16>), grid, dim3(kBlock), 0, s, p.d_hdr, p.d_lineV, p.d_col, Q, out, a); break;
+    }
+}
+
+template <int GEO>
+void launch_cell_geo(const Plan3d &p, int kop, int R, const float2 *WX, const float2 *WY, float *out, int nb,
+                     hipStream_t s)
+{
+    if (kop == CELL_A) launch_cell<CELL_A, GEO>(p, R, WX, WY, out, nb, s);
+    else if (kop == CELL_BT) launch_cell<CELL_BT, GEO>(p, R, WX, WY, out, nb, s);
+    else launch_cell<CELL_BWT, GEO>(p, R, WX, WY, out, nb, s);
+}
+
+template <int GEO>
+void launch_voxel_geo(const Plan3d &p, int kop, int KC, const float2 *Q, float *out, int nb, hipStream_t s)
+{
+    if (kop == VOX_B) launch_voxel<VOX_B, GEO>(p, KC, Q, out, nb, s);
+    else if (kop == VOX_BW) launch_voxel<VOX_BW, GEO>(p, KC, Q, out, nb, s);
+    else launch_voxel<VOX_AT, GEO>(p, KC, Q, out, nb, s);
+}
+
+template <int GEO>
+void launch_sino_prefix(const Plan3d &p, bool at, const float *in, float2 *Q, int nbatch, hipStream_t s)
+{
+    const SinoArgs a = make_sino_args(p, nbatch);
+    dim3 grid(ceil_div(a.nu, kBlock), p.nview, nbatch);
+    if (at) hipLaunchKernelGGL((k_sino_prefix<SINO_AT, GEO>), grid, dim3(kBlock), 0, s, p.d_hdr, p.d_col, in, Q, a);
+    else hipLaunchKernelGGL((k_sino_prefix<SINO_PLAIN, GEO>), grid, dim3(kBlock), 0, s, p.d_hdr, p.d_col, in, Q, a);
+}
+
+}  // namespace
+
+size_t workspace_bytes(const Plan3d &p, int op, int nbatch)
+{
+    if (nbatch <= 0) return 0;
+    const Route rt = route(p, op);
+    if (rt.kop < 0) return 0;
+    if (rt.cell) return align256(vol_prefix_bytes(p, nbatch)) * (p.any_y ? 2 : 1);
+    return align256(sino_prefix_bytes(p, nbatch));
+}
+
+int execute(const Plan3d &p, int op, const float *in, float *out, int nbatch, void *ws, size_t ws_bytes,
+            int tile_rows, int voxel_chunk, hipStream_t s)
+{
+    return execute_phase(p, op, 0, in, out, nbatch, ws, ws_bytes, tile_rows, voxel_chunk, s);
+}
+
+int execute_phase(const Plan3d &p, int op, int phase, const float *in, float *out, int nbatch, void *ws,
+                  size_t ws_bytes, int tile_rows, int voxel_chunk, hipStream_t s)
+{
+    if (phase < 0 || phase > 2) return fail(XR_EINVAL, "phase must be 0 (all), 1 (prefix) or 2 (gather)");
+    const bool do_prefix = phase != 2, do_gather = phase != 1;
+    const Route rt = route(p, op);
+    if (rt.kop < 0) return fail(XR_EINVAL, "unknown operator " + std::to_string(op));
+    if (nbatch < 0) return fail(XR_EINVAL, "negative batch count");
+    if (nbatch == 0) return XR_OK;
+    if ((do_prefix && !in) || (do_gather && !out)) return fail(XR_EINVAL, "null input or output pointer");
+    if (p.nview == 0) {
+        // No views: A and B^T produce an empty sinogram; B and A^T an all-zero volume.
+        if (!rt.cell && do_gather) {
+            const int64_t n = (int64_t)nbatch * p.vol.nz * p.vol.ny * p.vol.nx;
+            hipLaunchKernelGGL(k_zero, dim3(std::min<int64_t>(ceil_div(n, kBlock), 4096)), dim3(kBlock), 0, s, out, n);
+            if (hipGetLastError() != hipSuccess) return fail(XR_EHIP, "zero-fill launch failed");
+        }
+        return XR_OK;
+    }
+    const size_t need = workspace_bytes(p, op, nbatch);
+    if (ws_bytes < need || (need && !ws))
+        return fail(XR_EWORKSPACE, "workspace of " + std::to_string(ws_bytes) + " bytes, need " + std::to_string(need));
+    if (tile_rows == 0) tile_rows = default_rows(p);
+    if (voxel_chunk == 0) voxel_chunk = default_chunk(p);
+    if (tile_rows != 1 && tile_rows != 2 && tile_rows != 4 && tile_rows != 8)
+        return fail(XR_EINVAL, "tile_rows must be 1, 2, 4 or 8");
+    if (voxel_chunk != 1 && voxel_chunk != 4 && voxel_chunk != 8 && voxel_chunk != 16)
+        return fail(XR_EINVAL, "voxel_chunk must be 1, 4, 8 or 16");
+
+    if (rt.cell) {
+        float2 *WX = static_cast<float2 *>(ws);
+        float2 *WY = p.any_y ? reinterpret_cast<float2 *>(static_cast<char *>(ws) + align256(vol_prefix_bytes(p, nbatch)))
+                             : nullptr;
+        if (do_prefix) {
+            const int64_t cols = (int64_t)p.vol.nx * p.vol.ny * nbatch;
+            hipLaunchKernelGGL(k_vol_prefix, dim3(ceil_div(cols, kBlock)), dim3(kBlock), 0, s, in, WX, p.vol.nx,
+                               p.vol.ny, p.vol.nz, nbatch);
+            if (p.any_y) {
+                dim3 grid(ceil_div(p.vol.nx, 32), ceil_div(p.vol.ny, 32), nbatch * (p.vol.nz + 1));
+                hipLaunchKernelGGL(k_plane_transpose, grid, dim3(kBlock), 0, s, WX, WY, p.vol.nx, p.vol.ny);
+            }
+        }
+        if (do_gather) {
+            if (p.kind == GEO_FLAT) launch_cell_geo<GEO_FLAT>(p, rt.kop, tile_rows, WX, WY, out, nbatch, s);
+            else if (p.kind == GEO_ARC) launch_cell_geo<GEO_ARC>(p, rt.kop, tile_rows, WX, WY, out, nbatch, s);
+            else launch_cell_geo<GEO_PAR>(p, rt.kop, tile_rows, WX, WY, out, nbatch, s);
+        }
+    } else {
+        float2 *Q = static_cast<float2 *>(ws);
+        const bool at = rt.kop == VOX_AT;
+        if (do_prefix) {
+            if (p.kind == GEO_FLAT) launch_sino_prefix<GEO_FLAT>(p, at, in, Q, nbatch, s);
+            else if (p.kind == GEO_ARC) launch_sino_prefix<GEO_ARC>(p, at, in, Q, nbatch, s);
+            else launch_sino_prefix<GEO_PAR>(p, at, in, Q, nbatch, s);
+        }
+        if (do_gather) {
+            if (p.kind == GEO_FLAT) launch_voxel_geo<GEO_FLAT>(p, rt.kop, voxel_chunk, Q, out, nbatch, s);
+            else if (p.kind == GEO_ARC) launch_voxel_geo<GEO_ARC>(p, rt.kop, voxel_chunk, Q, out, nbatch, s);
+            else launch_voxel_geo<GEO_PAR>(p, rt.kop, voxel_chunk, Q, out, nbatch, s);
+        }
+    }
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(XR_EHIP, std::string("kernel launch failed: ") + hipGetErrorString(e));
+    return XR_OK;
+}
+
+size_t batch_workspace_bytes(const std::vector<BatchItem> &items, int op)
+{
+    size_t need = 0;
+    for (const BatchItem &it : items) need = std::max(need, workspace_bytes(*it.plan, op, it.nbatch));
+    return need;
+}
+
+int execute_batch(const std::vector<BatchItem> &items, int op, void *ws, size_t ws_bytes, int tile_rows,
+                  int voxel_chunk, hipStream_t s)
+{
+    for (const BatchItem &it : items) {
+        const int rc = execute(*it.plan, op, it.in, it.out, it.nbatch, ws, ws_bytes, tile_rows, voxel_chunk, s);
+        if (rc) return rc;
+    }
+    return XR_OK;
+}
+
+}  // namespace xr

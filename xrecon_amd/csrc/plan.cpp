@@ -1,0 +1,248 @@
+// Host-side plan builder: per-view geometry tables computed in FP64, stored as FP32.
+//
+// The reference recomputes this geometry per voxel (or per detector row block) per view
+// in FP32 on the device (include/xrmath.h:116-123, src/kernels/backward3d_flat.cu:30-61).
+// Here it is computed once per (geometry, angle set), in double precision, and the
+// kernels only read it.  The discrete decisions that select the operator (x-driven vs
+// y-driven and the column order) are made exactly as the reference makes them: FP32
+// sinf/cosf of the float32 angle, squared in FP32, compared with 0.5f
+// (include/cpu3d_common.h:65, :87; src/cpu/forward3d.cpp:309-310).
+#include <cmath>
+#include <cstring>
+#include <mutex>
+
+#include "xr_internal.h"
+
+namespace xr {
+
+namespace {
+thread_local std::string g_error;
+
+inline double axis_map(bool xdrv, double sx, double sy, double px, double py)
+{
+    // include/xrmath.h:8-22 (map2x / map2y), in double precision.
+    return xdrv ? (sx * py - sy * px) / (py - sy) : (sy * px - sx * py) / (px - sx);
+}
+}  // namespace
+
+void set_error(const std::string &msg) { g_error = msg; }
+const char *last_error() { return g_error.c_str(); }
+
+int fail(int code, const std::string &msg)
+{
+    set_error(msg);
+    return code;
+}
+
+int build_tables(Plan3d &p, const float *angles, PlanTables &t, std::string &err)
+{
+    const xr_volume3d &vol = p.vol;
+    const xr_geometry3d &geo = p.geo;
+    const int nx = vol.nx, ny = vol.ny, nz = vol.nz, nu = geo.nu, nv = geo.nv;
+    const int V = p.nview;
+    const double dx = vol.dx, dy = vol.dy, dz = vol.dz;
+    const double shx = vol.shift_x, shy = vol.shift_y, shz = vol.shift_z;
+    const double du = geo.du, dv = geo.dv, shu = geo.shift_u, shv = geo.shift_v;
+    const double sod = geo.sod, sdd = geo.sdd;
+    const bool cone = p.kind != GEO_PAR;
+    const bool is2d = p.is2d != 0;
+
+    // Virtual scale of the flat detector (include/xrmath.h:175-183): 3D flat detectors are
+    // given at their physical pitch and scaled to the iso plane; 2D pitches arrive
+    // already effective (xrecon/projector/geometry.py:106-109).
+    const double mag = (p.kind == GEO_FLAT && !is2d) ? sod / sdd : 1.0;
+    const double rowscale = (p.kind == GEO_FLAT && !is2d) ? sod / sdd : 1.0;
+
+    // Row boundaries v_r = v0 + r * dvv on the iso plane (src/cpu/forward3d.cpp:318-320).
+    // A 2D operator is a one-slice volume seen by a one-row detector whose row interval
+    // is centred and exactly covered: v = -1/2, +1/2 with sigma = zeta = 1, so that every
+    // depth overlap is 1 and the ray cosine sees hbar = 0 (src/cpu/system_matrix2d.cpp:
+    // 58-68 cos_weight_x has no height term).
+    double v0, dvv, zb, dzz;
+    if (is2d) {
+        v0 = -0.5; dvv = 1.0; zb = -0.5; dzz = 1.0;
+    } else {
+        v0 = ((0 - nv * 0.5) * dv + shv) * rowscale;
+        dvv = dv * rowscale;
+        zb = -nz * 0.5 * dz + shz;
+        dzz = dz;
+    }
+    p.v0 = (float)v0;
+    p.dvv = (float)dvv;
+    p.rv0 = (float)(v0 / dvv);
+    p.zb = (float)zb;
+    p.dz = (float)dzz;
+    p.zoff = (float)(-zb / dzz);
+    p.sod = (float)sod;
+
+    // Backprojection column width w (include/xrmath.h:188-198; 2D fan-flat uses du,
+    // src/cpu/backward2d.cpp:86; 2D arc 2*SOD*tan(du/2), :109; 2D parallel du, :187).
+    double colw;
+    if (p.kind == GEO_FLAT) colw = du * mag;
+    else if (p.kind == GEO_ARC) colw = 2.0 * sod * std::tan(0.5 * du);
+    else colw = du;
+
+    p.maxline = nx > ny ? nx : ny;
+    const int ML = p.maxline;
+    t.hdr.assign(V, ViewHdr{});
+    t.lineC.assign((size_t)V * ML, float4{0, 0, 0, 0});
+    t.lineV.assign((size_t)V * ML, float4{0, 0, 0, 0});
+    t.col.assign((size_t)V * (nu + 1), float4{0, 0, 0, 0});
+    p.xdrv.assign(V, 0);
+    p.keep.assign(V, 0);
+    p.any_x = p.any_y = 0;
+
+    std::vector<double> d(nu + 1), sig(nu);
+    for (int v = 0; v < V; ++v) {
+        const float af = angles[v];
+        if (!std::isfinite(af)) {
+            err = "angle " + std::to_string(v) + " is not finite";
+            return XR_EINVAL;
+        }
+        const float sf = std::sin(af), cf = std::cos(af);  // float overloads: sinf/cosf
+        const bool xdrv = cf * cf > 0.5f;
+        const bool keep = xdrv ? (cf >= 0.0f) : (sf >= 0.0f);
+        p.xdrv[v] = xdrv;
+        p.keep[v] = keep;
+        (xdrv ? p.any_x : p.any_y) = 1;
+        const double a = (double)af;
+        const double sb = std::sin(a), cb = std::cos(a);
+        const double sx = -sod * sb, sy = sod * cb;
+        const double sk = keep ? 1.0 : -1.0;
+
+        ViewHdr &h = t.hdr[v];
+        h.xdrv = xdrv;
+        h.keep = keep;
+        h.nline = xdrv ? ny : nx;
+        h.nalong = xdrv ? nx : ny;
+        h.delta = (float)(xdrv ? dy : dx);
+        h.colw = (float)colw;
+        h.sx = (float)sx;
+        h.sy = (float)sy;
+        h.P = (float)(xdrv ? sx : sy);
+        h.S = (float)(xdrv ? sy : sx);
+
+        // Lines: pixel edges on the axis are P0 + m * Delta along a line, voxel heights
+        // on the axis plane scale by zeta (include/cpu3d_common.h:68-78).
+        for (int L = 0; L < h.nline; ++L) {
+            double P0, Dl, zeta;
+            if (xdrv) {
+                const double py = (ny * 0.5 - L - 0.5) * dy + shy;
+                const double xl = -nx * 0.5 * dx + shx;
+                if (cone) {
+                    zeta = sy / (sy - py);
+                    P0 = axis_map(true, sx, sy, xl, py);
+                    Dl = dx * zeta;
+                } else {
+                    // src/cpu/forward3d.cpp:61-64: axis = tan(beta) * py + x
+                    zeta = 1.0;
+                    P0 = (sb / cb) * py + xl;
+                    Dl = dx;
+                }
+            } else {
+                const double px = (L - nx * 0.5 + 0.5) * dx + shx;
+                const double yb = -ny * 0.5 * dy + shy;
+                if (cone) {
+                    zeta = sx / (sx - px);
+                    P0 = axis_map(false, sx, sy, px, yb);
+                    Dl = dy * zeta;
+                } else {
+                    // src/cpu/forward3d.cpp:65-68: axis = cot(beta) * px + y
+                    zeta = 1.0;
+                    P0 = (cb / sb) * px + yb;
+                    Dl = dy;
+                }
+            }
+            if (is2d) zeta = 1.0;  // the one-slice embedding has no depth magnification
+            if (!(Dl > 0.0) || !(zeta > 0.0)) {
+                err = "view " + std::to_string(v) + ": source inside the volume or degenerate pixel footprint";
+                return XR_EINVAL;
+            }
+            const size_t li = (size_t)v * ML + L;
+            t.lineC[li] = float4{(float)(-P0 / Dl), (float)(1.0 / Dl), (float)(1.0 / (zeta * dzz)),
+                                 (float)(Dl * zeta * dzz)};
+            t.lineV[li] = float4{(float)P0, (float)Dl, (float)zeta, 0.0f};
+        }
+
+        // Columns: boundaries on the axis in internal order; per-column depth scale sigma
+        // from the column centre (include/cpu3d_common.h:94-131 flat, :150-182 arc;
+        // src/cpu/forward3d.cpp:37-58 parallel).
+        for (int c = 0; c <= nu; ++c) {
+            const double b = keep ? c : (nu - c);
+            if (p.kind == GEO_FLAT) {
+                const double u = ((b - nu * 0.5) * du + shu) * mag;
+                d[c] = axis_map(xdrv, sx, sy, u * cb, u * sb);
+            } else if (p.kind == GEO_ARC) {
+                const double g = (b - nu * 0.5) * du + shu + a;
+                d[c] = xdrv ? sx + sy * std::tan(g) : sy + sx / std::tan(g);
+            } else {
+                d[c] = ((b - nu * 0.5) * du + shu) / (xdrv ? cb : sb);
+            }
+        }
+        for (int c = 0; c < nu; ++c) {
+            if (!cone || is2d) {
+                sig[c] = 1.0;
+                continue;
+            }
+            const double bm = keep ? (c + 0.5) : (nu - c - 0.5);
+            double mx, my;
+            if (p.kind == GEO_FLAT) {
+                const double u = ((bm - nu * 0.5) * du + shu) * mag;
+                mx = u * cb;
+                my = u * sb;
+            } else {
+                const double g = (bm - nu * 0.5) * du + shu + a;
+                mx = std::sin(g) * sdd + sx;
+                my = -std::cos(g) * sdd + sy;
+            }
+            sig[c] = xdrv ? sy / (sy - my) : sx / (sx - mx);
+        }
+        for (int c = 0; c < nu; ++c) {
+            if (!(d[c + 1] > d[c]) || !(sig[c] > 0.0)) {
+                err = "view " + std::to_string(v) + ": detector column " + std::to_string(c) +
+                      " does not map to an increasing interval (detector behind the source?)";
+                return XR_EINVAL;
+            }
+        }
+        const size_t cb0 = (size_t)v * (nu + 1);
+        // W is the width that normalises the forward projection: the column's axis width
+        // for the cone geometries (src/cpu/forward3d.cpp:333-336), and the detector pitch
+        // for parallel beams, whose forward coefficient is ov * (dy or dx) / du
+        // (src/cpu/forward3d.cpp:58, :92).
+        for (int c = 0; c < nu; ++c)
+            t.col[cb0 + c] = float4{(float)d[c], (float)sig[c], (float)(1.0 / (sig[c] * dvv)),
+                                    (float)(cone ? d[c + 1] - d[c] : du)};
+        t.col[cb0 + nu] = float4{(float)d[nu], 0.0f, 0.0f, 0.0f};
+        h.dlo = (float)d[0];
+        h.dhi = (float)d[nu];
+
+        // Inverse map axis point -> fractional internal column, used by the voxel kernels
+        // to find a footprint's first column (then corrected against the table).
+        if (p.kind == GEO_FLAT) {
+            // Virtual-detector coordinate of the ray through the axis point a:
+            //   x-driven u = SOD a cos / (SOD + a sin), y-driven u = SOD a sin / (SOD - a cos);
+            // internal column = Nu/2 + sk * (u / mag - shu) / du.
+            if (xdrv) { h.tp1 = (float)(sod * cb); h.tp0 = 0; h.tq1 = (float)sb; h.tq0 = (float)sod; }
+            else      { h.tp1 = (float)(sod * sb); h.tp0 = 0; h.tq1 = (float)(-cb); h.tq0 = (float)sod; }
+            h.tk0 = (float)(nu * 0.5 - sk * shu / du);
+            h.tk1 = (float)(sk / (mag * du));
+        } else if (p.kind == GEO_ARC) {
+            // Fan angle of the ray relative to the column-centre direction beta + shu:
+            //   tan(gamma_rel) = T(a); internal column = Nu/2 + sk * atan(T) / du.
+            const double bp = a + shu, cp = std::cos(bp), sp = std::sin(bp);
+            const double s_sh = std::sin(shu), c_sh = std::cos(shu);
+            if (xdrv) { h.tp1 = (float)cp; h.tp0 = (float)(-sod * s_sh); h.tq1 = (float)sp; h.tq0 = (float)(sod * c_sh); }
+            else      { h.tp1 = (float)sp; h.tp0 = (float)(-sod * s_sh); h.tq1 = (float)(-cp); h.tq0 = (float)(sod * c_sh); }
+            h.tk0 = (float)(nu * 0.5);
+            h.tk1 = (float)(sk / du);
+        } else {
+            // d_c = ((b - Nu/2) du + shu) / cos  ->  c = Nu/2 + sk (a cos - shu) / du
+            h.tp1 = 1; h.tp0 = 0; h.tq1 = 0; h.tq0 = 1;
+            h.tk0 = (float)(nu * 0.5 - sk * shu / du);
+            h.tk1 = (float)(sk * (xdrv ? cb : sb) / du);
+        }
+    }
+    return XR_OK;
+}
+
+}  // namespace xr

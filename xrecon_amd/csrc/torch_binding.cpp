@@ -1,0 +1,256 @@
+// PyTorch-ROCm extension module xrecon_amd._C: the counterpart of the reference's
+// pybind11 module xrecon_C (src/api/torch_binding.cpp:615-638).  It validates tensors,
+// unpacks the options vector into host descriptors, allocates outputs and workspaces
+// through torch's caching allocator, and calls the C-ABI of libxrecon_amd.so on the
+// stream it is given.  It contains no compute of its own.
+//
+// Differences from the reference binding, each deliberate:
+//  * geometry is read on the host once per plan (options may live on the CPU), instead
+//    of 18 device->host .item() reads per call (torch_binding.cpp:85-108);
+//  * work goes to the caller's stream (the reference uses the legacy default stream,
+//    src/kernels/forward3d_flat.cu:506), so calls can be captured into a graph;
+//  * CPU tensors are rejected: this build computes on the GPU only (the reference falls
+//    back to its std::thread CPU kernels, torch_binding.cpp:421-425).
+#include <torch/extension.h>
+
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/xrecon_amd.h"
+
+namespace {
+
+[[noreturn]] void raise_xr(int rc, const char *what)
+{
+    TORCH_CHECK(false, what, " failed (code ", rc, "): ", xr_last_error());
+}
+
+std::vector<float> host_floats(const torch::Tensor &t, int64_t min_numel, const char *name)
+{
+    TORCH_CHECK(t.numel() >= min_numel, name, " tensor must have at least ", min_numel, " elements.");
+    auto c = t.detach().to(torch::kCPU, torch::kFloat32).contiguous();
+    return std::vector<float>(c.data_ptr<float>(), c.data_ptr<float>() + c.numel());
+}
+
+struct Desc {
+    xr_volume3d vol{};
+    xr_geometry3d geo{};
+    bool is2d = false;
+    int W = 0, H = 0, D = 0, Nu = 0, Nv = 0;
+};
+
+// 3D options layout: xrecon/projector/projector3d.py:98-107, unpacked as in
+// src/api/torch_binding.cpp:82-124.
+Desc unpack3d(const std::vector<float> &o)
+{
+    TORCH_CHECK(o.size() >= 18, "3D option tensor must have at least 18 elements.");
+    Desc d;
+    d.W = (int)o[0]; d.H = (int)o[1]; d.D = (int)o[2]; d.Nu = (int)o[3]; d.Nv = (int)o[4];
+    const int type = (int)o[17];
+    TORCH_CHECK(type == XR_PARALLEL || type == XR_FLAT || type == XR_ARC, "3D scan_type must be parallel, flat, or arc");
+    d.vol = xr_volume3d{d.W, d.H, d.D, o[5], o[6], o[7], o[12], o[13], o[14]};
+    d.geo = xr_geometry3d{type, d.Nu, d.Nv, o[8], o[9], o[15], o[16], o[10], o[11]};
+    return d;
+}
+
+// 2D options layout: xrecon/projector/projector.py:106-110, unpacked as in
+// src/api/torch_binding.cpp:62-80.
+Desc unpack2d(const std::vector<float> &o)
+{
+    TORCH_CHECK(o.size() >= 12, "Option tensor must have at least 12 elements.");
+    Desc d;
+    d.is2d = true;
+    d.W = (int)o[0]; d.H = (int)o[1]; d.D = 1; d.Nu = (int)o[2]; d.Nv = 1;
+    const int type = (int)o[11];
+    TORCH_CHECK(type == XR_PARALLEL || type == XR_FLAT || type == XR_ARC, "2D scan_type must be parallel, flat, or arc");
+    xr_volume2d v2{d.W, d.H, o[3], o[4], o[8], o[9]};
+    xr_geometry2d g2{type, d.Nu, o[5], o[10], o[6], o[7]};
+    const int rc = xr_geometry2d_as_3d(&v2, &g2, &d.vol, &d.geo);
+    if (rc) raise_xr(rc, "xr_geometry2d_as_3d");
+    return d;
+}
+
+bool volume_in(int op)
+{
+    return op == XR_OP_PROJECTION || op == XR_OP_BACKPROJECTION_T || op == XR_OP_BACKPROJECTION_W_T;
+}
+
+// A plan: host descriptors + the device geometry tables for one angle set.
+class Plan {
+public:
+    Plan(const std::vector<float> &options, const torch::Tensor &angles, int device, bool is2d)
+    {
+        desc_ = is2d ? unpack2d(options) : unpack3d(options);
+        TORCH_CHECK(angles.dim() == 1, "Expected angles shape [V], got ", angles.sizes());
+        auto a = angles.detach().to(torch::kCPU, torch::kFloat32).contiguous();
+        nview_ = (int)a.numel();
+        device_ = device;
+        const int rc = xr_plan3d_create(&desc_.vol, &desc_.geo, a.data_ptr<float>(), nview_, device, &plan_);
+        if (rc) raise_xr(rc, "xr_plan3d_create");
+    }
+    ~Plan()
+    {
+        if (plan_) xr_plan3d_destroy(plan_);
+    }
+    Plan(const Plan &) = delete;
+    Plan &operator=(const Plan &) = delete;
+
+    int num_view() const { return nview_; }
+    bool is2d() const { return desc_.is2d; }
+    int64_t handle() const { return reinterpret_cast<int64_t>(plan_); }
+
+    std::vector<int64_t> volume_shape(int64_t B, int64_t C) const
+    {
+        if (desc_.is2d) return {B, C, desc_.H, desc_.W};
+        return {B, C, desc_.D, desc_.H, desc_.W};
+    }
+    std::vector<int64_t> sino_shape(int64_t B, int64_t C) const
+    {
+        if (desc_.is2d) return {B, C, nview_, desc_.Nu};
+        return {B, C, nview_, desc_.Nv, desc_.Nu};
+    }
+
+    int64_t workspace_bytes(int op, int64_t nbatch) const
+    {
+        return (int64_t)xr_plan3d_workspace_bytes(plan_, op, (int)nbatch);
+    }
+
+    std::vector<int> view_flags() const
+    {
+        std::vector<int> x(nview_), k(nview_);
+        xr_plan3d_view_flags(plan_, x.data(), k.data());
+        std::vector<int> out;
+        for (int v = 0; v < nview_; ++v) out.push_back(x[v] | (k[v] << 1));
+        return out;
+    }
+
+    void check_input(int op, const torch::Tensor &x) const
+    {
+        TORCH_CHECK(x.is_cuda(), "xrecon_amd computes on the GPU: input must be a GPU tensor "
+                                 "(CPU tensors are not supported by this build)");
+        TORCH_CHECK(x.scalar_type() == torch::kFloat32, "input must be float32");
+        TORCH_CHECK(x.is_contiguous(), "Expected all tensors to be contiguous");
+        TORCH_CHECK(x.get_device() == device_, "input is on device ", x.get_device(), " but the plan on ", device_);
+        const int nd = desc_.is2d ? 4 : 5;
+        TORCH_CHECK(x.dim() == nd, desc_.is2d ? (volume_in(op) ? "Expected image shape [B, C, H, W]"
+                                                               : "Expected projection shape [B, C, V, Nu]")
+                                              : (volume_in(op) ? "Expected image shape [B, C, D, H, W]"
+                                                               : "Expected projection shape [B, C, V, Nv, Nu]"));
+        const auto want = volume_in(op) ? volume_shape(x.size(0), x.size(1)) : sino_shape(x.size(0), x.size(1));
+        TORCH_CHECK(x.sizes().vec() == want, volume_in(op) ? "Image shape does not match geometry: "
+                                                           : "Projection shape does not match geometry: ",
+                    x.sizes(), " vs expected ", want);
+    }
+
+    // out and workspace are caller-provided (graph capture / preallocated loops).
+    void execute_into(int op, const torch::Tensor &x, torch::Tensor &out, const torch::Tensor &ws, int64_t stream,
+                      int tile_rows, int voxel_chunk, int phase) const
+    {
+        check_input(op, x);
+        const int64_t nb = x.size(0) * x.size(1);
+        const auto want = volume_in(op) ? sino_shape(x.size(0), x.size(1)) : volume_shape(x.size(0), x.size(1));
+        TORCH_CHECK(out.is_cuda() && out.scalar_type() == torch::kFloat32 && out.is_contiguous() &&
+                        out.sizes().vec() == want && out.get_device() == device_,
+                    "output must be a contiguous float32 GPU tensor of shape ", want);
+        const size_t need = xr_plan3d_workspace_bytes(plan_, op, (int)nb);
+        void *wsp = nullptr;
+        size_t wsb = 0;
+        if (ws.defined() && ws.numel() > 0) {
+            TORCH_CHECK(ws.is_cuda() && ws.is_contiguous() && ws.get_device() == device_, "workspace must be a contiguous GPU tensor");
+            wsp = ws.data_ptr();
+            wsb = (size_t)ws.numel() * ws.element_size();
+        }
+        TORCH_CHECK(wsb >= need, "workspace of ", wsb, " bytes, need ", need);
+        const int rc = xr_plan3d_execute_phase(plan_, op, phase, x.data_ptr<float>(), out.data_ptr<float>(), (int)nb,
+                                               wsp, wsb, tile_rows, voxel_chunk, reinterpret_cast<void *>(stream));
+        if (rc) raise_xr(rc, "xr_plan3d_execute");
+    }
+
+    torch::Tensor execute(int op, const torch::Tensor &x, int64_t stream) const
+    {
+        check_input(op, x);
+        const int64_t B = x.size(0), C = x.size(1);
+        const auto shape = volume_in(op) ? sino_shape(B, C) : volume_shape(B, C);
+        auto out = torch::empty(shape, x.options());
+        const int64_t need = workspace_bytes(op, B * C);
+        auto ws = torch::empty({need > 0 ? need : 0}, x.options().dtype(torch::kUInt8));
+        execute_into(op, x, out, ws, stream, 0, 0, XR_PHASE_ALL);
+        return out;
+    }
+
+private:
+    Desc desc_;
+    int nview_ = 0;
+    int device_ = 0;
+    xr_plan3d *plan_ = nullptr;
+};
+
+// Reference-named operator functions: (input, options, angles[, stream]) -> output, as in
+// src/api/torch_binding.cpp:407-523 (3D) and :139-351 (2D).  They build a plan per call
+// (host geometry tables); xrecon_amd.Projector3d/Projector2d cache plans instead.
+torch::Tensor run_op(int op, bool is2d, const torch::Tensor &x, const torch::Tensor &options,
+                     const torch::Tensor &angles, int64_t stream)
+{
+    TORCH_CHECK(x.is_cuda(), "xrecon_amd computes on the GPU: input must be a GPU tensor "
+                             "(CPU tensors are not supported by this build)");
+    TORCH_CHECK(!angles.is_cuda() || angles.get_device() == x.get_device(),
+                "Expected all tensors on the same device");
+    TORCH_CHECK(options.scalar_type() == torch::kFloat32, "options must be float32");
+    TORCH_CHECK(angles.scalar_type() == torch::kFloat32, "ang must be float32");
+    TORCH_CHECK(angles.is_contiguous() && options.is_contiguous(), "Expected all tensors to be contiguous");
+    Plan plan(host_floats(options, is2d ? 12 : 18, "Option"), angles, x.get_device(), is2d);
+    return plan.execute(op, x, stream);
+}
+
+#define XR_DEF_OP(name, op, is2d)                                                                            \
+    m.def(                                                                                                   \
+        name,                                                                                                \
+        [](const torch::Tensor &x, const torch::Tensor &o, const torch::Tensor &a, int64_t stream) {         \
+            return run_op(op, is2d, x, o, a, stream);                                                        \
+        },                                                                                                   \
+        py::arg("input"), py::arg("options"), py::arg("angles"), py::arg("stream") = 0)
+
+}  // namespace
+
+PYBIND11_MODULE(TORCH_EXTENSION_NAME, m)
+{
+    m.doc() = "xrecon_amd native binding over libxrecon_amd.so (gfx950)";
+    py::class_<Plan, std::shared_ptr<Plan>>(m, "Plan")
+        .def(py::init([](const std::vector<float> &options, const torch::Tensor &angles, int device, bool is2d) {
+                 return std::make_shared<Plan>(options, angles, device, is2d);
+             }),
+             py::arg("options"), py::arg("angles"), py::arg("device"), py::arg("is2d") = false)
+        .def_property_readonly("num_view", &Plan::num_view)
+        .def_property_readonly("is2d", &Plan::is2d)
+        .def_property_readonly("handle", &Plan::handle)
+        .def("workspace_bytes", &Plan::workspace_bytes)
+        .def("view_flags", &Plan::view_flags)
+        .def("execute", &Plan::execute, py::arg("op"), py::arg("input"), py::arg("stream") = 0)
+        .def("execute_into", &Plan::execute_into, py::arg("op"), py::arg("input"), py::arg("out"),
+             py::arg("workspace"), py::arg("stream") = 0, py::arg("tile_rows") = 0, py::arg("voxel_chunk") = 0,
+             py::arg("phase") = XR_PHASE_ALL);
+
+    m.attr("OP_PROJECTION") = XR_OP_PROJECTION;
+    m.attr("OP_PROJECTION_T") = XR_OP_PROJECTION_T;
+    m.attr("OP_BACKPROJECTION") = XR_OP_BACKPROJECTION;
+    m.attr("OP_BACKPROJECTION_W") = XR_OP_BACKPROJECTION_W;
+    m.attr("OP_BACKPROJECTION_T") = XR_OP_BACKPROJECTION_T;
+    m.attr("OP_BACKPROJECTION_W_T") = XR_OP_BACKPROJECTION_W_T;
+    m.def("version", []() { return std::string(xr_version()); });
+
+    XR_DEF_OP("forward_projection2d", XR_OP_PROJECTION, true);
+    XR_DEF_OP("backward_projection2d", XR_OP_BACKPROJECTION, true);
+    XR_DEF_OP("weighted_backward_projection2d", XR_OP_BACKPROJECTION_W, true);
+    XR_DEF_OP("projection_transpose2d", XR_OP_PROJECTION_T, true);
+    XR_DEF_OP("backward_projection_transpose2d", XR_OP_BACKPROJECTION_T, true);
+    XR_DEF_OP("weighted_backward_projection_transpose2d", XR_OP_BACKPROJECTION_W_T, true);
+    XR_DEF_OP("back_projection2d", XR_OP_BACKPROJECTION, true);
+    XR_DEF_OP("weighted_back_projection2d", XR_OP_BACKPROJECTION_W, true);
+    XR_DEF_OP("forward_projection3d", XR_OP_PROJECTION, false);
+    XR_DEF_OP("projection_transpose3d", XR_OP_PROJECTION_T, false);
+    XR_DEF_OP("backward_projection3d", XR_OP_BACKPROJECTION, false);
+    XR_DEF_OP("weighted_backward_projection3d", XR_OP_BACKPROJECTION_W, false);
+    XR_DEF_OP("backward_projection_transpose3d", XR_OP_BACKPROJECTION_T, false);
+    XR_DEF_OP("weighted_backward_projection_transpose3d", XR_OP_BACKPROJECTION_W_T, false);
+}

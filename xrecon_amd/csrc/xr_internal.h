@@ -1,0 +1,98 @@
+// Internal definitions shared by the host plan builder (plan.cpp), the kernels
+// (dd_kernels.hip) and the C-ABI (capi.cpp).  Not part of the public interface.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/xrecon_amd.h"
+
+namespace xr {
+
+// Per-view header, one per view, FP32 values computed in FP64 on the host.
+//
+// Every geometry is reduced to the same picture: a driving axis (the x axis on y = 0
+// for x-driven views, the y axis on x = 0 otherwise, include/cpu3d_common.h:65-78), on
+// which the detector column boundaries d_c (c = 0..Nu, in increasing "internal" order,
+// include/cpu3d_common.h:87) and the voxel footprints are compared.  Voxels are
+// organised as "lines" (image rows for x-driven views, image columns otherwise) whose
+// pixel edges land on the axis as an arithmetic sequence P0 + m * Delta.
+struct alignas(16) ViewHdr {
+    int xdrv;    // 1: x-driven (cos^2 > 0.5 in FP32), 0: y-driven
+    int keep;    // 1: internal column c is physical column c; 0: physical Nu-1-c
+    int nline;   // Ny (x-driven) or Nx (y-driven)
+    int nalong;  // Nx (x-driven) or Ny (y-driven)
+    float P, S;      // cos(theta) = |S| / sqrt((dbar - P)^2 + hbar^2 + S^2)
+    float delta;     // forward-projection numerator: dy (x-driven) or dx (y-driven)
+    float colw;      // backprojection column width w
+    float tp1, tp0;  // T(a) = (tp1 * a + tp0) / (tq1 * a + tq0)
+    float tq1, tq0;
+    float tk0, tk1;  // fractional internal column of axis point a: tk0 + tk1 * g(T(a))
+    float sx, sy;    // source position (cone) for the FDK weights
+    float dlo, dhi;  // d_0 and d_Nu
+    float pad0, pad1;
+};
+static_assert(sizeof(ViewHdr) == 80, "ViewHdr layout");
+
+// Geometry family of a plan; selects the column inverse g() and the weights.
+enum GeoKind : int { GEO_PAR = 0, GEO_FLAT = 1, GEO_ARC = 2 };
+
+struct Plan3d {
+    xr_volume3d vol{};
+    xr_geometry3d geo{};
+    int kind = GEO_FLAT;     // geometry family
+    int is2d = 0;            // one-slice/one-row embedding of a 2D operator
+    int device = 0;
+    int nview = 0;
+    int maxline = 0;         // max(Nx, Ny)
+    int any_x = 0, any_y = 0;
+    // Scalars shared by all views.
+    float zb = 0, dz = 1;    // volume bottom z and slice thickness
+    float v0 = 0, dvv = 1;   // first row boundary and row pitch on the iso plane
+    float rv0 = 0;           // v0 / dvv
+    float zoff = 0;          // -zb / dz
+    float sod = 0;
+    // Host copies of the per-view flags.
+    std::vector<int> xdrv, keep;
+    // Device tables.
+    ViewHdr *d_hdr = nullptr;   // [nview]
+    float4 *d_lineC = nullptr;  // [nview][maxline] {toff, invd, iz, lam}: cell kernels
+    float4 *d_lineV = nullptr;  // [nview][maxline] {P0, Delta, zeta, 0}: voxel kernels
+    float4 *d_col = nullptr;    // [nview][Nu+1]    {d, sigma, kappa, W}
+};
+
+// Host-side tables before upload (kept separate so tests can inspect the builder).
+struct PlanTables {
+    std::vector<ViewHdr> hdr;
+    std::vector<float4> lineC, lineV, col;
+};
+
+// plan.cpp
+int build_tables(Plan3d &p, const float *angles_host, PlanTables &t, std::string &err);
+void set_error(const std::string &msg);
+int fail(int code, const std::string &msg);
+
+// dd_kernels.hip
+size_t workspace_bytes(const Plan3d &p, int op, int nbatch);
+int execute(const Plan3d &p, int op, const float *in, float *out, int nbatch, void *ws,
+            size_t ws_bytes, int tile_rows, int voxel_chunk, hipStream_t stream);
+// phase 0: prefix + gather; 1: prefix pass only (fills the workspace from `in`);
+// 2: gather pass only (reads the workspace, writes `out`).
+int execute_phase(const Plan3d &p, int op, int phase, const float *in, float *out, int nbatch,
+                  void *ws, size_t ws_bytes, int tile_rows, int voxel_chunk, hipStream_t stream);
+
+struct BatchItem {
+    const Plan3d *plan;
+    const float *in;
+    float *out;
+    int nbatch;
+};
+size_t batch_workspace_bytes(const std::vector<BatchItem> &items, int op);
+int execute_batch(const std::vector<BatchItem> &items, int op, void *ws, size_t ws_bytes,
+                  int tile_rows, int voxel_chunk, hipStream_t stream);
+
+}  // namespace xr
