@@ -43,7 +43,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-views", type=int, default=36, help="views in the CPU baseline sample")
+    ap.add_argument("--cpu-views", type=int, default=72, help="views in the CPU baseline sample")
     return ap.parse_args()
 
 
