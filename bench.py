@@ -44,7 +44,51 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-views", type=int, default=72, help="views in the CPU baseline sample")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run 3 device-to-device copies of 512 MiB (calibrates FETCH_SIZE/WRITE_SIZE counters)")
+    ap.add_argument("--summarize", nargs="+", metavar="DB",
+                    help="only print a per-kernel summary of rocprofv3 SQLite result files")
+    ap.add_argument("--verbose", action="store_true", help="print autotuner timings")
     return ap.parse_args()
+
+
+def summarize_rocpd(paths):
+    """Per-kernel summary of rocprofv3 SQLite outputs (kernel-trace and/or --pmc runs):
+    dispatch count, mean/min/max duration, and the mean of each collected counter."""
+    import sqlite3
+    from collections import defaultdict
+
+    for path in paths:
+        con = sqlite3.connect(path)
+        print(f"# {path}")
+        rows = con.execute("select name, duration from kernels").fetchall()
+        dur = defaultdict(list)
+        for name, d in rows:
+            dur[name].append(d)
+        pmc = defaultdict(lambda: defaultdict(list))
+        try:
+            for name, cname, val in con.execute("select name, counter_name, counter_value from pmc_events"):
+                pmc[name][cname].append(val)
+        except sqlite3.OperationalError:
+            pass
+        print(f"{'kernel':70s} {'calls':>5s} {'mean_us':>10s} {'min_us':>10s} {'max_us':>10s}  counters(mean)")
+        for name in sorted(dur, key=lambda n: -sum(dur[n])):
+            d = dur[name]
+            ctr = " ".join(f"{c}={sum(v) / len(v):.1f}" for c, v in sorted(pmc[name].items()))
+            short = name.split("(")[0][-70:]
+            print(f"{short:70s} {len(d):5d} {sum(d) / len(d) / 1e3:10.2f} {min(d) / 1e3:10.2f} {max(d) / 1e3:10.2f}  {ctr}")
+
+
+def calibrate_copy():
+    import torch
+
+    n = 512 * 1024 * 1024 // 4
+    src = torch.ones(n, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read_per_launch": n * 4, "copy_bytes_written_per_launch": n * 4, "launches": 3}))
 
 
 def workload():
@@ -83,6 +127,12 @@ def cpu_baseline(proj_opts, phantom, angles, nviews):
 
 def main():
     args = parse()
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
+    if args.summarize:
+        summarize_rocpd(args.summarize)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build runs on a single MI355X and has no multi-GPU path "
               f"(--gpus {args.gpus} requested)", file=sys.stderr)
@@ -113,15 +163,23 @@ def main():
     ws_a = torch.empty(plan.workspace_bytes(A, 1), dtype=torch.uint8, device=dev)
     ws_b = torch.empty(plan.workspace_bytes(B, 1), dtype=torch.uint8, device=dev)
 
+    from xrecon_amd import autotune
+
+    # Tile shapes chosen by the autotuner (untimed; the sinogram is produced first so B is
+    # tuned on real data).
+    plan.execute_into(A, x, sino, ws_a, sid, 0, 0, 0)
+    tune_a = autotune.tune(plan, A, x, sino, ws_a, verbose=args.verbose)
+    tune_b = autotune.tune(plan, B, sino, recon, ws_b, verbose=args.verbose)
+
     def step(ev=None):
         # A: prefix pass, then the cell-gather kernel (events bracket the gather only)
-        plan.execute_into(A, x, sino, ws_a, sid, 0, 0, 1)
+        plan.execute_into(A, x, sino, ws_a, sid, tune_a[0], tune_a[1], 1)
         if ev: ev[0].record(stream)
-        plan.execute_into(A, x, sino, ws_a, sid, 0, 0, 2)
+        plan.execute_into(A, x, sino, ws_a, sid, tune_a[0], tune_a[1], 2)
         if ev: ev[1].record(stream)
-        plan.execute_into(B, sino, recon, ws_b, sid, 0, 0, 1)
+        plan.execute_into(B, sino, recon, ws_b, sid, tune_b[0], tune_b[1], 1)
         if ev: ev[2].record(stream)
-        plan.execute_into(B, sino, recon, ws_b, sid, 0, 0, 2)
+        plan.execute_into(B, sino, recon, ws_b, sid, tune_b[0], tune_b[1], 2)
         if ev: ev[3].record(stream)
 
     for _ in range(args.warmup):
@@ -144,14 +202,15 @@ def main():
 
     # adjoint pair A + A^T (the autograd pair), same unit, untimed-kernel breakdown
     ws_at = torch.empty(plan.workspace_bytes(AT, 1), dtype=torch.uint8, device=dev)
+    tune_at = autotune.tune(plan, AT, sino, grad, ws_at, verbose=args.verbose)
     for _ in range(2):
-        plan.execute_into(AT, sino, grad, ws_at, sid, 0, 0, 0)
+        plan.execute_into(AT, sino, grad, ws_at, sid, tune_at[0], tune_at[1], 0)
     torch.cuda.synchronize()
     s2, e2 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     s2.record(stream)
     for _ in range(K):
-        plan.execute_into(A, x, sino, ws_a, sid, 0, 0, 0)
-        plan.execute_into(AT, sino, grad, ws_at, sid, 0, 0, 0)
+        plan.execute_into(A, x, sino, ws_a, sid, tune_a[0], tune_a[1], 0)
+        plan.execute_into(AT, sino, grad, ws_at, sid, tune_at[0], tune_at[1], 0)
     e2.record(stream)
     torch.cuda.synchronize()
     ms_adj = s2.elapsed_time(e2) / K
@@ -187,6 +246,7 @@ def main():
                      "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
                      "note": "FP32 vector-ALU roof (packed FP32 peak); no dense contraction, so no MFMA; "
                              "F = 4*nnz per launch"},
+        "tiles": {"A_tile_rows": tune_a[0], "B_voxel_chunk": tune_b[1]},
         "kernels_ms": {"cell_gather_A": t_cell, "voxel_gather_B": t_vox,
                        "prefix_passes": ms_step - t_cell - t_vox},
         "adjoint": {"ms_per_pair": ms_adj, "value": vox_views / (ms_adj * 1e-3), "unit": "voxel-views/s",

@@ -119,6 +119,13 @@ int xr_plan3d_execute_phase(const xr_plan3d *plan, int op, int phase, const floa
                             int num_batch, void *workspace, size_t workspace_bytes, int tile_rows,
                             int voxel_chunk, void *stream);
 
+/* Which gather family runs `op` on this plan: XR_KERNEL_CELL (detector-cell gather,
+ * tuned by tile_rows) or XR_KERNEL_VOXEL (voxel-column gather, tuned by voxel_chunk);
+ * negative for an unknown operator. */
+#define XR_KERNEL_CELL 1
+#define XR_KERNEL_VOXEL 2
+int xr_plan3d_kernel_kind(const xr_plan3d *plan, int op);
+
 /* Number of views / a copy of the plan's per-view flags (x-driven, keep-order), as
  * decided on the host with the reference's FP32 rule (include/cpu3d_common.h:65, :87). */
 int xr_plan3d_num_view(const xr_plan3d *plan);

@@ -47,6 +47,8 @@ def _load_oracle():
                                     ctypes.c_int, ctypes.c_int, ctypes.c_int]
         lib.oracle_view3d.restype = ctypes.c_int
         lib.oracle_view3d.argtypes = [ctypes.c_float] + [ctypes.c_void_p] * 8
+        lib.oracle_op2d.restype = ctypes.c_int
+        lib.oracle_op2d.argtypes = lib.oracle_op3d.argtypes
         _lib = lib
     return _lib
 
@@ -120,6 +122,27 @@ def op3d(op: str, x: np.ndarray, angles: np.ndarray, options: Sequence[float], n
     out = np.zeros(out_shape, dtype=np.float64)
     vn, vf, gn, gf = split3d(options)
     lib.oracle_op3d(OPS[op], _ptr(x), _ptr(out), _ptr(angles), _ptr(vn), _ptr(vf), _ptr(gn), _ptr(gf),
+                    nbatch, nview, nthreads or os.cpu_count() or 1)
+    return out
+
+
+def op2d(op: str, x: np.ndarray, angles: np.ndarray, options: Sequence[float], nthreads: int | None = None) -> np.ndarray:
+    """FP64 oracle for one 2D operator: image [nb, H, W] <-> sinogram [nb, V, Nu]."""
+    lib = _load_oracle()
+    angles = np.ascontiguousarray(angles, dtype=np.float32)
+    nview = angles.shape[0]
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    nbatch = x.shape[0]
+    o = [float(v) for v in options]
+    W, H, Nu = int(o[0]), int(o[1]), int(o[2])
+    in_shape, out_shape = ((nbatch, H, W), (nbatch, nview, Nu))
+    if op not in _VOLUME_IN:
+        in_shape, out_shape = out_shape, in_shape
+    if tuple(x.shape) != in_shape:
+        raise ValueError(f"oracle 2D {op}: expected input {in_shape}, got {x.shape}")
+    out = np.zeros(out_shape, dtype=np.float64)
+    vn, vf, gn, gf = split2d(options)
+    lib.oracle_op2d(OPS[op], _ptr(x), _ptr(out), _ptr(angles), _ptr(vn), _ptr(vf), _ptr(gn), _ptr(gf),
                     nbatch, nview, nthreads or os.cpu_count() or 1)
     return out
 

@@ -362,6 +362,162 @@ int64_t oracle_op3d(int op, const double *in, double *out, const float *ang,
     return op == OP_COUNT ? nnz : 0;
 }
 
+/* ------------------------------------------------------------------- 2D operators */
+
+/* The reference's 2D path, restated separately from the 3D one:
+ *   A    src/cpu/system_matrix2d.cpp:87-174 (parallel), :176-267 (fan-flat), :269-357 (fan-arc)
+ *        coefficient = ov * (dy or dx) / (W_c * cos2d), cos2d = |S|/sqrt((dbar-P)^2 + S^2)
+ *        (cos_weight_x/y, :58-68); parallel: ov * (dy or dx) / du.
+ *   B    src/cpu/backward2d.cpp:30-98 (fan-flat), :100-163 (fan-arc), :165-205 (parallel)
+ *        coefficient = ov * w * U / (p1 - p0) with w = du (flat), 2 SOD tan(du/2) (arc);
+ *        parallel: ov * du / (dx or dy), no weighting.
+ *   A^T, B^T are the transposes (forward2d.cpp:80-111, backward2d.cpp:433-475).
+ * Here du is the effective pitch (geometry.py:106-109); geometry passed as
+ * vn = {Nx, Ny}, vf = {dx, dy, shx, shy}, gn = {type, Nu}, gf = {du, shu, SOD, SDD}. */
+typedef struct {
+    int nx, ny, type, nu, nview, nbatch, op;
+    double dx, dy, shx, shy, du, shu, sod, sdd;
+    const float *ang;
+    const double *in;
+    double *out;
+    int lo, hi;
+} job2_t;
+
+static void view2_pixel(job2_t *jb, int v, int j, double *dcol, int xdrv, int keep, double sb, double cb)
+{
+    const int nx = jb->nx, ny = jb->ny, nu = jb->nu;
+    const int64_t nimg = (int64_t)nx * ny, nsino = (int64_t)jb->nview * nu;
+    const double sx = -jb->sod * sb, sy = jb->sod * cb;
+    const double P = xdrv ? sx : sy, S = xdrv ? sy : sx;
+    const double delta = xdrv ? jb->dy : jb->dx;
+    const double py = (ny * 0.5 - j - 0.5) * jb->dy + jb->shy;
+    for (int i = 0; i < nx; ++i) {
+        const double px = (i - nx * 0.5 + 0.5) * jb->dx + jb->shx;
+        double a0, a1;
+        if (jb->type == 0) {
+            if (xdrv) { a0 = (sb / cb) * py + px - 0.5 * jb->dx; a1 = a0 + jb->dx; }
+            else { a0 = (cb / sb) * px + py - 0.5 * jb->dy; a1 = a0 + jb->dy; }
+        } else if (xdrv) {
+            a0 = axis_map(1, sx, sy, px - 0.5 * jb->dx, py);
+            a1 = axis_map(1, sx, sy, px + 0.5 * jb->dx, py);
+        } else {
+            a0 = axis_map(0, sx, sy, px, py - 0.5 * jb->dy);
+            a1 = axis_map(0, sx, sy, px, py + 0.5 * jb->dy);
+        }
+        double U = 1.0;
+        if (jb->op == OP_BW || jb->op == OP_BWT) {
+            if (jb->type == 1) {
+                const double dd = (sx * px + sy * py) / jb->sod;
+                U = jb->sod * jb->sod / ((jb->sod - dd) * (jb->sod - dd));
+            } else if (jb->type == 2) {
+                U = 1.0 / ((sx - px) * (sx - px) + (sy - py) * (sy - py));
+            }
+        }
+        const double colw = jb->type == 2 ? 2.0 * jb->sod * tan(0.5 * jb->du) : jb->du;
+        for (int c = 0; c < nu; ++c) {
+            const double ou = ovl(a0, a1, dcol[c], dcol[c + 1]);
+            if (!(ou > 0.0)) continue;
+            double coef;
+            if (jb->op == OP_A || jb->op == OP_AT) {
+                if (jb->type == 0) {
+                    coef = ou * delta / jb->du;
+                } else {
+                    const double W = dcol[c + 1] - dcol[c], dbar = 0.5 * (dcol[c] + dcol[c + 1]);
+                    const double cosw = fabs(S) / sqrt((dbar - P) * (dbar - P) + S * S);
+                    coef = ou * delta / (W * cosw);
+                }
+            } else {
+                coef = ou * colw * U / (a1 - a0);
+            }
+            const int col = keep ? c : (nu - 1 - c);
+            const int64_t ii = (int64_t)j * nx + i, si = (int64_t)v * nu + col;
+            for (int b = 0; b < jb->nbatch; ++b) {
+                if (jb->op == OP_A || jb->op == OP_BT || jb->op == OP_BWT)
+                    jb->out[b * nsino + si] += coef * jb->in[b * nimg + ii];
+                else
+                    jb->out[b * nimg + ii] += coef * jb->in[b * nsino + si];
+            }
+        }
+    }
+}
+
+static void view2_cols(const job2_t *jb, float angf, double *dcol, int *xdrv, int *keep, double *sb, double *cb)
+{
+    const float sf = sinf(angf), cf = cosf(angf);
+    *xdrv = cf * cf > 0.5f;
+    *keep = *xdrv ? (cf >= 0.0f) : (sf >= 0.0f);
+    const double a = (double)angf;
+    *sb = sin(a);
+    *cb = cos(a);
+    const double sx = -jb->sod * *sb, sy = jb->sod * *cb;
+    for (int c = 0; c <= jb->nu; ++c) {
+        const double idx = *keep ? (c - jb->nu * 0.5) : (jb->nu * 0.5 - c);
+        if (jb->type == 0) {
+            dcol[c] = (idx * jb->du + jb->shu) / (*xdrv ? *cb : *sb);
+        } else if (jb->type == 1) {
+            const double u = idx * jb->du + jb->shu;
+            dcol[c] = axis_map(*xdrv, sx, sy, u * *cb, u * *sb);
+        } else {
+            const double g = idx * jb->du + jb->shu + a;
+            dcol[c] = *xdrv ? sx + sy * tan(g) : sy + sx / tan(g);
+        }
+    }
+}
+
+static void *worker2d(void *arg)
+{
+    job2_t *jb = (job2_t *)arg;
+    double *dcol = (double *)malloc(sizeof(double) * (jb->nu + 1));
+    const int to_sino = (jb->op == OP_A || jb->op == OP_BT || jb->op == OP_BWT);
+    int xdrv, keep;
+    double sb, cb;
+    if (to_sino) {
+        for (int v = jb->lo; v < jb->hi; ++v) {
+            view2_cols(jb, jb->ang[v], dcol, &xdrv, &keep, &sb, &cb);
+            for (int j = 0; j < jb->ny; ++j) view2_pixel(jb, v, j, dcol, xdrv, keep, sb, cb);
+        }
+    } else {
+        for (int v = 0; v < jb->nview; ++v) {
+            view2_cols(jb, jb->ang[v], dcol, &xdrv, &keep, &sb, &cb);
+            for (int j = jb->lo; j < jb->hi; ++j) view2_pixel(jb, v, j, dcol, xdrv, keep, sb, cb);
+        }
+    }
+    free(dcol);
+    return NULL;
+}
+
+/* op as oracle_op3d (0..5).  Image [nbatch, Ny, Nx], sinogram [nbatch, nview, Nu]. */
+int oracle_op2d(int op, const double *in, double *out, const float *ang, const int *vn, const float *vf,
+                const int *gn, const float *gf, int nbatch, int nview, int nthreads)
+{
+    if (op < 0 || op > OP_BWT) return -1;
+    job2_t base;
+    memset(&base, 0, sizeof(base));
+    base.nx = vn[0]; base.ny = vn[1];
+    base.dx = vf[0]; base.dy = vf[1]; base.shx = vf[2]; base.shy = vf[3];
+    base.type = gn[0]; base.nu = gn[1];
+    base.du = gf[0]; base.shu = gf[1]; base.sod = gf[2]; base.sdd = gf[3];
+    base.nview = nview; base.nbatch = nbatch; base.op = op; base.ang = ang; base.in = in; base.out = out;
+    const int to_sino = (op == OP_A || op == OP_BT || op == OP_BWT);
+    const int64_t n = to_sino ? (int64_t)nbatch * nview * base.nu : (int64_t)nbatch * base.nx * base.ny;
+    memset(out, 0, sizeof(double) * (size_t)n);
+    const int items = to_sino ? nview : base.ny;
+    if (nthreads < 1) nthreads = 1;
+    if (nthreads > items) nthreads = items > 0 ? items : 1;
+    pthread_t *th = (pthread_t *)malloc(sizeof(pthread_t) * nthreads);
+    job2_t *jobs = (job2_t *)malloc(sizeof(job2_t) * nthreads);
+    for (int t = 0; t < nthreads; ++t) {
+        jobs[t] = base;
+        jobs[t].lo = (int)((int64_t)items * t / nthreads);
+        jobs[t].hi = (int)((int64_t)items * (t + 1) / nthreads);
+        pthread_create(&th[t], NULL, worker2d, &jobs[t]);
+    }
+    for (int t = 0; t < nthreads; ++t) pthread_join(th[t], NULL);
+    free(th);
+    free(jobs);
+    return 0;
+}
+
 /* Per-view FP64 geometry of the cone oracle, exported so tests can check the tables
  * the product builds: flags[2] = {xdrv, keep}, src[2] = {sx, sy}, d[nu+1], sig[nu]. */
 int oracle_view3d(float ang, const int *vn, const float *vf, const int *gn, const float *gf,

@@ -1,0 +1,92 @@
+"""CPU: the product's kernel bodies (xrecon_amd/csrc/dd_body.h), run one thread at a time
+by the test-only emulator (tests/native/emu.cpp) with the product's own host plan
+builder and operator routing, against the FP64 oracle.  This checks the kernel logic on
+machines without a GPU; tests/test_gpu_parity.py checks the compiled gfx950 kernels."""
+import numpy as np
+import pytest
+
+import oracle
+from helpers import CASES3D, OPS, VOLUME_IN, angles_with_ties, emu_available, emu_op3d, opts2d, opts2d_as_3d, \
+    opts3d, rel_l2, shapes3d
+
+pytestmark = pytest.mark.skipif(not emu_available(), reason="emulator not built (run __graft_entry__.build())")
+
+CASES2D = {
+    "flat": opts2d(33, 17, 45, 1.0, 1.1, 0.5, 500.0, 1000.0, 1, 0.3, -0.2, 0.25),
+    "arc": opts2d(33, 17, 45, 1.0, 1.1, 2 * np.arctan(1.0 / 2000.0), 500.0, 1000.0, 2, 0.3, -0.2, 0.0003),
+    "par": opts2d(33, 17, 45, 1.0, 1.1, 0.9, 0.0, 0.0, 0, 0.3, -0.2, 0.25),
+}
+
+
+@pytest.mark.parametrize("geo", sorted(CASES3D))
+@pytest.mark.parametrize("op", OPS)
+def test_3d_bodies_vs_oracle(geo, op):
+    opts = CASES3D[geo]
+    V = 40
+    ang = angles_with_ties(V)
+    vol, sino = shapes3d(opts, 2, V)
+    x = np.random.default_rng(1).random(vol if op in VOLUME_IN else sino).astype(np.float32)
+    got = emu_op3d(op, x, ang, opts)
+    want = oracle.op3d(op, x.astype(np.float64), ang, opts)
+    assert rel_l2(got, want) <= 1e-5
+
+
+@pytest.mark.parametrize("geo", sorted(CASES2D))
+@pytest.mark.parametrize("op", OPS)
+def test_2d_bodies_vs_oracle(geo, op):
+    o2 = CASES2D[geo]
+    V = 40
+    ang = angles_with_ties(V)
+    rng = np.random.default_rng(2)
+    if op in VOLUME_IN:
+        x = rng.random((2, 17, 33)).astype(np.float32)
+        got = emu_op3d(op, x[:, None], ang, opts2d_as_3d(o2), is2d=True).reshape(2, V, 45)
+    else:
+        x = rng.random((2, V, 45)).astype(np.float32)
+        got = emu_op3d(op, x[:, :, None], ang, opts2d_as_3d(o2), is2d=True).reshape(2, 17, 33)
+    want = oracle.op2d(op, x.astype(np.float64), ang, o2)
+    assert rel_l2(got, want) <= 1e-5
+
+
+@pytest.mark.parametrize("R,KC", [(1, 1), (2, 4), (4, 8), (8, 16)])
+def test_tilings_give_same_results(R, KC):
+    """Tile shapes change only the work split; results agree with the oracle for each."""
+    opts = CASES3D["flat"]
+    V = 12
+    ang = angles_with_ties(V)
+    vol, sino = shapes3d(opts, 1, V)
+    rng = np.random.default_rng(3)
+    x, y = rng.random(vol).astype(np.float32), rng.random(sino).astype(np.float32)
+    assert rel_l2(emu_op3d("A", x, ang, opts, R=R), oracle.op3d("A", x.astype(np.float64), ang, opts)) <= 1e-5
+    assert rel_l2(emu_op3d("B", y, ang, opts, KC=KC), oracle.op3d("B", y.astype(np.float64), ang, opts)) <= 1e-5
+
+
+def test_wide_columns_take_general_path():
+    """Detector columns wider than a pixel (a coarse detector) exercise the general
+    multi-pixel loop of the cell gather instead of the two-pixel fast path."""
+    opts = opts3d(20, 18, 6, 12, 5, 0.5, 0.5, 0.5, 4.0, 2.0, 500.0, 1000.0, 1, 0.1, 0.0, 0.0, 0.3, 0.1)
+    ang = angles_with_ties(16)
+    vol, sino = shapes3d(opts, 1, 16)
+    rng = np.random.default_rng(4)
+    x, y = rng.random(vol).astype(np.float32), rng.random(sino).astype(np.float32)
+    for op, inp in (("A", x), ("BT", x), ("B", y), ("AT", y)):
+        assert rel_l2(emu_op3d(op, inp, ang, opts), oracle.op3d(op, inp.astype(np.float64), ang, opts)) <= 1e-5
+
+
+def test_slice_chunk_remainder_and_odd_sizes():
+    """Nz not a multiple of the slice chunk, Nv not a multiple of the row tile, odd Nu."""
+    opts = opts3d(21, 19, 70, 37, 13, 1.0, 1.0, 0.4, 1.0, 1.0, 500.0, 1000.0, 1)
+    ang = angles_with_ties(8)
+    vol, sino = shapes3d(opts, 1, 8)
+    rng = np.random.default_rng(5)
+    x, y = rng.random(vol).astype(np.float32), rng.random(sino).astype(np.float32)
+    assert rel_l2(emu_op3d("A", x, ang, opts, R=8), oracle.op3d("A", x.astype(np.float64), ang, opts)) <= 1e-5
+    assert rel_l2(emu_op3d("B", y, ang, opts, KC=16), oracle.op3d("B", y.astype(np.float64), ang, opts)) <= 1e-5
+
+
+def test_volume_outside_field_of_view_gives_zero():
+    """A volume shifted entirely out of the detector's field of view projects to zero."""
+    opts = opts3d(8, 8, 4, 16, 4, 1.0, 1.0, 1.0, 1.0, 1.0, 500.0, 1000.0, 1, 300.0, 0.0, 0.0)
+    ang = np.array([0.0], np.float32)
+    x = np.ones((1, 4, 8, 8), np.float32)
+    assert np.all(emu_op3d("A", x, ang, opts) == 0.0)

@@ -1,0 +1,81 @@
+"""Autotuner for the gather kernels' tile shapes.
+
+Two knobs exist, one per kernel family (``Plan.kernel_kind(op)``):
+
+* cell gathers (A, B^T, B_W^T): ``tile_rows`` in {1, 2, 4, 8} -- detector rows per
+  thread.  More rows amortise each line's column overlaps and share row boundaries,
+  but cost registers and therefore resident waves.
+* voxel gathers (B, B_W, A^T): ``voxel_chunk`` in {1, 4, 8, 16} -- slices per thread.
+  Longer chunks amortise each view's column search, shorter ones give more waves.
+
+``tune`` times the gather pass alone (the prefix pass is shared) for every candidate
+with HIP events on the caller's stream, takes the median of a few launches, and caches
+the winner per (geometry, views, operator, batch).  Tuning launches kernels and
+synchronises, so run it before graph capture or timed regions.
+"""
+from __future__ import annotations
+
+import statistics
+from typing import Dict, Tuple
+
+import torch
+
+from . import _backend as be
+
+CELL = 1
+VOXEL = 2
+TILE_ROWS = (1, 2, 4, 8)
+VOXEL_CHUNKS = (1, 4, 8, 16)
+
+_CACHE: Dict[tuple, Tuple[int, int]] = {}
+
+
+def _key(plan, op: int, nbatch: int):
+    return (plan.handle, op, nbatch)
+
+
+def cached(plan, op: int, nbatch: int):
+    return _CACHE.get(_key(plan, op, nbatch))
+
+
+def candidates(plan, op: int):
+    if plan.kernel_kind(op) == CELL:
+        return [(r, 0) for r in TILE_ROWS]
+    return [(0, k) for k in VOXEL_CHUNKS]
+
+
+def tune(plan, op: int, x: torch.Tensor, out: torch.Tensor, ws: torch.Tensor, reps: int = 3,
+         verbose: bool = False) -> Tuple[int, int]:
+    """Pick (tile_rows, voxel_chunk) for `op` on input `x` (shape fixes the batch).
+    Returns the cached choice if this (plan, op, batch) was tuned before."""
+    nb = x.shape[0] * x.shape[1]
+    key = _key(plan, op, nb)
+    if key in _CACHE:
+        return _CACHE[key]
+    stream = torch.cuda.current_stream(x.device)
+    sid = stream.cuda_stream
+    plan.execute_into(op, x, out, ws, sid, 0, 0, 1)  # prefix pass, shared by all candidates
+    best, best_t = None, float("inf")
+    timings = {}
+    for rows, chunk in candidates(plan, op):
+        plan.execute_into(op, x, out, ws, sid, rows, chunk, 2)  # warm-up
+        ts = []
+        for _ in range(reps):
+            s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            s.record(stream)
+            plan.execute_into(op, x, out, ws, sid, rows, chunk, 2)
+            e.record(stream)
+            e.synchronize()
+            ts.append(s.elapsed_time(e))
+        t = statistics.median(ts)
+        timings[(rows, chunk)] = t
+        if t < best_t:
+            best, best_t = (rows, chunk), t
+    if verbose:
+        print("autotune op", op, {k: round(v, 3) for k, v in timings.items()}, "->", best, flush=True)
+    _CACHE[key] = best
+    return best
+
+
+def clear() -> None:
+    _CACHE.clear()

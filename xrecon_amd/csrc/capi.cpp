@@ -167,6 +167,12 @@ int xr_plan3d_execute(const xr_plan3d *plan, int op, const float *in, float *out
 
 int xr_plan3d_num_view(const xr_plan3d *plan) { return plan ? plan->p.nview : XR_EINVAL; }
 
+int xr_plan3d_kernel_kind(const xr_plan3d *plan, int op)
+{
+    if (!plan) return fail(XR_EINVAL, "null plan");
+    return xr::kernel_kind(plan->p, op);
+}
+
 int xr_plan3d_view_flags(const xr_plan3d *plan, int *xdriven, int *keep_order)
 {
     if (!plan) return fail(XR_EINVAL, "null plan");

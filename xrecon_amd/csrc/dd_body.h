@@ -33,6 +33,7 @@ struct VoxArgs {
     int nx, ny, nz, nu, nv, nview, maxline, nbatch, nchunk;
     float zb, dz, rv0, sod;
     float dx, dy, shx, shy;
+    float dvv;
 };
 
 struct SinoArgs {
@@ -85,6 +86,7 @@ inline VoxArgs make_vox_args(const Plan3d &p, int nbatch, int KC)
     a.nchunk = (a.nz + KC - 1) / KC;
     a.zb = p.zb; a.dz = p.dz; a.rv0 = p.rv0; a.sod = p.sod;
     a.dx = p.vol.dx; a.dy = p.vol.dy; a.shx = p.vol.shift_x; a.shy = p.vol.shift_y;
+    a.dvv = p.dvv;
     return a;
 }
 
@@ -94,6 +96,26 @@ inline SinoArgs make_sino_args(const Plan3d &p, int nbatch)
     a.nu = p.geo.nu; a.nv = p.geo.nv; a.nview = p.nview; a.nbatch = nbatch;
     a.v0 = p.v0; a.dvv = p.dvv;
     return a;
+}
+
+// clamp(x, lo, hi) for finite x: one v_med3_f32 on the device.
+__host__ __device__ inline float clampf(float x, float lo, float hi)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_fmed3f(x, lo, hi);
+#else
+    return fminf(fmaxf(x, lo), hi);
+#endif
+}
+
+// x - floor(x) for x >= 0: one v_fract_f32 on the device.
+__host__ __device__ inline float fractf(float x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_fractf(x);
+#else
+    return x - floorf(x);
+#endif
 }
 
 // FDK distance weight U of a voxel centre p for a view with source s
@@ -207,49 +229,83 @@ __host__ __device__ inline void cell_body(const ViewHdr *__restrict__ hdr, const
     float acc[R];
     for (int q = 0; q < R; ++q) acc[q] = 0.0f;
 
+    // Weight of pixel m of line L: its column overlap ou (pixel units) times the
+    // per-line factor of the operator.
+    auto pixel_weight = [&](float ou, float lam, int L, int m) -> float {
+        if (OPK == CELL_A) return ou * lam;
+        if (OPK == CELL_BWT) {
+            float px, py;
+            if (h.xdrv) {
+                px = ((float)m - 0.5f * a.nx + 0.5f) * a.dx + a.shx;
+                py = (0.5f * a.ny - (float)L - 0.5f) * a.dy + a.shy;
+            } else {
+                px = ((float)L - 0.5f * a.nx + 0.5f) * a.dx + a.shx;
+                py = ((float)m - 0.5f * a.ny + 0.5f) * a.dy + a.shy;
+            }
+            return ou * fdk_weight<GEO>(h.sx, h.sy, px, py, a.sod);
+        }
+        return ou;
+    };
+
+    float4 lnext = lt[0];
     for (int L = 0; L < nline; ++L) {
-        const float4 lc = lt[L];  // {toff, invd, iz, lam}
+        const float4 lc = lnext;  // {toff, invd, iz, lam}
+        lnext = lt[L + 1 < nline ? L + 1 : L];
         // Column boundaries in pixel units along the line.
         const float t0 = fmaf(d0, lc.y, lc.x);
         const float t1 = fmaf(d1, lc.y, lc.x);
         if (!(t1 > 0.0f && t0 < fal)) continue;
         // Row boundaries in voxel units along z at this line's magnification.
-        int koff[R + 1];
+        unsigned koff[R + 1];
         float fr[R + 1];
+#pragma unroll
         for (int q = 0; q <= R; ++q) {
-            float z = fmaf(hv[q], lc.z, a.zoff);
-            z = fminf(fmaxf(z, 0.0f), fnz);
-            const int k = (int)z;
-            fr[q] = z - (float)k;
-            koff[q] = k * plane;
+            const float z = clampf(fmaf(hv[q], lc.z, a.zoff), 0.0f, fnz);
+            koff[q] = (unsigned)(int)z * (unsigned)plane;
+            fr[q] = fractf(z);
         }
-        const int m0 = t0 > 0.0f ? (int)floorf(t0) : 0;
-        const int m1 = t1 < fal ? (int)ceilf(t1) : nalong;
         const float2 *Wl = W + L * nalong;
-        for (int m = m0; m < m1; ++m) {
-            const float ou = fminf(t1, (float)(m + 1)) - fmaxf(t0, (float)m);
-            float w;
-            if (OPK == CELL_A) {
-                w = ou * lc.w;
-            } else if (OPK == CELL_BWT) {
-                float px, py;
-                if (h.xdrv) {
-                    px = ((float)m - 0.5f * a.nx + 0.5f) * a.dx + a.shx;
-                    py = (0.5f * a.ny - (float)L - 0.5f) * a.dy + a.shy;
-                } else {
-                    px = ((float)L - 0.5f * a.nx + 0.5f) * a.dx + a.shx;
-                    py = ((float)m - 0.5f * a.ny + 0.5f) * a.dy + a.shy;
-                }
-                w = ou * fdk_weight<GEO>(h.sx, h.sy, px, py, a.sod);
-            } else {
-                w = ou;
-            }
-            float J[R + 1];
+        if (h.narrow) {
+            // The column spans less than one pixel: it overlaps pixel m0 and possibly m0+1.
+            // Both pixels' boundary lookups are issued together.
+            const int m0 = t0 > 0.0f ? (int)t0 : 0;
+            const bool has1 = m0 + 1 < nalong;
+            const int m1 = has1 ? m0 + 1 : m0;
+            const float w0 = pixel_weight(fminf(t1, (float)(m0 + 1)) - fmaxf(t0, (float)m0), lc.w, L, m0);
+            const float w1 = has1 ? pixel_weight(fmaxf(t1 - (float)(m0 + 1), 0.0f), lc.w, L, m1) : 0.0f;
+            float2 e0[R + 1], e1[R + 1];
+#pragma unroll
             for (int q = 0; q <= R; ++q) {
-                const float2 e = Wl[koff[q] + m];
-                J[q] = fmaf(fr[q], e.y, e.x);
+                e0[q] = Wl[m0 + koff[q]];
+                e1[q] = Wl[m1 + koff[q]];
             }
-            for (int q = 0; q < R; ++q) acc[q] = fmaf(w, J[q + 1] - J[q], acc[q]);
+            float J0 = fmaf(fr[0], e0[0].y, e0[0].x);
+            float J1 = fmaf(fr[0], e1[0].y, e1[0].x);
+#pragma unroll
+            for (int q = 0; q < R; ++q) {
+                const float K0 = fmaf(fr[q + 1], e0[q + 1].y, e0[q + 1].x);
+                const float K1 = fmaf(fr[q + 1], e1[q + 1].y, e1[q + 1].x);
+                acc[q] = fmaf(w0, K0 - J0, fmaf(w1, K1 - J1, acc[q]));
+                J0 = K0;
+                J1 = K1;
+            }
+        } else {
+            const int m0 = t0 > 0.0f ? (int)floorf(t0) : 0;
+            const int m1 = t1 < fal ? (int)ceilf(t1) : nalong;
+            for (int m = m0; m < m1; ++m) {
+                const float w = pixel_weight(fminf(t1, (float)(m + 1)) - fmaxf(t0, (float)m), lc.w, L, m);
+                const float2 *Wm = Wl + m;
+                float2 e[R + 1];
+#pragma unroll
+                for (int q = 0; q <= R; ++q) e[q] = Wm[koff[q]];
+                float Jp = fmaf(fr[0], e[0].y, e[0].x);
+#pragma unroll
+                for (int q = 0; q < R; ++q) {
+                    const float Jn = fmaf(fr[q + 1], e[q + 1].y, e[q + 1].x);
+                    acc[q] = fmaf(w, Jn - Jp, acc[q]);
+                    Jp = Jn;
+                }
+            }
         }
     }
 
@@ -277,14 +333,6 @@ __host__ __device__ inline void cell_body(const ViewHdr *__restrict__ hdr, const
 
 // ---------------------------------------------------------------- voxel gather
 
-__host__ __device__ inline float row_integral(const float2 *__restrict__ Qc, int nu, float fnv, float rho)
-{
-    rho = fminf(fmaxf(rho, 0.0f), fnv);
-    const int r = (int)rho;
-    const float2 e = Qc[(int64_t)r * nu];
-    return fmaf(rho - (float)r, e.y, e.x);
-}
-
 // One thread: voxel column (i, j), slices k0..k0+KC-1, batch b.
 template <int OPK, int GEO, int KC>
 __host__ __device__ inline void voxel_body(const ViewHdr *__restrict__ hdr, const float4 *__restrict__ lineV,
@@ -310,28 +358,28 @@ __host__ __device__ inline void voxel_body(const ViewHdr *__restrict__ hdr, cons
 
         float hB = 1.0f;
         if (OPK != VOX_AT) {
-            // U * w / A_vox with A_vox = Delta * zeta * dz (include/cpu3d_common.h:80-85)
+            // U * w / A_vox with A_vox = Delta * zeta * dz (include/cpu3d_common.h:80-85),
+            // times the row pitch dvv of the depth overlap measured in row units.
             const float U = (OPK == VOX_BW) ? fdk_weight<GEO>(h.sx, h.sy, px, py, a.sod) : 1.0f;
-            hB = U * h.colw / (lv.y * lv.z * a.dz);
+            hB = U * h.colw * a.dvv / (lv.y * lv.z * a.dz);
         }
 
         // First column whose right boundary exceeds a0: closed-form inverse, then
         // corrected against the table so the walk is exact w.r.t. the FP32 boundaries.
         const float T = (h.tp1 * a0 + h.tp0) / (h.tq1 * a0 + h.tq0);
         const float tc = h.tk0 + h.tk1 * (GEO == GEO_ARC ? atanf(T) : T);
-        int c = (int)floorf(fminf(fmaxf(tc, 0.0f), (float)(nu - 1)));
+        int c = (int)floorf(clampf(tc, 0.0f, (float)(nu - 1)));
         const float4 *ct = colt + (int64_t)v * (nu + 1);
-        float dc = ct[c].x;
-        while (c > 0 && dc > a0) { --c; dc = ct[c].x; }
-        float dn = ct[c + 1].x;
-        while (c < nu - 1 && dn <= a0) { ++c; dc = dn; dn = ct[c + 1].x; }
+        float4 cc = ct[c], cn = ct[c + 1];
+        while (c > 0 && cc.x > a0) { --c; cn = cc; cc = ct[c]; }
+        while (c < nu - 1 && cn.x <= a0) { ++c; cc = cn; cn = ct[c + 1]; }
 
         const float zbz = lv.z * a.zb, zsz = lv.z * a.dz;
         const float2 *Qv = Q + ((int64_t)b * a.nview + v) * (int64_t)(a.nv + 1) * nu;
-        while (c < nu && dc < a1) {
-            const float4 cc = ct[c];
-            dn = ct[c + 1].x;
-            const float ou = fminf(a1, dn) - fmaxf(a0, dc);
+        while (c < nu && cc.x < a1) {
+            // prefetch the next column's table entry one iteration ahead
+            const float4 cnn = ct[c + 2 <= nu ? c + 2 : nu];
+            const float ou = fminf(a1, cn.x) - fmaxf(a0, cc.x);
             if (ou > 0.0f) {
                 // Row coordinate of slice boundary k: rho_k = (zeta z_k / sigma - v0) / dvv.
                 const float kap = cc.z;
@@ -340,18 +388,35 @@ __host__ __device__ inline void voxel_body(const ViewHdr *__restrict__ hdr, cons
                 const float rlo = fmaf((float)k0, step, base);
                 const float rhi = fmaf((float)(k0 + KC), step, base);
                 if (rhi > 0.0f && rlo < fnv) {
-                    const float w = (OPK == VOX_AT) ? ou : ou * hB / kap;
+                    // B: ou * U w sigma dvv / (Delta zeta dz); A^T: ou (normalisation folded
+                    // into the prefix table).
+                    const float w = (OPK == VOX_AT) ? ou : ou * hB * cc.y;
                     const float2 *Qc = Qv + c;
-                    float Ip = row_integral(Qc, nu, fnv, rlo);
+                    // All KC+1 boundary lookups are independent: compute the rows, issue
+                    // the loads back to back, then consume.
+                    unsigned off[KC + 1];
+                    float fr[KC + 1];
+#pragma unroll
+                    for (int q = 0; q <= KC; ++q) {
+                        const float rho = clampf(fmaf((float)q, step, rlo), 0.0f, fnv);
+                        off[q] = (unsigned)(int)rho * (unsigned)nu;
+                        fr[q] = fractf(rho);
+                    }
+                    float2 e[KC + 1];
+#pragma unroll
+                    for (int q = 0; q <= KC; ++q) e[q] = Qc[off[q]];
+                    float Ip = fmaf(fr[0], e[0].y, e[0].x);
+#pragma unroll
                     for (int q = 0; q < KC; ++q) {
-                        const float In = row_integral(Qc, nu, fnv, fmaf((float)(k0 + q + 1), step, base));
+                        const float In = fmaf(fr[q + 1], e[q + 1].y, e[q + 1].x);
                         acc[q] = fmaf(w, In - Ip, acc[q]);
                         Ip = In;
                     }
                 }
             }
             ++c;
-            dc = dn;
+            cc = cn;
+            cn = cnn;
         }
     }
 

@@ -202,6 +202,13 @@ void launch_sino_prefix(const Plan3d &p, bool at, const float *in, float2 *Q, in
 
 }  // namespace
 
+int kernel_kind(const Plan3d &p, int op)
+{
+    const Route rt = route(p, op);
+    if (rt.kop < 0) return fail(XR_EINVAL, "unknown operator " + std::to_string(op));
+    return rt.cell ? XR_KERNEL_CELL : XR_KERNEL_VOXEL;
+}
+
 size_t workspace_bytes(const Plan3d &p, int op, int nbatch)
 {
     if (nbatch <= 0) return 0;

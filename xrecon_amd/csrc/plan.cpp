@@ -7,9 +7,9 @@
 // y-driven and the column order) are made exactly as the reference makes them: FP32
 // sinf/cosf of the float32 angle, squared in FP32, compared with 0.5f
 // (include/cpu3d_common.h:65, :87; src/cpu/forward3d.cpp:309-310).
+#include <algorithm>
 #include <cmath>
 #include <cstring>
-#include <mutex>
 
 #include "xr_internal.h"
 
@@ -215,6 +215,16 @@ int build_tables(Plan3d &p, const float *angles, PlanTables &t, std::string &err
         t.col[cb0 + nu] = float4{(float)d[nu], 0.0f, 0.0f, 0.0f};
         h.dlo = (float)d[0];
         h.dhi = (float)d[nu];
+        // Narrow view: the widest column, measured in pixel widths of the line with the
+        // smallest pixel footprint, stays below one, so a column overlaps at most two
+        // consecutive pixels of any line (the cell gather then takes a fixed two-pixel
+        // path).  A small margin keeps FP32 rounding of the boundaries on the safe side.
+        {
+            double wmax = 0.0, dmin = 1e300;
+            for (int c = 0; c < nu; ++c) wmax = std::max(wmax, d[c + 1] - d[c]);
+            for (int L = 0; L < h.nline; ++L) dmin = std::min(dmin, 1.0 / (double)t.lineC[(size_t)v * ML + L].y);
+            h.narrow = (wmax < 0.99 * dmin) ? 1 : 0;
+        }
 
         // Inverse map axis point -> fractional internal column, used by the voxel kernels
         // to find a footprint's first column (then corrected against the table).

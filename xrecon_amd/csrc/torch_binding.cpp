@@ -116,6 +116,13 @@ public:
         return (int64_t)xr_plan3d_workspace_bytes(plan_, op, (int)nbatch);
     }
 
+    int kernel_kind(int op) const
+    {
+        const int k = xr_plan3d_kernel_kind(plan_, op);
+        if (k < 0) raise_xr(k, "xr_plan3d_kernel_kind");
+        return k;
+    }
+
     std::vector<int> view_flags() const
     {
         std::vector<int> x(nview_), k(nview_);
@@ -226,6 +233,7 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m)
         .def_property_readonly("handle", &Plan::handle)
         .def("workspace_bytes", &Plan::workspace_bytes)
         .def("view_flags", &Plan::view_flags)
+        .def("kernel_kind", &Plan::kernel_kind)
         .def("execute", &Plan::execute, py::arg("op"), py::arg("input"), py::arg("stream") = 0)
         .def("execute_into", &Plan::execute_into, py::arg("op"), py::arg("input"), py::arg("out"),
              py::arg("workspace"), py::arg("stream") = 0, py::arg("tile_rows") = 0, py::arg("voxel_chunk") = 0,

@@ -34,7 +34,8 @@ struct alignas(16) ViewHdr {
     float tk0, tk1;  // fractional internal column of axis point a: tk0 + tk1 * g(T(a))
     float sx, sy;    // source position (cone) for the FDK weights
     float dlo, dhi;  // d_0 and d_Nu
-    float pad0, pad1;
+    int narrow;      // 1: every column spans at most one pixel width on every line of this view
+    float pad1;
 };
 static_assert(sizeof(ViewHdr) == 80, "ViewHdr layout");
 
@@ -80,6 +81,8 @@ int fail(int code, const std::string &msg);
 size_t workspace_bytes(const Plan3d &p, int op, int nbatch);
 int execute(const Plan3d &p, int op, const float *in, float *out, int nbatch, void *ws,
             size_t ws_bytes, int tile_rows, int voxel_chunk, hipStream_t stream);
+// XR_KERNEL_CELL / XR_KERNEL_VOXEL for op on this plan, XR_EINVAL if unknown.
+int kernel_kind(const Plan3d &p, int op);
 // phase 0: prefix + gather; 1: prefix pass only (fills the workspace from `in`);
 // 2: gather pass only (reads the workspace, writes `out`).
 int execute_phase(const Plan3d &p, int op, int phase, const float *in, float *out, int nbatch,
