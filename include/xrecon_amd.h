@@ -133,18 +133,27 @@ int xr_plan3d_view_flags(const xr_plan3d *plan, int *xdriven, int *keep_order);
 
 /* ------------------------------------------ batched uneven instances (one launch) */
 
-/* Runs the same `op` over `num_instances` independent plans (uneven volumes, detectors
- * and view counts) as persistent kernels whose work list spans every instance:
- * one prefix pass, one gather pass.  in[i]/out[i] are the instances' device buffers,
- * num_batch[i] their batch counts.  `workspace` must hold
- * xr_batch3d_workspace_bytes().  `tile_rows` (1, 2, 4 or 8; 0 = default) is the
- * detector-row register blocking of the cell kernels, `voxel_chunk` (4, 8 or 16;
- * 0 = default) the z-chunk of the voxel kernels. */
-size_t xr_batch3d_workspace_bytes(xr_plan3d *const *plans, int num_instances, int op,
-                                  const int *num_batch);
-int xr_batch3d_execute(xr_plan3d *const *plans, int num_instances, int op, const float *const *in,
-                       float *const *out, const int *num_batch, void *workspace,
-                       size_t workspace_bytes, int tile_rows, int voxel_chunk, void *stream);
+/* A batch groups up to XR_BATCH_MAX independent plans of one geometry family (uneven
+ * volumes, detectors and view counts), each with its own batch count.  Executing an
+ * operator on a batch launches each pass (prefix, transpose, gather) ONCE for all
+ * instances: every block of the launch maps its index to (instance, tile) from the
+ * per-instance descriptors uploaded at creation.  The per-call input/output pointers
+ * travel as kernel arguments, so execution copies nothing and allocates nothing and
+ * can be captured into a hipGraph.  The plans must outlive the batch. */
+#define XR_BATCH_MAX 32
+typedef struct xr_batch3d xr_batch3d;
+
+int xr_batch3d_create(xr_plan3d *const *plans, int num_instances, const int *num_batch,
+                      xr_batch3d **out);
+void xr_batch3d_destroy(xr_batch3d *batch);
+/* Workspace (device bytes) that xr_batch3d_execute needs for `op`. */
+size_t xr_batch3d_workspace_bytes(const xr_batch3d *batch, int op);
+/* in[i] / out[i]: instance i's device buffers in the layouts above.  `tile_rows`
+ * (1, 2, 4, 8; 0 = default) and `voxel_chunk` (1, 4, 8, 16; 0 = default) apply to all
+ * instances of the launch. */
+int xr_batch3d_execute(const xr_batch3d *batch, int op, const float *const *in, float *const *out,
+                       void *workspace, size_t workspace_bytes, int tile_rows, int voxel_chunk,
+                       void *stream);
 
 /* Same as xr_plan3d_execute with explicit tiling parameters (see above); used by the
  * autotuner. */

@@ -1,0 +1,366 @@
+// Batched execution of one operator over many uneven instances in single launches.
+//
+// The reference runs one instance per call (B*C folded into the grid,
+// src/api/torch_binding.cpp:422).  Here a batch of up to XR_BATCH_MAX plans of one
+// geometry family (different volumes, detectors, view counts, batch sizes) runs each pass
+// -- volume prefix, transpose, cell gather; or sinogram prefix, voxel gather -- as ONE
+// launch: block b of a launch walks the per-instance block counts (computed from the
+// static descriptors uploaded at batch creation and the launch's tile shape) to find its
+// instance and tile, then runs the same per-thread body as the single-instance kernels.
+// Per-call pointers travel by value in the kernel arguments, so executing a batch copies
+// nothing, allocates nothing and is capturable into a hipGraph.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "dd_body.h"
+
+namespace xr {
+
+namespace {
+
+struct BatchDesc {
+    const ViewHdr *hdr;
+    const float4 *lineC, *lineV, *col;
+    CellArgs ca;
+    VoxArgs va;  // nchunk is filled per launch from the launch's voxel chunk
+    SinoArgs sa;
+    int nb, any_y;
+    unsigned long long ws_volx, ws_voly, ws_sino;  // byte offsets into the workspace
+};
+
+struct IOPtrs {
+    const float *in[XR_BATCH_MAX];
+    float *out[XR_BATCH_MAX];
+};
+
+// Block counts per instance and pass (identical on host and device).
+__host__ __device__ inline int blocks_vol_prefix(const BatchDesc &d)
+{
+    return ceil_div((int64_t)d.nb * d.ca.nx * d.ca.ny, kBlock);
+}
+__host__ __device__ inline int blocks_transpose(const BatchDesc &d)
+{
+    return d.any_y ? ceil_div(d.ca.nx, 32) * ceil_div(d.ca.ny, 32) * d.nb * (d.ca.nz + 1) : 0;
+}
+__host__ __device__ inline int blocks_cell(const BatchDesc &d, int R)
+{
+    return ceil_div(d.ca.nu, kWave) * ceil_div(d.ca.nv, R * kWavesPerBlock) * d.nb * d.ca.nview;
+}
+__host__ __device__ inline int blocks_sino_prefix(const BatchDesc &d)
+{
+    return ceil_div(d.sa.nu, kBlock) * d.sa.nview * d.nb;
+}
+__host__ __device__ inline int blocks_voxel(const BatchDesc &d, int KC)
+{
+    return ceil_div(d.va.nx, kWave) * ceil_div(d.va.ny, kWavesPerBlock) * d.nb * ceil_div(d.va.nz, KC);
+}
+
+// Maps the flat block index to (instance, local block); returns the instance or -1.
+template <typename F>
+__device__ inline int find_instance(const BatchDesc *__restrict__ D, int n, int &bid, F count)
+{
+    for (int i = 0; i < n; ++i) {
+        const int c = count(D[i]);
+        if (bid < c) return i;
+        bid -= c;
+    }
+    return -1;
+}
+
+__global__ __launch_bounds__(kBlock) void k_vol_prefix_batch(const BatchDesc *__restrict__ D, int n, IOPtrs io,
+                                                             char *__restrict__ ws)
+{
+    int bid = blockIdx.x;
+    const int i = find_instance(D, n, bid, [](const BatchDesc &d) { return blocks_vol_prefix(d); });
+    if (i < 0) return;
+    const BatchDesc &d = D[i];
+    const int64_t plane = (int64_t)d.ca.nx * d.ca.ny;
+    const int64_t idx = (int64_t)bid * kBlock + threadIdx.x;
+    if (idx >= plane * d.nb) return;
+    const int64_t b = idx / plane;
+    vol_prefix_body(io.in[i], reinterpret_cast<float2 *>(ws + d.ws_volx), plane, d.ca.nz, b, idx - b * plane);
+}
+
+__global__ __launch_bounds__(kBlock) void k_transpose_batch(const BatchDesc *__restrict__ D, int n,
+                                                            char *__restrict__ ws)
+{
+    __shared__ float2 tile[32][33];
+    int bid = blockIdx.x;
+    const int i = find_instance(D, n, bid, [](const BatchDesc &d) { return blocks_transpose(d); });
+    if (i < 0) return;
+    const BatchDesc &d = D[i];
+    const int nx = d.ca.nx, ny = d.ca.ny;
+    const int tx_n = ceil_div(nx, 32), ty_n = ceil_div(ny, 32);
+    const int bx = bid % tx_n, by = (bid / tx_n) % ty_n, p = bid / (tx_n * ty_n);
+    const int64_t plane = (int64_t)nx * ny;
+    const float2 *src = reinterpret_cast<const float2 *>(ws + d.ws_volx) + (int64_t)p * plane;
+    float2 *dst = reinterpret_cast<float2 *>(ws + d.ws_voly) + (int64_t)p * plane;
+    const int i0 = bx * 32, j0 = by * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    for (int r = ty; r < 32; r += 8) {
+        const int j = j0 + r, ii = i0 + tx;
+        if (j < ny && ii < nx) tile[r][tx] = src[(int64_t)j * nx + ii];
+    }
+    __syncthreads();
+    for (int r = ty; r < 32; r += 8) {
+        const int ii = i0 + r, j = j0 + tx;
+        if (ii < nx && j < ny) dst[(int64_t)ii * ny + (ny - 1 - j)] = tile[tx][r];
+    }
+}
+
+template <int OPK, int GEO, int R>
+__global__ __launch_bounds__(kBlock) void k_cell_batch(const BatchDesc *__restrict__ D, int n, IOPtrs io,
+                                                       char *__restrict__ ws)
+{
+    int bid = blockIdx.x;
+    const int i = find_instance(D, n, bid, [](const BatchDesc &d) { return blocks_cell(d, R); });
+    if (i < 0) return;
+    const BatchDesc &d = D[i];
+    const CellArgs a = d.ca;
+    const int gx = ceil_div(a.nu, kWave), gy = ceil_div(a.nv, R * kWavesPerBlock);
+    const int bx = bid % gx, by = (bid / gx) % gy, bz = bid / (gx * gy);
+    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+    const int c = bx * kWave + lane;
+    const int r0 = (by * kWavesPerBlock + wave) * R;
+    if (c >= a.nu || r0 >= a.nv) return;
+    cell_body<OPK, GEO, R>(d.hdr, d.lineC, d.col, reinterpret_cast<const float2 *>(ws + d.ws_volx),
+                           reinterpret_cast<const float2 *>(ws + d.ws_voly), io.out[i], a, c, r0, bz % a.nview,
+                           bz / a.nview);
+}
+
+template <int MODE, int GEO>
+__global__ __launch_bounds__(kBlock) void k_sino_prefix_batch(const BatchDesc *__restrict__ D, int n, IOPtrs io,
+                                                              char *__restrict__ ws)
+{
+    int bid = blockIdx.x;
+    const int i = find_instance(D, n, bid, [](const BatchDesc &d) { return blocks_sino_prefix(d); });
+    if (i < 0) return;
+    const BatchDesc &d = D[i];
+    const int gx = ceil_div(d.sa.nu, kBlock);
+    const int c = (bid % gx) * kBlock + threadIdx.x;
+    const int v = (bid / gx) % d.sa.nview, b = bid / (gx * d.sa.nview);
+    if (c >= d.sa.nu) return;
+    sino_prefix_body<MODE, GEO>(d.hdr, d.col, io.in[i], reinterpret_cast<float2 *>(ws + d.ws_sino), d.sa, b, v, c);
+}
+
+template <int OPK, int GEO, int KC>
+__global__ __launch_bounds__(kBlock) void k_voxel_batch(const BatchDesc *__restrict__ D, int n, IOPtrs io,
+                                                        char *__restrict__ ws)
+{
+    int bid = blockIdx.x;
+    const int i = find_instance(D, n, bid, [](const BatchDesc &d) { return blocks_voxel(d, KC); });
+    if (i < 0) return;
+    const BatchDesc &d = D[i];
+    VoxArgs a = d.va;
+    a.nchunk = ceil_div(a.nz, KC);
+    const int gx = ceil_div(a.nx, kWave), gy = ceil_div(a.ny, kWavesPerBlock);
+    const int bx = bid % gx, by = (bid / gx) % gy, bz = bid / (gx * gy);
+    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+    const int ii = bx * kWave + lane, j = by * kWavesPerBlock + wave;
+    if (ii >= a.nx || j >= a.ny) return;
+    voxel_body<OPK, GEO, KC>(d.hdr, d.lineV, d.col, reinterpret_cast<const float2 *>(ws + d.ws_sino), io.out[i], a,
+                             ii, j, (bz % a.nchunk) * KC, bz / a.nchunk);
+}
+
+template <int OPK, int GEO>
+void launch_cell_batch(int R, int grid, const BatchDesc *D, int n, const IOPtrs &io, char *ws, hipStream_t s)
+{
+    switch (R) {
+    case 1: hipLaunchKernelGGL((k_cell_batch<OPK, GEO, 1>), dim3(grid), dim3(kBlock), 0, s, D, n, io, ws); break;
+    case 2: hipLaunchKernelGGL((k_cell_batch<OPK, GEO, 2>), dim3(grid), dim3(kBlock), 0, s, D, n, io, ws); break;
+    case 4: hipLaunchKernelGGL((k_cell_batch<OPK, GEO, 4>), dim3(grid), dim3(kBlock), 0, s, D, n, io, ws); break;
+    default: hipLaunchKernelGGL((k_cell_batch<OPK, GEO, 8>), dim3(grid), dim3(kBlock), 0, s, D, n, io, ws); break;
+    }
+}
+
+template <int OPK, int GEO>
+void launch_voxel_batch(int KC, int grid, const BatchDesc *D, int n, const IOPtrs &io, char *ws, hipStream_t s)
+{
+    switch (KC) {
+    case 1: hipLaunchKernelGGL((k_voxel_batch<OPK, GEO, 1>), dim3(grid), dim3(kBlock), 0, s, D, n, io, ws); break;
+    case 4: hipLaunchKernelGGL((k_voxel_batch<OPK, GEO, 4>), dim3(grid), dim3(kBlock), 0, s, D, n, io, ws); break;
+    case 8: hipLaunchKernelGGL((k_voxel_batch<OPK, GEO, 8>), dim3(grid), dim3(kBlock), 0, s, D, n, io, ws); break;
+    default: hipLaunchKernelGGL((k_voxel_batch<OPK, GEO, 16>), dim3(grid), dim3(kBlock), 0, s, D, n, io, ws); break;
+    }
+}
+
+template <int GEO>
+void dispatch_gather(const Route &rt, int R, int KC, int grid, const BatchDesc *D, int n, const IOPtrs &io, char *ws,
+                     hipStream_t s)
+{
+    if (rt.cell) {
+        if (rt.kop == CELL_A) launch_cell_batch<CELL_A, GEO>(R, grid, D, n, io, ws, s);
+        else if (rt.kop == CELL_BT) launch_cell_batch<CELL_BT, GEO>(R, grid, D, n, io, ws, s);
+        else launch_cell_batch<CELL_BWT, GEO>(R, grid, D, n, io, ws, s);
+    } else {
+        if (rt.kop == VOX_B) launch_voxel_batch<VOX_B, GEO>(KC, grid, D, n, io, ws, s);
+        else if (rt.kop == VOX_BW) launch_voxel_batch<VOX_BW, GEO>(KC, grid, D, n, io, ws, s);
+        else launch_voxel_batch<VOX_AT, GEO>(KC, grid, D, n, io, ws, s);
+    }
+}
+
+template <int GEO>
+void dispatch_sino_prefix(bool at, int grid, const BatchDesc *D, int n, const IOPtrs &io, char *ws, hipStream_t s)
+{
+    if (at) hipLaunchKernelGGL((k_sino_prefix_batch<SINO_AT, GEO>), dim3(grid), dim3(kBlock), 0, s, D, n, io, ws);
+    else hipLaunchKernelGGL((k_sino_prefix_batch<SINO_PLAIN, GEO>), dim3(grid), dim3(kBlock), 0, s, D, n, io, ws);
+}
+
+}  // namespace
+
+}  // namespace xr
+
+struct xr_batch3d {
+    std::vector<const xr::Plan3d *> plans;
+    std::vector<xr::BatchDesc> desc;
+    xr::BatchDesc *d_desc = nullptr;
+    int device = 0;
+    size_t ws_cell = 0, ws_vox = 0;
+};
+
+using xr::fail;
+
+extern "C" {
+
+int xr_batch3d_create(xr_plan3d *const *plans, int n, const int *num_batch, xr_batch3d **out)
+{
+    xr::set_error("");
+    if (!out) return fail(XR_EINVAL, "null output batch pointer");
+    *out = nullptr;
+    if (!plans || !num_batch || n <= 0 || n > XR_BATCH_MAX)
+        return fail(XR_EINVAL, "a batch holds 1.." + std::to_string(XR_BATCH_MAX) + " plans");
+    std::unique_ptr<xr_batch3d> b(new xr_batch3d());
+    size_t off_cell = 0, off_vox = 0;
+    for (int i = 0; i < n; ++i) {
+        if (!plans[i]) return fail(XR_EINVAL, "null plan in batch");
+        const xr::Plan3d &p = plans[i]->p;
+        if (num_batch[i] <= 0) return fail(XR_EINVAL, "batch counts must be positive");
+        if (i > 0) {
+            const xr::Plan3d &p0 = plans[0]->p;
+            if (p.kind != p0.kind || p.is2d != p0.is2d)
+                return fail(XR_EINVAL, "a batch must use one geometry family (all flat, all arc or all parallel)");
+            if (p.device != p0.device) return fail(XR_EINVAL, "batched plans must live on one device");
+        }
+        if (p.nview <= 0) return fail(XR_EINVAL, "batched plans need at least one view");
+        xr::BatchDesc d{};
+        d.hdr = p.d_hdr;
+        d.lineC = p.d_lineC;
+        d.lineV = p.d_lineV;
+        d.col = p.d_col;
+        d.ca = xr::make_cell_args(p, num_batch[i]);
+        d.va = xr::make_vox_args(p, num_batch[i], 1);
+        d.sa = xr::make_sino_args(p, num_batch[i]);
+        d.nb = num_batch[i];
+        d.any_y = p.any_y;
+        const size_t vb = xr::align256(xr::vol_prefix_bytes(p, num_batch[i]));
+        d.ws_volx = off_cell;
+        d.ws_voly = off_cell + vb;
+        off_cell += vb * (p.any_y ? 2 : 1);
+        d.ws_sino = off_vox;
+        off_vox += xr::align256(xr::sino_prefix_bytes(p, num_batch[i]));
+        b->desc.push_back(d);
+        b->plans.push_back(&p);
+    }
+    b->ws_cell = off_cell;
+    b->ws_vox = off_vox;
+    b->device = plans[0]->p.device;
+    int prev = -1;
+    (void)hipGetDevice(&prev);
+    if (prev != b->device) (void)hipSetDevice(b->device);
+    hipError_t e = hipMalloc(&b->d_desc, sizeof(xr::BatchDesc) * n);
+    if (e == hipSuccess)
+        e = hipMemcpy(b->d_desc, b->desc.data(), sizeof(xr::BatchDesc) * n, hipMemcpyHostToDevice);
+    if (prev >= 0 && prev != b->device) (void)hipSetDevice(prev);
+    if (e != hipSuccess) {
+        if (b->d_desc) (void)hipFree(b->d_desc);
+        return fail(XR_ENOMEM, std::string("batch descriptor upload failed: ") + hipGetErrorString(e));
+    }
+    *out = b.release();
+    return XR_OK;
+}
+
+void xr_batch3d_destroy(xr_batch3d *b)
+{
+    if (!b) return;
+    if (b->d_desc) (void)hipFree(b->d_desc);
+    delete b;
+}
+
+size_t xr_batch3d_workspace_bytes(const xr_batch3d *b, int op)
+{
+    if (!b) return 0;
+    const xr::Route rt = xr::route(*b->plans[0], op);
+    if (rt.kop < 0) return 0;
+    return rt.cell ? b->ws_cell : b->ws_vox;
+}
+
+int xr_batch3d_execute(const xr_batch3d *b, int op, const float *const *in, float *const *out, void *workspace,
+                       size_t workspace_bytes, int tile_rows, int voxel_chunk, void *stream)
+{
+    if (!b) return fail(XR_EINVAL, "null batch");
+    if (!in || !out) return fail(XR_EINVAL, "null input/output arrays");
+    const int n = (int)b->desc.size();
+    const xr::Plan3d &p0 = *b->plans[0];
+    const xr::Route rt = xr::route(p0, op);
+    if (rt.kop < 0) return fail(XR_EINVAL, "unknown operator " + std::to_string(op));
+    const size_t need = rt.cell ? b->ws_cell : b->ws_vox;
+    if (workspace_bytes < need || !workspace)
+        return fail(XR_EWORKSPACE, "workspace of " + std::to_string(workspace_bytes) + " bytes, need " +
+                                       std::to_string(need));
+    xr::IOPtrs io{};
+    for (int i = 0; i < n; ++i) {
+        if (!in[i] || !out[i]) return fail(XR_EINVAL, "null buffer for batch instance " + std::to_string(i));
+        io.in[i] = in[i];
+        io.out[i] = out[i];
+    }
+    int R = tile_rows, KC = voxel_chunk;
+    if (R == 0) {
+        R = 1;
+        for (const xr::Plan3d *p : b->plans) R = std::max(R, xr::default_rows(*p));
+    }
+    if (KC == 0) {
+        KC = 1;
+        for (const xr::Plan3d *p : b->plans) KC = std::max(KC, xr::default_chunk(*p));
+    }
+    if (!xr::valid_rows(R)) return fail(XR_EINVAL, "tile_rows must be 1, 2, 4 or 8");
+    if (!xr::valid_chunk(KC)) return fail(XR_EINVAL, "voxel_chunk must be 1, 4, 8 or 16");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    char *ws = static_cast<char *>(workspace);
+    const xr::BatchDesc *D = b->d_desc;
+    int64_t g_pre = 0, g_tr = 0, g_gather = 0;
+    for (const xr::BatchDesc &d : b->desc) {
+        if (rt.cell) {
+            g_pre += xr::blocks_vol_prefix(d);
+            g_tr += xr::blocks_transpose(d);
+            g_gather += xr::blocks_cell(d, R);
+        } else {
+            g_pre += xr::blocks_sino_prefix(d);
+            g_gather += xr::blocks_voxel(d, KC);
+        }
+    }
+    if (g_pre >= INT32_MAX || g_tr >= INT32_MAX || g_gather >= INT32_MAX)
+        return fail(XR_EINVAL, "batch too large for one launch");
+    const int kind = p0.kind;
+    if (rt.cell) {
+        hipLaunchKernelGGL(xr::k_vol_prefix_batch, dim3((int)g_pre), dim3(xr::kBlock), 0, s, D, n, io, ws);
+        if (g_tr > 0) hipLaunchKernelGGL(xr::k_transpose_batch, dim3((int)g_tr), dim3(xr::kBlock), 0, s, D, n, ws);
+    } else {
+        const bool at = rt.kop == xr::VOX_AT;
+        if (kind == xr::GEO_FLAT) xr::dispatch_sino_prefix<xr::GEO_FLAT>(at, (int)g_pre, D, n, io, ws, s);
+        else if (kind == xr::GEO_ARC) xr::dispatch_sino_prefix<xr::GEO_ARC>(at, (int)g_pre, D, n, io, ws, s);
+        else xr::dispatch_sino_prefix<xr::GEO_PAR>(at, (int)g_pre, D, n, io, ws, s);
+    }
+    if (kind == xr::GEO_FLAT) xr::dispatch_gather<xr::GEO_FLAT>(rt, R, KC, (int)g_gather, D, n, io, ws, s);
+    else if (kind == xr::GEO_ARC) xr::dispatch_gather<xr::GEO_ARC>(rt, R, KC, (int)g_gather, D, n, io, ws, s);
+    else xr::dispatch_gather<xr::GEO_PAR>(rt, R, KC, (int)g_gather, D, n, io, ws, s);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(XR_EHIP, std::string("batched launch failed: ") + hipGetErrorString(e));
+    return XR_OK;
+}
+
+}  // extern "C"

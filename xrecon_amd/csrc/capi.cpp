@@ -17,10 +17,6 @@ const char *last_error();
 using xr::fail;
 using xr::Plan3d;
 
-struct xr_plan3d {
-    Plan3d p;
-};
-
 namespace {
 
 const int kType2d = 0x100;  // flag OR-ed into xr_geometry3d.type by xr_geometry2d_as_3d
@@ -181,33 +177,6 @@ int xr_plan3d_view_flags(const xr_plan3d *plan, int *xdriven, int *keep_order)
         if (keep_order) keep_order[v] = plan->p.keep[v];
     }
     return XR_OK;
-}
-
-size_t xr_batch3d_workspace_bytes(xr_plan3d *const *plans, int num_instances, int op, const int *num_batch)
-{
-    if (!plans || num_instances <= 0 || !num_batch) return 0;
-    std::vector<xr::BatchItem> items;
-    for (int i = 0; i < num_instances; ++i) {
-        if (!plans[i]) return 0;
-        items.push_back({&plans[i]->p, nullptr, nullptr, num_batch[i]});
-    }
-    return xr::batch_workspace_bytes(items, op);
-}
-
-int xr_batch3d_execute(xr_plan3d *const *plans, int num_instances, int op, const float *const *in,
-                       float *const *out, const int *num_batch, void *workspace, size_t workspace_bytes,
-                       int tile_rows, int voxel_chunk, void *stream)
-{
-    if (!plans || !in || !out || !num_batch || num_instances < 0) return fail(XR_EINVAL, "null batch arrays");
-    std::vector<xr::BatchItem> items;
-    for (int i = 0; i < num_instances; ++i) {
-        if (!plans[i]) return fail(XR_EINVAL, "null plan in batch");
-        if (i > 0 && plans[i]->p.device != plans[0]->p.device)
-            return fail(XR_EINVAL, "batched plans must live on one device");
-        items.push_back({&plans[i]->p, in[i], out[i], num_batch[i]});
-    }
-    return xr::execute_batch(items, op, workspace, workspace_bytes, tile_rows, voxel_chunk,
-                             static_cast<hipStream_t>(stream));
 }
 
 static int one_shot(int op, const float *in, float *out, const float *angles_host, const xr_volume3d *vol,

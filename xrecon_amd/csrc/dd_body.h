@@ -68,6 +68,44 @@ inline Route route(const Plan3d &p, int op)
     return {true, -1};
 }
 
+__host__ __device__ inline int ceil_div(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }
+
+inline size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
+
+// Workspace of the cell gathers: z-prefix of the volume in one orientation.
+inline size_t vol_prefix_bytes(const Plan3d &p, int nbatch)
+{
+    return (size_t)nbatch * (p.vol.nz + 1) * (size_t)p.vol.nx * p.vol.ny * sizeof(float2);
+}
+
+// Workspace of the voxel gathers: row-prefix of the sinogram.
+inline size_t sino_prefix_bytes(const Plan3d &p, int nbatch)
+{
+    return (size_t)nbatch * p.nview * (size_t)(p.geo.nv + 1) * p.geo.nu * sizeof(float2);
+}
+
+// Default tile shapes (the autotuner may choose others).
+inline int default_rows(const Plan3d &p)
+{
+    const int nv = p.geo.nv;
+    if (nv >= 32) return 8;
+    if (nv >= 16) return 4;
+    if (nv >= 4) return 2;
+    return 1;
+}
+
+inline int default_chunk(const Plan3d &p)
+{
+    const int nz = p.vol.nz;
+    if (nz >= 64) return 16;
+    if (nz >= 8) return 8;
+    if (nz >= 4) return 4;
+    return 1;
+}
+
+inline bool valid_rows(int r) { return r == 1 || r == 2 || r == 4 || r == 8; }
+inline bool valid_chunk(int k) { return k == 1 || k == 4 || k == 8 || k == 16; }
+
 inline CellArgs make_cell_args(const Plan3d &p, int nbatch)
 {
     CellArgs a;

@@ -116,38 +116,6 @@ __global__ __launch_bounds__(kBlock) void k_voxel(const ViewHdr *__restrict__ hd
     voxel_body<OPK, GEO, KC>(hdr, lineV, colt, Q, out, a, i, j, chunk * KC, b);
 }
 
-inline int ceil_div(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }
-
-inline size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
-
-inline size_t vol_prefix_bytes(const Plan3d &p, int nbatch)
-{
-    return (size_t)nbatch * (p.vol.nz + 1) * (size_t)p.vol.nx * p.vol.ny * sizeof(float2);
-}
-
-inline size_t sino_prefix_bytes(const Plan3d &p, int nbatch)
-{
-    return (size_t)nbatch * p.nview * (size_t)(p.geo.nv + 1) * p.geo.nu * sizeof(float2);
-}
-
-int default_rows(const Plan3d &p)
-{
-    const int nv = p.geo.nv;
-    if (nv >= 32) return 8;
-    if (nv >= 16) return 4;
-    if (nv >= 4) return 2;
-    return 1;
-}
-
-int default_chunk(const Plan3d &p)
-{
-    const int nz = p.vol.nz;
-    if (nz >= 64) return 16;
-    if (nz >= 8) return 8;
-    if (nz >= 4) return 4;
-    return 1;
-}
-
 template <int OPK, int GEO>
 void launch_cell(const Plan3d &p, int R, const float2 *WX, const float2 *WY, float *out, int nbatch, hipStream_t s)
 {
@@ -287,23 +255,6 @@ int execute_phase(const Plan3d &p, int op, int phase, const float *in, float *ou
     }
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(XR_EHIP, std::string("kernel launch failed: ") + hipGetErrorString(e));
-    return XR_OK;
-}
-
-size_t batch_workspace_bytes(const std::vector<BatchItem> &items, int op)
-{
-    size_t need = 0;
-    for (const BatchItem &it : items) need = std::max(need, workspace_bytes(*it.plan, op, it.nbatch));
-    return need;
-}
-
-int execute_batch(const std::vector<BatchItem> &items, int op, void *ws, size_t ws_bytes, int tile_rows,
-                  int voxel_chunk, hipStream_t s)
-{
-    for (const BatchItem &it : items) {
-        const int rc = execute(*it.plan, op, it.in, it.out, it.nbatch, ws, ws_bytes, tile_rows, voxel_chunk, s);
-        if (rc) return rc;
-    }
     return XR_OK;
 }
 

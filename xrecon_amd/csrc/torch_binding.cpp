@@ -193,6 +193,59 @@ private:
     xr_plan3d *plan_ = nullptr;
 };
 
+// A batch of plans executed by single launches (xr_batch3d_*).
+class Batch {
+public:
+    Batch(const std::vector<std::shared_ptr<Plan>> &plans, const std::vector<int> &num_batch)
+        : plans_(plans), nb_(num_batch)
+    {
+        TORCH_CHECK(plans.size() == num_batch.size(), "one batch count per plan");
+        std::vector<xr_plan3d *> h;
+        for (const auto &p : plans) h.push_back(reinterpret_cast<xr_plan3d *>(p->handle()));
+        const int rc = xr_batch3d_create(h.data(), (int)h.size(), num_batch.data(), &batch_);
+        if (rc) raise_xr(rc, "xr_batch3d_create");
+    }
+    ~Batch()
+    {
+        if (batch_) xr_batch3d_destroy(batch_);
+    }
+    Batch(const Batch &) = delete;
+    Batch &operator=(const Batch &) = delete;
+
+    int64_t workspace_bytes(int op) const { return (int64_t)xr_batch3d_workspace_bytes(batch_, op); }
+
+    void execute_into(int op, const std::vector<torch::Tensor> &xs, std::vector<torch::Tensor> &outs,
+                      const torch::Tensor &ws, int64_t stream, int tile_rows, int voxel_chunk) const
+    {
+        TORCH_CHECK(xs.size() == plans_.size() && outs.size() == plans_.size(), "one input and output per plan");
+        std::vector<const float *> in;
+        std::vector<float *> out;
+        for (size_t i = 0; i < xs.size(); ++i) {
+            plans_[i]->check_input(op, xs[i]);
+            TORCH_CHECK(xs[i].size(0) * xs[i].size(1) == nb_[i], "instance ", i, " batch count differs from the batch");
+            const auto want = volume_in(op) ? plans_[i]->sino_shape(xs[i].size(0), xs[i].size(1))
+                                            : plans_[i]->volume_shape(xs[i].size(0), xs[i].size(1));
+            TORCH_CHECK(outs[i].is_cuda() && outs[i].is_contiguous() && outs[i].scalar_type() == torch::kFloat32 &&
+                            outs[i].sizes().vec() == want,
+                        "output ", i, " must be a contiguous float32 GPU tensor of shape ", want);
+            in.push_back(xs[i].data_ptr<float>());
+            out.push_back(outs[i].data_ptr<float>());
+        }
+        const size_t need = xr_batch3d_workspace_bytes(batch_, op);
+        TORCH_CHECK(ws.defined() && ws.is_cuda() && (size_t)ws.numel() * ws.element_size() >= need,
+                    "batch workspace needs ", need, " bytes");
+        const int rc = xr_batch3d_execute(batch_, op, in.data(), out.data(), ws.data_ptr(),
+                                          (size_t)ws.numel() * ws.element_size(), tile_rows, voxel_chunk,
+                                          reinterpret_cast<void *>(stream));
+        if (rc) raise_xr(rc, "xr_batch3d_execute");
+    }
+
+private:
+    std::vector<std::shared_ptr<Plan>> plans_;
+    std::vector<int> nb_;
+    xr_batch3d *batch_ = nullptr;
+};
+
 // Reference-named operator functions: (input, options, angles[, stream]) -> output, as in
 // src/api/torch_binding.cpp:407-523 (3D) and :139-351 (2D).  They build a plan per call
 // (host geometry tables); xrecon_amd.Projector3d/Projector2d cache plans instead.
@@ -238,6 +291,16 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m)
         .def("execute_into", &Plan::execute_into, py::arg("op"), py::arg("input"), py::arg("out"),
              py::arg("workspace"), py::arg("stream") = 0, py::arg("tile_rows") = 0, py::arg("voxel_chunk") = 0,
              py::arg("phase") = XR_PHASE_ALL);
+
+    py::class_<Batch, std::shared_ptr<Batch>>(m, "Batch")
+        .def(py::init([](const std::vector<std::shared_ptr<Plan>> &plans, const std::vector<int> &num_batch) {
+                 return std::make_shared<Batch>(plans, num_batch);
+             }),
+             py::arg("plans"), py::arg("num_batch"))
+        .def("workspace_bytes", &Batch::workspace_bytes)
+        .def("execute_into", &Batch::execute_into, py::arg("op"), py::arg("inputs"), py::arg("outputs"),
+             py::arg("workspace"), py::arg("stream") = 0, py::arg("tile_rows") = 0, py::arg("voxel_chunk") = 0);
+    m.attr("BATCH_MAX") = XR_BATCH_MAX;
 
     m.attr("OP_PROJECTION") = XR_OP_PROJECTION;
     m.attr("OP_PROJECTION_T") = XR_OP_PROJECTION_T;

@@ -88,14 +88,10 @@ int kernel_kind(const Plan3d &p, int op);
 int execute_phase(const Plan3d &p, int op, int phase, const float *in, float *out, int nbatch,
                   void *ws, size_t ws_bytes, int tile_rows, int voxel_chunk, hipStream_t stream);
 
-struct BatchItem {
-    const Plan3d *plan;
-    const float *in;
-    float *out;
-    int nbatch;
-};
-size_t batch_workspace_bytes(const std::vector<BatchItem> &items, int op);
-int execute_batch(const std::vector<BatchItem> &items, int op, void *ws, size_t ws_bytes,
-                  int tile_rows, int voxel_chunk, hipStream_t stream);
 
 }  // namespace xr
+
+// The opaque C-ABI handle (include/xrecon_amd.h).
+struct xr_plan3d {
+    xr::Plan3d p;
+};
