@@ -204,6 +204,25 @@ int xr_op2d(int op, const float *in, float *out, const float *angles_host, const
 int xr_geometry2d_as_3d(const xr_volume2d *vol2, const xr_geometry2d *geo2, xr_volume3d *vol3,
                         xr_geometry3d *geo3);
 
+/* ------------------------------------------------------- system matrix (host) */
+
+/* COO triplets of the forward projector A: rows index the sinogram
+ * ((view * Nv + row) * Nu + col), columns the volume ((z * Ny + y) * Nx + x); values that
+ * are 0 or not finite are dropped.  Built on the host by the reference's own FP32
+ * detector-driven traversal so that the index and sparsity pattern are identical to the
+ * reference's (include/functions.h:296-323 system_matrix2d_cpu / system_matrix3d_cpu,
+ * src/cpu/system_matrix3d.cpp:39-62, src/cpu/forward3d.cpp:611-658,
+ * src/cpu/system_matrix2d.cpp:359-389).  Triplets come in traversal order, not
+ * coalesced. */
+typedef struct xr_coo xr_coo;
+int xr_system_matrix3d(const xr_volume3d *vol, const xr_geometry3d *geo, const float *angles_host,
+                       int num_view, xr_coo **out);
+int xr_system_matrix2d(const xr_volume2d *vol, const xr_geometry2d *geo, const float *angles_host,
+                       int num_view, xr_coo **out);
+int64_t xr_coo_nnz(const xr_coo *coo);
+int xr_coo_copy(const xr_coo *coo, int64_t *rows, int64_t *cols, float *vals);
+void xr_coo_destroy(xr_coo *coo);
+
 #ifdef __cplusplus
 }
 #endif

@@ -263,7 +263,43 @@ torch::Tensor run_op(int op, bool is2d, const torch::Tensor &x, const torch::Ten
     return plan.execute(op, x, stream);
 }
 
-#define XR_DEF_OP(name, op, is2d)                                                                            \
+// system_matrix2d / system_matrix3d (src/api/torch_binding.cpp:353-405): CPU tensors in,
+// coalesced sparse COO [V*Nv*Nu, D*H*W] out.
+torch::Tensor system_matrix(bool is2d, const torch::Tensor &options, const torch::Tensor &ang)
+{
+    TORCH_CHECK(!options.is_cuda() && !ang.is_cuda(), is2d ? "system_matrix2d builds the matrix on CPU"
+                                                           : "system_matrix3d builds the matrix on CPU");
+    TORCH_CHECK(options.scalar_type() == torch::kFloat32, "options must be float32");
+    TORCH_CHECK(ang.scalar_type() == torch::kFloat32, "ang must be float32");
+    TORCH_CHECK(options.is_contiguous() && ang.is_contiguous(), "Expected all tensors to be contiguous");
+    const auto o = host_floats(options, is2d ? 12 : 18, "Option");
+    const int V = (int)ang.numel();
+    xr_coo *coo = nullptr;
+    int rc;
+    int64_t nrows, ncols;
+    if (is2d) {
+        const int type = (int)o[11];
+        xr_volume2d v2{(int)o[0], (int)o[1], o[3], o[4], o[8], o[9]};
+        xr_geometry2d g2{type, (int)o[2], o[5], o[10], o[6], o[7]};
+        rc = xr_system_matrix2d(&v2, &g2, ang.data_ptr<float>(), V, &coo);
+        nrows = (int64_t)V * g2.nu;
+        ncols = (int64_t)v2.nx * v2.ny;
+    } else {
+        const Desc d = unpack3d(o);
+        rc = xr_system_matrix3d(&d.vol, &d.geo, ang.data_ptr<float>(), V, &coo);
+        nrows = (int64_t)V * d.Nv * d.Nu;
+        ncols = (int64_t)d.D * d.H * d.W;
+    }
+    if (rc) raise_xr(rc, is2d ? "xr_system_matrix2d" : "xr_system_matrix3d");
+    const int64_t n = xr_coo_nnz(coo);
+    auto rows = torch::empty({n}, torch::kInt64), cols = torch::empty({n}, torch::kInt64);
+    auto vals = torch::empty({n}, torch::kFloat32);
+    xr_coo_copy(coo, rows.data_ptr<int64_t>(), cols.data_ptr<int64_t>(), vals.data_ptr<float>());
+    xr_coo_destroy(coo);
+    return torch::sparse_coo_tensor(torch::stack({rows, cols}), vals, {nrows, ncols}).coalesce();
+}
+
+#define XR_DEF_OP(name, op, is2d)                                                                         \
     m.def(                                                                                                   \
         name,                                                                                                \
         [](const torch::Tensor &x, const torch::Tensor &o, const torch::Tensor &a, int64_t stream) {         \
@@ -310,6 +346,10 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m)
     m.attr("OP_BACKPROJECTION_W_T") = XR_OP_BACKPROJECTION_W_T;
     m.def("version", []() { return std::string(xr_version()); });
 
+    m.def("system_matrix2d", [](const torch::Tensor &o, const torch::Tensor &a) { return system_matrix(true, o, a); },
+          "2D distance-driven projection matrix");
+    m.def("system_matrix3d", [](const torch::Tensor &o, const torch::Tensor &a) { return system_matrix(false, o, a); },
+          "3D distance-driven projection matrix");
     XR_DEF_OP("forward_projection2d", XR_OP_PROJECTION, true);
     XR_DEF_OP("backward_projection2d", XR_OP_BACKPROJECTION, true);
     XR_DEF_OP("weighted_backward_projection2d", XR_OP_BACKPROJECTION_W, true);
