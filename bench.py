@@ -49,7 +49,108 @@ def parse():
     ap.add_argument("--summarize", nargs="+", metavar="DB",
                     help="only print a per-kernel summary of rocprofv3 SQLite result files")
     ap.add_argument("--verbose", action="store_true", help="print autotuner timings")
+    ap.add_argument("--no-batched", action="store_true", help="skip the 16-instance batched section")
     return ap.parse_args()
+
+
+def batched_instances(seed=1003):
+    """16 uneven cone-flat instances: volumes from {128^2x32, 192^2x48, 256^2x64, 384^2x96}
+    at 1 mm, flat detectors of 2W columns x D rows at 1 mm pitch, SOD 500 / SDD 1000 mm,
+    V uniform in 180..984 over 2 pi with a random start angle per instance."""
+    rng = np.random.default_rng(seed)
+    shapes = [(128, 32), (192, 48), (256, 64), (384, 96)]
+    out = []
+    for i in range(16):
+        W, D = shapes[int(rng.integers(0, 4))]
+        V = int(rng.integers(180, 985))
+        start = np.float32(rng.uniform(0, 2 * np.pi))
+        out.append(dict(W=W, D=D, Nu=2 * W, Nv=D, V=V, start=start, seed=seed * 100 + i))
+    return out
+
+
+def bench_batched(dev, steps, verbose=False):
+    """A + B over the 16 uneven instances: the plain per-instance loop (each instance's own
+    autotuned tiles) vs one batched launch per pass vs the batched passes replayed from a
+    captured HIP graph.  Synthetic phantoms: seeded random ellipsoids."""
+    import torch
+
+    import xrecon_amd
+    from xrecon_amd import _backend as be
+    from xrecon_amd import autotune
+    from xrecon_amd.batched import BatchedProjector3d
+    from xrecon_amd.phantom import random_ellipsoids
+
+    A, B = be.OP_PROJECTION, be.OP_BACKPROJECTION
+    inst = batched_instances()
+    projs, angs, vols = [], [], []
+    for it in inst:
+        W, D = it["W"], it["D"]
+        projs.append(xrecon_amd.Projector3d((W, W, D), (it["Nu"], it["Nv"]), 1.0, 1.0, 500.0, 1000.0))
+        angs.append(it["start"] + torch.arange(it["V"], dtype=torch.float32) * np.float32(2 * np.pi / it["V"]))
+        ph = random_ellipsoids((W, W, D), (1.0, 1.0, 1.0), count=16, semi_axes=(5.0, 80.0), seed=it["seed"])
+        vols.append(torch.from_numpy(ph).to(dev).view(1, 1, D, W, W))
+    stream = torch.cuda.current_stream(dev)
+    sid = stream.cuda_stream
+    bp = BatchedProjector3d(projs, angs, device=dev)
+    vv = bp.voxel_views()
+
+    # plain loop: per-instance plans, workspaces and tuned tiles
+    plans = bp.plans
+    sinos = [torch.empty(bp.sino_shape(i), device=dev) for i in range(16)]
+    recs = [torch.empty(bp.volume_shape(i), device=dev) for i in range(16)]
+    wsa = [torch.empty(max(p.workspace_bytes(A, 1), 1), dtype=torch.uint8, device=dev) for p in plans]
+    wsb = [torch.empty(max(p.workspace_bytes(B, 1), 1), dtype=torch.uint8, device=dev) for p in plans]
+    ta, tb = [], []
+    for i, p in enumerate(plans):
+        p.execute_into(A, vols[i], sinos[i], wsa[i], sid, 0, 0, 0)
+        ta.append(autotune.tune(p, A, vols[i], sinos[i], wsa[i]))
+        tb.append(autotune.tune(p, B, sinos[i], recs[i], wsb[i]))
+
+    def plain():
+        for i, p in enumerate(plans):
+            p.execute_into(A, vols[i], sinos[i], wsa[i], sid, ta[i][0], ta[i][1], 0)
+            p.execute_into(B, sinos[i], recs[i], wsb[i], sid, tb[i][0], tb[i][1], 0)
+
+    bta = bp.tune(A, vols, verbose=verbose)
+    bsino = bp.run(A, vols)
+    btb = bp.tune(B, bsino, verbose=verbose)
+    brec = bp.empty_outputs(B)
+
+    def batched():
+        bp.run(A, vols, bsino)
+        bp.run(B, bsino, brec)
+
+    def timeit(fn, n):
+        fn()
+        torch.cuda.synchronize()
+        s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        s.record(stream)
+        for _ in range(n):
+            fn()
+        e.record(stream)
+        torch.cuda.synchronize()
+        return s.elapsed_time(e) / n
+
+    t_plain = timeit(plain, steps)
+    t_batch = timeit(batched, steps)
+    bp.capture([A, B], vols)
+    t_graph = timeit(bp.replay, steps)
+    # the batched path must reproduce the plain loop with the same tiles
+    same = True
+    for i, p in enumerate(plans):
+        p.execute_into(A, vols[i], sinos[i], wsa[i], sid, bta[0], bta[1], 0)
+    bp.run(A, vols, bsino)
+    torch.cuda.synchronize()
+    same = all(torch.equal(bsino[i], sinos[i]) for i in range(16))
+    return {"workload": "16 uneven cone-flat instances (volumes 128^2x32..384^2x96, detectors 2W x D, "
+                        "V in 180..984, random start angles), A+B",
+            "voxel_views_per_operator": vv,
+            "plain_loop": {"ms": t_plain, "value": vv / (t_plain * 1e-3)},
+            "batched": {"ms": t_batch, "value": vv / (t_batch * 1e-3), "tiles": [bta[0], btb[1]]},
+            "batched_graph": {"ms": t_graph, "value": vv / (t_graph * 1e-3)},
+            "speedup_vs_plain": t_plain / t_graph,
+            "batched_equals_plain_bitwise": bool(same),
+            "unit": "voxel-views/s"}
 
 
 def summarize_rocpd(paths):
@@ -226,6 +327,8 @@ def main():
     if not args.no_cpu:
         cpu = cpu_baseline(list(proj._host_options), phantom, angles.numpy(), args.cpu_views)
 
+    batched = None if args.no_batched else bench_batched(dev, max(2, K // 2), args.verbose)
+
     out = {
         "metric": METRIC,
         "value": value,
@@ -252,6 +355,7 @@ def main():
         "adjoint": {"ms_per_pair": ms_adj, "value": vox_views / (ms_adj * 1e-3), "unit": "voxel-views/s",
                     "pair": "A + A^T"},
         "cpu_baseline": cpu,
+        "batched": batched,
     }
     print(json.dumps(out), flush=True)
 

@@ -77,7 +77,7 @@ void dispatch(const Plan3d &p, const Route &rt, int R, int KC, const float *in, 
 {
     if (rt.cell) {
         const int64_t plane = (int64_t)p.vol.nx * p.vol.ny;
-        std::vector<float2> WX((size_t)nb * (p.vol.nz + 1) * plane), WY(WX.size());
+        std::vector<float2> WX((size_t)nb * (p.vol.nz + 1) * plane + 1), WY(WX.size());  // + zeroed pad entry
         for (int b = 0; b < nb; ++b)
             for (int64_t e = 0; e < plane; ++e) vol_prefix_body(in, WX.data(), plane, p.vol.nz, b, e);
         const int nx = p.vol.nx, ny = p.vol.ny;

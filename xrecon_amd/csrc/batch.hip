@@ -81,6 +81,8 @@ __global__ __launch_bounds__(kBlock) void k_vol_prefix_batch(const BatchDesc *__
     const BatchDesc &d = D[i];
     const int64_t plane = (int64_t)d.ca.nx * d.ca.ny;
     const int64_t idx = (int64_t)bid * kBlock + threadIdx.x;
+    if (idx == 0)
+        reinterpret_cast<float2 *>(ws + d.ws_volx)[plane * (d.ca.nz + 1) * d.nb] = make_float2(0.0f, 0.0f);
     if (idx >= plane * d.nb) return;
     const int64_t b = idx / plane;
     vol_prefix_body(io.in[i], reinterpret_cast<float2 *>(ws + d.ws_volx), plane, d.ca.nz, b, idx - b * plane);
@@ -98,6 +100,8 @@ __global__ __launch_bounds__(kBlock) void k_transpose_batch(const BatchDesc *__r
     const int tx_n = ceil_div(nx, 32), ty_n = ceil_div(ny, 32);
     const int bx = bid % tx_n, by = (bid / tx_n) % ty_n, p = bid / (tx_n * ty_n);
     const int64_t plane = (int64_t)nx * ny;
+    if (bid == 0 && threadIdx.x == 0)
+        reinterpret_cast<float2 *>(ws + d.ws_voly)[plane * (d.ca.nz + 1) * d.nb] = make_float2(0.0f, 0.0f);
     const float2 *src = reinterpret_cast<const float2 *>(ws + d.ws_volx) + (int64_t)p * plane;
     float2 *dst = reinterpret_cast<float2 *>(ws + d.ws_voly) + (int64_t)p * plane;
     const int i0 = bx * 32, j0 = by * 32;

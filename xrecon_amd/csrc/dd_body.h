@@ -72,10 +72,17 @@ __host__ __device__ inline int ceil_div(int64_t a, int64_t b) { return (int)((a 
 
 inline size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
 
-// Workspace of the cell gathers: z-prefix of the volume in one orientation.
+// Workspace of the cell gathers: z-prefix of the volume in one orientation, plus one
+// zeroed pad entry after the table (the two-pixel loads of the cell gather may read one
+// entry past the last pixel, with weight zero).
+inline size_t vol_prefix_entries(const Plan3d &p, int nbatch)
+{
+    return (size_t)nbatch * (p.vol.nz + 1) * (size_t)p.vol.nx * p.vol.ny;
+}
+
 inline size_t vol_prefix_bytes(const Plan3d &p, int nbatch)
 {
-    return (size_t)nbatch * (p.vol.nz + 1) * (size_t)p.vol.nx * p.vol.ny * sizeof(float2);
+    return (vol_prefix_entries(p, nbatch) + 1) * sizeof(float2);
 }
 
 // Workspace of the voxel gathers: row-prefix of the sinogram.
@@ -145,6 +152,24 @@ __host__ __device__ inline float clampf(float x, float lo, float hi)
     return fminf(fmaxf(x, lo), hi);
 #endif
 }
+
+// a * b for operands below 2^24: one full-rate v_mul_u32_u24 on the device (a 32-bit
+// v_mul_lo_u32 issues at quarter rate).  Table offsets here stay far below 2^24 per
+// factor: rows/slices <= 2^16, plane strides < 2^24 (checked at plan creation).
+__host__ __device__ inline unsigned mul24(unsigned a, unsigned b)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __umul24(a, b);
+#else
+    return a * b;
+#endif
+}
+
+// Two consecutive float2 table entries (pixels m and m+1 of a line) in one 16-byte load.
+// The entries are only 8-byte aligned, which gfx950 vector memory accesses allow.
+struct __attribute__((aligned(8))) Pair2 {
+    float x0, y0, x1, y1;
+};
 
 // x - floor(x) for x >= 0: one v_fract_f32 on the device.
 __host__ __device__ inline float fractf(float x)
@@ -299,7 +324,7 @@ __host__ __device__ inline void cell_body(const ViewHdr *__restrict__ hdr, const
 #pragma unroll
         for (int q = 0; q <= R; ++q) {
             const float z = clampf(fmaf(hv[q], lc.z, a.zoff), 0.0f, fnz);
-            koff[q] = (unsigned)(int)z * (unsigned)plane;
+            koff[q] = mul24((unsigned)(int)z, (unsigned)plane);
             fr[q] = fractf(z);
         }
         const float2 *Wl = W + L * nalong;
@@ -311,18 +336,20 @@ __host__ __device__ inline void cell_body(const ViewHdr *__restrict__ hdr, const
             const int m1 = has1 ? m0 + 1 : m0;
             const float w0 = pixel_weight(fminf(t1, (float)(m0 + 1)) - fmaxf(t0, (float)m0), lc.w, L, m0);
             const float w1 = has1 ? pixel_weight(fmaxf(t1 - (float)(m0 + 1), 0.0f), lc.w, L, m1) : 0.0f;
-            float2 e0[R + 1], e1[R + 1];
+            // Entries m0 and m0+1 are adjacent: one 16-byte load per boundary.  When
+            // m0 is the last pixel the second half belongs to the next line (or to the
+            // zeroed pad after the table) and is weighted by w1 = 0.
+            (void)m1;
+            const float2 *Wm = Wl + m0;
+            Pair2 e[R + 1];
 #pragma unroll
-            for (int q = 0; q <= R; ++q) {
-                e0[q] = Wl[m0 + koff[q]];
-                e1[q] = Wl[m1 + koff[q]];
-            }
-            float J0 = fmaf(fr[0], e0[0].y, e0[0].x);
-            float J1 = fmaf(fr[0], e1[0].y, e1[0].x);
+            for (int q = 0; q <= R; ++q) e[q] = *reinterpret_cast<const Pair2 *>(Wm + koff[q]);
+            float J0 = fmaf(fr[0], e[0].y0, e[0].x0);
+            float J1 = fmaf(fr[0], e[0].y1, e[0].x1);
 #pragma unroll
             for (int q = 0; q < R; ++q) {
-                const float K0 = fmaf(fr[q + 1], e0[q + 1].y, e0[q + 1].x);
-                const float K1 = fmaf(fr[q + 1], e1[q + 1].y, e1[q + 1].x);
+                const float K0 = fmaf(fr[q + 1], e[q + 1].y0, e[q + 1].x0);
+                const float K1 = fmaf(fr[q + 1], e[q + 1].y1, e[q + 1].x1);
                 acc[q] = fmaf(w0, K0 - J0, fmaf(w1, K1 - J1, acc[q]));
                 J0 = K0;
                 J1 = K1;
@@ -437,7 +464,7 @@ __host__ __device__ inline void voxel_body(const ViewHdr *__restrict__ hdr, cons
 #pragma unroll
                     for (int q = 0; q <= KC; ++q) {
                         const float rho = clampf(fmaf((float)q, step, rlo), 0.0f, fnv);
-                        off[q] = (unsigned)(int)rho * (unsigned)nu;
+                        off[q] = mul24((unsigned)(int)rho, (unsigned)nu);
                         fr[q] = fractf(rho);
                     }
                     float2 e[KC + 1];

@@ -50,6 +50,7 @@ __global__ __launch_bounds__(kBlock) void k_vol_prefix(const float *__restrict__
 {
     const int64_t plane = (int64_t)nx * ny;
     const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx == 0) W[plane * (nz + 1) * nbatch] = make_float2(0.0f, 0.0f);  // pad entry
     if (idx >= plane * nbatch) return;
     const int64_t b = idx / plane;
     vol_prefix_body(f, W, plane, nz, b, idx - b * plane);
@@ -62,6 +63,8 @@ __global__ __launch_bounds__(kBlock) void k_plane_transpose(const float2 *__rest
 {
     __shared__ float2 tile[32][33];
     const int64_t plane = (int64_t)nx * ny;
+    if ((blockIdx.x | blockIdx.y | blockIdx.z | threadIdx.x) == 0)
+        WY[plane * gridDim.z] = make_float2(0.0f, 0.0f);  // pad entry after the last plane
     const float2 *src = WX + (int64_t)blockIdx.z * plane;
     float2 *dst = WY + (int64_t)blockIdx.z * plane;
     const int i0 = blockIdx.x * 32, j0 = blockIdx.y * 32;
