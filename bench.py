@@ -176,7 +176,7 @@ def summarize_rocpd(paths):
         for name in sorted(dur, key=lambda n: -sum(dur[n])):
             d = dur[name]
             ctr = " ".join(f"{c}={sum(v) / len(v):.1f}" for c, v in sorted(pmc[name].items()))
-            short = name.split("(")[0][-70:]
+            short = name.replace("(anonymous namespace)::", "").split("(")[0][-70:]
             print(f"{short:70s} {len(d):5d} {sum(d) / len(d) / 1e3:10.2f} {min(d) / 1e3:10.2f} {max(d) / 1e3:10.2f}  {ctr}")
 
 

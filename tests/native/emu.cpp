@@ -16,8 +16,13 @@ using namespace xr;
 
 namespace {
 
-template <int OPK, int GEO, int R>
-void run_cell_R(const Plan3d &p, const float2 *WX, const float2 *WY, float *out, int nb, int nthreads)
+// Cell-gather variant: 0 = the one the library launches (two-pixel path when every view
+// is narrow, 64-bit offsets for tables of 4 GiB and more), 1 = the general 32-bit path,
+// 2 = the general 64-bit path.
+int g_cell_variant = 0;
+
+template <int OPK, int GEO, int R, bool NARROW, bool BIG>
+void run_cell_RV(const Plan3d &p, const float2 *WX, const float2 *WY, float *out, int nb, int nthreads)
 {
     const CellArgs a = make_cell_args(p, nb);
     const int ngroups = (a.nv + R - 1) / R;
@@ -29,11 +34,20 @@ void run_cell_R(const Plan3d &p, const float2 *WX, const float2 *WY, float *out,
                 const int v = it % p.nview, b = it / p.nview;
                 for (int g = 0; g < ngroups; ++g)
                     for (int c = 0; c < a.nu; ++c)
-                        cell_body<OPK, GEO, R>(p.d_hdr, p.d_lineC, p.d_col, WX, WY, out, a, c, g * R, v, b);
+                        cell_body<OPK, GEO, R, NARROW, BIG>(p.d_hdr, p.d_lineC, p.d_col, WX, WY, out, a, c, g * R, v, b);
             }
         });
     }
     for (auto &x : th) x.join();
+}
+
+template <int OPK, int GEO, int R>
+void run_cell_R(const Plan3d &p, const float2 *WX, const float2 *WY, float *out, int nb, int nt)
+{
+    const int var = g_cell_variant ? g_cell_variant : (p.big ? 2 : p.all_narrow ? 0 : 1);
+    if (var == 2) run_cell_RV<OPK, GEO, R, false, true>(p, WX, WY, out, nb, nt);
+    else if (var == 1) run_cell_RV<OPK, GEO, R, false, false>(p, WX, WY, out, nb, nt);
+    else run_cell_RV<OPK, GEO, R, true, false>(p, WX, WY, out, nb, nt);
 }
 
 template <int OPK, int GEO>
@@ -109,6 +123,8 @@ std::string g_err;
 extern "C" {
 
 const char *emu_error(void) { return g_err.c_str(); }
+
+void emu_set_cell_variant(int v) { g_cell_variant = v; }
 
 // vn = {nx, ny, nz}, vf = {dx, dy, dz, shx, shy, shz}, gn = {type(|0x100 for 2D), nu, nv},
 // gf = {du, dv, shu, shv, sod, sdd}; tile_rows / voxel_chunk 0 = the library defaults

@@ -30,6 +30,7 @@ struct BatchDesc {
     VoxArgs va;  // nchunk is filled per launch from the launch's voxel chunk
     SinoArgs sa;
     int nb, any_y;
+    int narrow, big;  // Plan3d::all_narrow, Plan3d::big
     unsigned long long ws_volx, ws_voly, ws_sino;  // byte offsets into the workspace
 };
 
@@ -132,9 +133,13 @@ __global__ __launch_bounds__(kBlock) void k_cell_batch(const BatchDesc *__restri
     const int c = bx * kWave + lane;
     const int r0 = (by * kWavesPerBlock + wave) * R;
     if (c >= a.nu || r0 >= a.nv) return;
-    cell_body<OPK, GEO, R>(d.hdr, d.lineC, d.col, reinterpret_cast<const float2 *>(ws + d.ws_volx),
-                           reinterpret_cast<const float2 *>(ws + d.ws_voly), io.out[i], a, c, r0, bz % a.nview,
-                           bz / a.nview);
+    const float2 *WX = reinterpret_cast<const float2 *>(ws + d.ws_volx);
+    const float2 *WY = reinterpret_cast<const float2 *>(ws + d.ws_voly);
+    const int v = bz % a.nview, b = bz / a.nview;
+    // the per-instance variant choice is uniform over the workgroup
+    if (d.big) cell_body<OPK, GEO, R, false, true>(d.hdr, d.lineC, d.col, WX, WY, io.out[i], a, c, r0, v, b);
+    else if (d.narrow) cell_body<OPK, GEO, R, true, false>(d.hdr, d.lineC, d.col, WX, WY, io.out[i], a, c, r0, v, b);
+    else cell_body<OPK, GEO, R, false, false>(d.hdr, d.lineC, d.col, WX, WY, io.out[i], a, c, r0, v, b);
 }
 
 template <int MODE, int GEO>
@@ -261,6 +266,8 @@ int xr_batch3d_create(xr_plan3d *const *plans, int n, const int *num_batch, xr_b
         d.sa = xr::make_sino_args(p, num_batch[i]);
         d.nb = num_batch[i];
         d.any_y = p.any_y;
+        d.narrow = p.all_narrow;
+        d.big = p.big;
         const size_t vb = xr::align256(xr::vol_prefix_bytes(p, num_batch[i]));
         d.ws_volx = off_cell;
         d.ws_voly = off_cell + vb;

@@ -45,6 +45,11 @@ int validate(const xr_volume3d *vol, const xr_geometry3d *geo, int num_view)
     // src/kernels/forward3d_flat.cu:140): (Nz+1)*Ny*Nx and (Nv+1)*Nu must fit.
     const int64_t volp = (int64_t)(vol->nz + 1) * vol->ny * vol->nx;
     if (volp >= ((int64_t)1 << 31)) return fail(XR_EINVAL, "volume too large for 32-bit slice indexing");
+    // One view's row-prefix table is addressed with 32-bit byte offsets.
+    if ((int64_t)(geo->nv + 1) * geo->nu * 8 >= ((int64_t)1 << 32))
+        return fail(XR_EINVAL, "detector too large for 32-bit row indexing");
+    if ((int64_t)vol->nx * vol->ny >= ((int64_t)1 << 24))
+        return fail(XR_EINVAL, "volume plane too large (Nx*Ny must stay below 2^24)");
     return XR_OK;
 }
 

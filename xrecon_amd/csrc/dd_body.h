@@ -27,6 +27,7 @@ struct CellArgs {
     int nx, ny, nz, nu, nv, nview, maxline, nbatch;
     float v0, dvv, zoff, sod;
     float dx, dy, shx, shy;
+    unsigned plane8;  // Nx*Ny*8: byte stride of a z-plane (below 2^24 unless the plan is big)
 };
 
 struct VoxArgs {
@@ -34,6 +35,7 @@ struct VoxArgs {
     float zb, dz, rv0, sod;
     float dx, dy, shx, shy;
     float dvv;
+    float hscale;  // dvv / dz
 };
 
 struct SinoArgs {
@@ -120,6 +122,7 @@ inline CellArgs make_cell_args(const Plan3d &p, int nbatch)
     a.nview = p.nview; a.maxline = p.maxline; a.nbatch = nbatch;
     a.v0 = p.v0; a.dvv = p.dvv; a.zoff = p.zoff; a.sod = p.sod;
     a.dx = p.vol.dx; a.dy = p.vol.dy; a.shx = p.vol.shift_x; a.shy = p.vol.shift_y;
+    a.plane8 = (unsigned)((uint64_t)p.vol.nx * p.vol.ny * 8u);
     return a;
 }
 
@@ -132,6 +135,7 @@ inline VoxArgs make_vox_args(const Plan3d &p, int nbatch, int KC)
     a.zb = p.zb; a.dz = p.dz; a.rv0 = p.rv0; a.sod = p.sod;
     a.dx = p.vol.dx; a.dy = p.vol.dy; a.shx = p.vol.shift_x; a.shy = p.vol.shift_y;
     a.dvv = p.dvv;
+    a.hscale = p.dvv / p.dz;
     return a;
 }
 
@@ -178,6 +182,17 @@ __host__ __device__ inline float fractf(float x)
     return __builtin_amdgcn_fractf(x);
 #else
     return x - floorf(x);
+#endif
+}
+
+// x / y where the quotient only seeds a search that is corrected against exact tables:
+// v_rcp_f32 times x on the device (about 1 ulp), a true division on the host.
+__host__ __device__ inline float seed_div(float x, float y)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __fdividef(x, y);
+#else
+    return x / y;
 #endif
 }
 
@@ -270,7 +285,13 @@ __host__ __device__ inline void sino_prefix_body(const ViewHdr *__restrict__ hdr
 // ---------------------------------------------------------------- cell gather
 
 // One thread: internal column c, rows r0..r0+R-1 of view v, batch b.
-template <int OPK, int GEO, int R>
+//
+// NARROW: every view of the plan is narrow (a column overlaps at most two consecutive
+// pixels of a line), so each line takes the fixed two-pixel path.
+// BIG: one batch item's z-prefix table reaches 2^32 bytes or a z-plane 2^24 bytes, so
+// table offsets are taken in 64-bit arithmetic; otherwise they are 32-bit byte offsets
+// (one 24-bit multiply-add) over a wave-uniform line base.
+template <int OPK, int GEO, int R, bool NARROW, bool BIG>
 __host__ __device__ inline void cell_body(const ViewHdr *__restrict__ hdr, const float4 *__restrict__ lineC,
                                           const float4 *__restrict__ colt, const float2 *__restrict__ WX,
                                           const float2 *__restrict__ WY, float *__restrict__ out, const CellArgs &a,
@@ -318,32 +339,35 @@ __host__ __device__ inline void cell_body(const ViewHdr *__restrict__ hdr, const
         const float t0 = fmaf(d0, lc.y, lc.x);
         const float t1 = fmaf(d1, lc.y, lc.x);
         if (!(t1 > 0.0f && t0 < fal)) continue;
-        // Row boundaries in voxel units along z at this line's magnification.
+        // Row boundaries in voxel units along z at this line's magnification, and the
+        // offsets of their z-planes (entries if BIG, else bytes).
         unsigned koff[R + 1];
         float fr[R + 1];
 #pragma unroll
         for (int q = 0; q <= R; ++q) {
             const float z = clampf(fmaf(hv[q], lc.z, a.zoff), 0.0f, fnz);
-            koff[q] = mul24((unsigned)(int)z, (unsigned)plane);
+            koff[q] = mul24((unsigned)(int)z, BIG ? (unsigned)plane : a.plane8);
             fr[q] = fractf(z);
         }
-        const float2 *Wl = W + L * nalong;
-        if (h.narrow) {
+        const float2 *Wl = W + (int64_t)L * nalong;
+        // Entry m of z-plane offset o on this line.
+        auto entry = [&](unsigned o, int m) -> const float2 * {
+            if (BIG) return Wl + ((int64_t)o + m);
+            return reinterpret_cast<const float2 *>(reinterpret_cast<const char *>(Wl) + (o + ((unsigned)m << 3)));
+        };
+        if (NARROW) {
             // The column spans less than one pixel: it overlaps pixel m0 and possibly m0+1.
             // Both pixels' boundary lookups are issued together.
             const int m0 = t0 > 0.0f ? (int)t0 : 0;
             const bool has1 = m0 + 1 < nalong;
-            const int m1 = has1 ? m0 + 1 : m0;
             const float w0 = pixel_weight(fminf(t1, (float)(m0 + 1)) - fmaxf(t0, (float)m0), lc.w, L, m0);
-            const float w1 = has1 ? pixel_weight(fmaxf(t1 - (float)(m0 + 1), 0.0f), lc.w, L, m1) : 0.0f;
+            const float w1 = has1 ? pixel_weight(fmaxf(t1 - (float)(m0 + 1), 0.0f), lc.w, L, m0 + 1) : 0.0f;
             // Entries m0 and m0+1 are adjacent: one 16-byte load per boundary.  When
             // m0 is the last pixel the second half belongs to the next line (or to the
             // zeroed pad after the table) and is weighted by w1 = 0.
-            (void)m1;
-            const float2 *Wm = Wl + m0;
             Pair2 e[R + 1];
 #pragma unroll
-            for (int q = 0; q <= R; ++q) e[q] = *reinterpret_cast<const Pair2 *>(Wm + koff[q]);
+            for (int q = 0; q <= R; ++q) e[q] = *reinterpret_cast<const Pair2 *>(entry(koff[q], m0));
             float J0 = fmaf(fr[0], e[0].y0, e[0].x0);
             float J1 = fmaf(fr[0], e[0].y1, e[0].x1);
 #pragma unroll
@@ -359,10 +383,9 @@ __host__ __device__ inline void cell_body(const ViewHdr *__restrict__ hdr, const
             const int m1 = t1 < fal ? (int)ceilf(t1) : nalong;
             for (int m = m0; m < m1; ++m) {
                 const float w = pixel_weight(fminf(t1, (float)(m + 1)) - fmaxf(t0, (float)m), lc.w, L, m);
-                const float2 *Wm = Wl + m;
                 float2 e[R + 1];
 #pragma unroll
-                for (int q = 0; q <= R; ++q) e[q] = Wm[koff[q]];
+                for (int q = 0; q <= R; ++q) e[q] = *entry(koff[q], m);
                 float Jp = fmaf(fr[0], e[0].y, e[0].x);
 #pragma unroll
                 for (int q = 0; q < R; ++q) {
@@ -416,7 +439,7 @@ __host__ __device__ inline void voxel_body(const ViewHdr *__restrict__ hdr, cons
         const ViewHdr h = hdr[v];
         int L, m;
         if (h.xdrv) { L = j; m = i; } else { L = i; m = a.ny - 1 - j; }
-        const float4 lv = lineV[(int64_t)v * a.maxline + L];  // {P0, Delta, zeta}
+        const float4 lv = lineV[(int64_t)v * a.maxline + L];  // {P0, Delta, zeta, 1/(Delta zeta)}
         const float a0 = fmaf((float)m, lv.y, lv.x);
         const float a1 = a0 + lv.y;
         if (!(a1 > h.dlo && a0 < h.dhi)) continue;
@@ -426,21 +449,25 @@ __host__ __device__ inline void voxel_body(const ViewHdr *__restrict__ hdr, cons
             // U * w / A_vox with A_vox = Delta * zeta * dz (include/cpu3d_common.h:80-85),
             // times the row pitch dvv of the depth overlap measured in row units.
             const float U = (OPK == VOX_BW) ? fdk_weight<GEO>(h.sx, h.sy, px, py, a.sod) : 1.0f;
-            hB = U * h.colw * a.dvv / (lv.y * lv.z * a.dz);
+            hB = U * h.colw * a.hscale * lv.w;
         }
 
-        // First column whose right boundary exceeds a0: closed-form inverse, then
-        // corrected against the table so the walk is exact w.r.t. the FP32 boundaries.
-        const float T = (h.tp1 * a0 + h.tp0) / (h.tq1 * a0 + h.tq0);
+        // A column at or before the first one whose right boundary exceeds a0: closed-form
+        // inverse, corrected downwards against the table.  Columns between it and the
+        // footprint have no overlap and are skipped by the walk below, so the result is
+        // exact w.r.t. the FP32 boundaries whatever the seed's rounding.
+        const float T = seed_div(fmaf(h.tp1, a0, h.tp0), fmaf(h.tq1, a0, h.tq0));
         const float tc = h.tk0 + h.tk1 * (GEO == GEO_ARC ? atanf(T) : T);
-        int c = (int)floorf(clampf(tc, 0.0f, (float)(nu - 1)));
+        int c = (int)clampf(tc, 0.0f, (float)(nu - 1));
         const float4 *ct = colt + (int64_t)v * (nu + 1);
         float4 cc = ct[c], cn = ct[c + 1];
         while (c > 0 && cc.x > a0) { --c; cn = cc; cc = ct[c]; }
-        while (c < nu - 1 && cn.x <= a0) { ++c; cc = cn; cn = ct[c + 1]; }
 
         const float zbz = lv.z * a.zb, zsz = lv.z * a.dz;
-        const float2 *Qv = Q + ((int64_t)b * a.nview + v) * (int64_t)(a.nv + 1) * nu;
+        // Byte offsets below 2^32 within one view's table ((Nv+1)*Nu*8, checked at plan
+        // creation) over a wave-uniform base.
+        const char *Qv = reinterpret_cast<const char *>(Q + ((int64_t)b * a.nview + v) * (int64_t)(a.nv + 1) * nu);
+        const unsigned nu8 = (unsigned)nu * 8u;
         while (c < nu && cc.x < a1) {
             // prefetch the next column's table entry one iteration ahead
             const float4 cnn = ct[c + 2 <= nu ? c + 2 : nu];
@@ -456,7 +483,7 @@ __host__ __device__ inline void voxel_body(const ViewHdr *__restrict__ hdr, cons
                     // B: ou * U w sigma dvv / (Delta zeta dz); A^T: ou (normalisation folded
                     // into the prefix table).
                     const float w = (OPK == VOX_AT) ? ou : ou * hB * cc.y;
-                    const float2 *Qc = Qv + c;
+                    const unsigned c8 = (unsigned)c * 8u;
                     // All KC+1 boundary lookups are independent: compute the rows, issue
                     // the loads back to back, then consume.
                     unsigned off[KC + 1];
@@ -464,12 +491,12 @@ __host__ __device__ inline void voxel_body(const ViewHdr *__restrict__ hdr, cons
 #pragma unroll
                     for (int q = 0; q <= KC; ++q) {
                         const float rho = clampf(fmaf((float)q, step, rlo), 0.0f, fnv);
-                        off[q] = mul24((unsigned)(int)rho, (unsigned)nu);
+                        off[q] = mul24((unsigned)(int)rho, nu8) + c8;
                         fr[q] = fractf(rho);
                     }
                     float2 e[KC + 1];
 #pragma unroll
-                    for (int q = 0; q <= KC; ++q) e[q] = Qc[off[q]];
+                    for (int q = 0; q <= KC; ++q) e[q] = *reinterpret_cast<const float2 *>(Qv + off[q]);
                     float Ip = fmaf(fr[0], e[0].y, e[0].x);
 #pragma unroll
                     for (int q = 0; q < KC; ++q) {

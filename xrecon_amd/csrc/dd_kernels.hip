@@ -89,7 +89,7 @@ __global__ __launch_bounds__(kBlock) void k_sino_prefix(const ViewHdr *__restric
     sino_prefix_body<MODE, GEO>(hdr, colt, p, Q, a, blockIdx.z, blockIdx.y, c);
 }
 
-template <int OPK, int GEO, int R>
+template <int OPK, int GEO, int R, bool NARROW, bool BIG>
 __global__ __launch_bounds__(kBlock) void k_cell(const ViewHdr *__restrict__ hdr, const float4 *__restrict__ lineC,
                                                  const float4 *__restrict__ colt, const float2 *__restrict__ WX,
                                                  const float2 *__restrict__ WY, float *__restrict__ out, CellArgs a)
@@ -101,7 +101,7 @@ __global__ __launch_bounds__(kBlock) void k_cell(const ViewHdr *__restrict__ hdr
     const int v = blockIdx.z % a.nview;
     const int b = blockIdx.z / a.nview;
     if (c >= a.nu || r0 >= a.nv) return;
-    cell_body<OPK, GEO, R>(hdr, lineC, colt, WX, WY, out, a, c, r0, v, b);
+    cell_body<OPK, GEO, R, NARROW, BIG>(hdr, lineC, colt, WX, WY, out, a, c, r0, v, b);
 }
 
 template <int OPK, int GEO, int KC>
@@ -123,13 +123,27 @@ template <int OPK, int GEO>
 void launch_cell(const Plan3d &p, int R, const float2 *WX, const float2 *WY, float *out, int nbatch, hipStream_t s)
 {
     const CellArgs a = make_cell_args(p, nbatch);
+    if (p.big) R = 8;  // tables of 4 GiB and more: one 64-bit general variant
     dim3 grid(ceil_div(a.nu, kWave), ceil_div(a.nv, R * kWavesPerBlock), nbatch * p.nview);
-    switch (R) {
-    case 1: hipLaunchKernelGGL((k_cell<OPK, GEO, 1>), grid, dim3(kBlock), 0, s, p.d_hdr, p.d_lineC, p.d_col, WX, WY, out, a); break;
-    case 2: hipLaunchKernelGGL((k_cell<OPK, GEO, 2>), grid, dim3(kBlock), 0, s, p.d_hdr, p.d_lineC, p.d_col, WX, WY, out, a); break;
-    case 4: hipLaunchKernelGGL((k_cell<OPK, GEO, 4>), grid, dim3(kBlock), 0, s, p.d_hdr, p.d_lineC, p.d_col, WX, WY, out, a); break;
-    default: hipLaunchKernelGGL((k_cell<OPK, GEO, 8>), grid, dim3(kBlock), 0, s, p.d_hdr, p.d_lineC, p.d_col, WX, WY, out, a); break;
+#define XR_CELL(RR, NN, BB) \
+    hipLaunchKernelGGL((k_cell<OPK, GEO, RR, NN, BB>), grid, dim3(kBlock), 0, s, p.d_hdr, p.d_lineC, p.d_col, WX, WY, out, a)
+    if (p.big) XR_CELL(8, false, true);
+    else if (p.all_narrow) {
+        switch (R) {
+        case 1: XR_CELL(1, true, false); break;
+        case 2: XR_CELL(2, true, false); break;
+        case 4: XR_CELL(4, true, false); break;
+        default: XR_CELL(8, true, false); break;
+        }
+    } else {
+        switch (R) {
+        case 1: XR_CELL(1, false, false); break;
+        case 2: XR_CELL(2, false, false); break;
+        case 4: XR_CELL(4, false, false); break;
+        default: XR_CELL(8, false, false); break;
+        }
     }
+#undef XR_CELL
 }
 
 template <int OPK, int GEO>

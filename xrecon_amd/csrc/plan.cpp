@@ -91,6 +91,8 @@ int build_tables(Plan3d &p, const float *angles, PlanTables &t, std::string &err
     p.xdrv.assign(V, 0);
     p.keep.assign(V, 0);
     p.any_x = p.any_y = 0;
+    p.all_narrow = V > 0 ? 1 : 0;
+    p.big = ((int64_t)(nz + 1) * nx * ny * 8 >= ((int64_t)1 << 32) || (int64_t)nx * ny * 8 >= ((int64_t)1 << 24)) ? 1 : 0;
 
     std::vector<double> d(nu + 1), sig(nu);
     for (int v = 0; v < V; ++v) {
@@ -161,7 +163,7 @@ int build_tables(Plan3d &p, const float *angles, PlanTables &t, std::string &err
             const size_t li = (size_t)v * ML + L;
             t.lineC[li] = float4{(float)(-P0 / Dl), (float)(1.0 / Dl), (float)(1.0 / (zeta * dzz)),
                                  (float)(Dl * zeta * dzz)};
-            t.lineV[li] = float4{(float)P0, (float)Dl, (float)zeta, 0.0f};
+            t.lineV[li] = float4{(float)P0, (float)Dl, (float)zeta, (float)(1.0 / (Dl * zeta))};
         }
 
         // Columns: boundaries on the axis in internal order; per-column depth scale sigma
@@ -224,6 +226,7 @@ int build_tables(Plan3d &p, const float *angles, PlanTables &t, std::string &err
             for (int c = 0; c < nu; ++c) wmax = std::max(wmax, d[c + 1] - d[c]);
             for (int L = 0; L < h.nline; ++L) dmin = std::min(dmin, 1.0 / (double)t.lineC[(size_t)v * ML + L].y);
             h.narrow = (wmax < 0.99 * dmin) ? 1 : 0;
+            if (!h.narrow) p.all_narrow = 0;
         }
 
         // Inverse map axis point -> fractional internal column, used by the voxel kernels

@@ -51,6 +51,8 @@ struct Plan3d {
     int nview = 0;
     int maxline = 0;         // max(Nx, Ny)
     int any_x = 0, any_y = 0;
+    int all_narrow = 0;      // every view narrow: the cell gather takes the two-pixel path
+    int big = 0;             // z-prefix table of one batch item >= 2^32 bytes or plane >= 2^24 bytes
     // Scalars shared by all views.
     float zb = 0, dz = 1;    // volume bottom z and slice thickness
     float v0 = 0, dvv = 1;   // first row boundary and row pitch on the iso plane
@@ -62,7 +64,7 @@ struct Plan3d {
     // Device tables.
     ViewHdr *d_hdr = nullptr;   // [nview]
     float4 *d_lineC = nullptr;  // [nview][maxline] {toff, invd, iz, lam}: cell kernels
-    float4 *d_lineV = nullptr;  // [nview][maxline] {P0, Delta, zeta, 0}: voxel kernels
+    float4 *d_lineV = nullptr;  // [nview][maxline] {P0, Delta, zeta, 1/(Delta zeta)}: voxel kernels
     float4 *d_col = nullptr;    // [nview][Nu+1]    {d, sigma, kappa, W}
 };
 
