@@ -107,6 +107,18 @@ def emu_op3d(op, x, angles, opts, R=0, KC=0, is2d=False, nthreads=None, cell_var
     return out
 
 
+def emu_view_spans(angles, opts):
+    """Span class per view (2 / 3: fixed-span cell path, 0: general loop)."""
+    lib = _load_emu()
+    vn, vf, gn, gf = oracle.split3d(opts)
+    angles = np.ascontiguousarray(angles, dtype=np.float32)
+    spans = np.zeros(angles.shape[0], dtype=np.int32)
+    rc = lib.emu_view_spans(_p(vn), _p(vf), _p(gn), _p(gf), _p(angles), angles.shape[0], _p(spans))
+    if rc != 0:
+        raise RuntimeError(lib.emu_error().decode())
+    return spans
+
+
 def opts2d_as_3d(o2):
     """2D options -> the one-slice 3D options the kernels use (with the 2D flag set by
     emu_op3d(is2d=True)): W, H, 1, Nu, 1, dx, dy, 1, du, 1, SOD, SDD, shx, shy, 0, shu, 0, type."""

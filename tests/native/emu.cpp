@@ -16,12 +16,11 @@ using namespace xr;
 
 namespace {
 
-// Cell-gather variant: 0 = the one the library launches (two-pixel path when every view
-// is narrow, 64-bit offsets for tables of 4 GiB and more), 1 = the general 32-bit path,
-// 2 = the general 64-bit path.
+// Cell-gather variant: 0 = the one the library launches (fixed-span paths by view,
+// 64-bit offsets for big plans), 1 = the general 32-bit path, 2 = the general 64-bit path.
 int g_cell_variant = 0;
 
-template <int OPK, int GEO, int R, bool NARROW, bool BIG>
+template <int OPK, int GEO, int R, bool FIXED, bool BIG>
 void run_cell_RV(const Plan3d &p, const float2 *WX, const float2 *WY, float *out, int nb, int nthreads)
 {
     const CellArgs a = make_cell_args(p, nb);
@@ -32,9 +31,15 @@ void run_cell_RV(const Plan3d &p, const float2 *WX, const float2 *WY, float *out
         th.emplace_back([&, t]() {
             for (int it = t; it < items; it += nthreads) {
                 const int v = it % p.nview, b = it / p.nview;
+                const int span = FIXED ? p.d_hdr[v].span : 0;
                 for (int g = 0; g < ngroups; ++g)
                     for (int c = 0; c < a.nu; ++c)
-                        cell_body<OPK, GEO, R, NARROW, BIG>(p.d_hdr, p.d_lineC, p.d_col, WX, WY, out, a, c, g * R, v, b);
+                        if (span == 2)
+                            cell_body<OPK, GEO, R, 2, false>(p.d_hdr, p.d_lineC, p.d_col, WX, WY, out, a, c, g * R, v, b);
+                        else if (span == 3)
+                            cell_body<OPK, GEO, R, 3, false>(p.d_hdr, p.d_lineC, p.d_col, WX, WY, out, a, c, g * R, v, b);
+                        else
+                            cell_body<OPK, GEO, R, 0, BIG>(p.d_hdr, p.d_lineC, p.d_col, WX, WY, out, a, c, g * R, v, b);
             }
         });
     }
@@ -44,7 +49,7 @@ void run_cell_RV(const Plan3d &p, const float2 *WX, const float2 *WY, float *out
 template <int OPK, int GEO, int R>
 void run_cell_R(const Plan3d &p, const float2 *WX, const float2 *WY, float *out, int nb, int nt)
 {
-    const int var = g_cell_variant ? g_cell_variant : (p.big ? 2 : p.all_narrow ? 0 : 1);
+    const int var = g_cell_variant ? g_cell_variant : (p.big ? 2 : 0);
     if (var == 2) run_cell_RV<OPK, GEO, R, false, true>(p, WX, WY, out, nb, nt);
     else if (var == 1) run_cell_RV<OPK, GEO, R, false, false>(p, WX, WY, out, nb, nt);
     else run_cell_RV<OPK, GEO, R, true, false>(p, WX, WY, out, nb, nt);
@@ -91,7 +96,7 @@ void dispatch(const Plan3d &p, const Route &rt, int R, int KC, const float *in, 
 {
     if (rt.cell) {
         const int64_t plane = (int64_t)p.vol.nx * p.vol.ny;
-        std::vector<float2> WX((size_t)nb * (p.vol.nz + 1) * plane + 1), WY(WX.size());  // + zeroed pad entry
+        std::vector<float2> WX((size_t)nb * (p.vol.nz + 1) * plane + kVolPad), WY(WX.size());  // + zeroed pad
         for (int b = 0; b < nb; ++b)
             for (int64_t e = 0; e < plane; ++e) vol_prefix_body(in, WX.data(), plane, p.vol.nz, b, e);
         const int nx = p.vol.nx, ny = p.vol.ny;
@@ -125,6 +130,27 @@ extern "C" {
 const char *emu_error(void) { return g_err.c_str(); }
 
 void emu_set_cell_variant(int v) { g_cell_variant = v; }
+
+// Per-view span class of the cell gather (ViewHdr::span) for a geometry.
+int emu_view_spans(const int *vn, const float *vf, const int *gn, const float *gf, const float *angles, int nview,
+                   int *spans)
+{
+    Plan3d p;
+    p.vol = xr_volume3d{vn[0], vn[1], vn[2], vf[0], vf[1], vf[2], vf[3], vf[4], vf[5]};
+    p.geo = xr_geometry3d{gn[0] & 0xff, gn[1], gn[2], gf[0], gf[1], gf[2], gf[3], gf[4], gf[5]};
+    p.is2d = (gn[0] & 0x100) ? 1 : 0;
+    p.kind = p.geo.type;
+    p.nview = nview;
+    PlanTables t;
+    std::string err;
+    const int rc = build_tables(p, angles, t, err);
+    if (rc) {
+        g_err = err;
+        return rc;
+    }
+    for (int v = 0; v < nview; ++v) spans[v] = t.hdr[v].span;
+    return XR_OK;
+}
 
 // vn = {nx, ny, nz}, vf = {dx, dy, dz, shx, shy, shz}, gn = {type(|0x100 for 2D), nu, nv},
 // gf = {du, dv, shu, shv, sod, sdd}; tile_rows / voxel_chunk 0 = the library defaults

@@ -6,8 +6,8 @@ import numpy as np
 import pytest
 
 import oracle
-from helpers import CASES3D, OPS, VOLUME_IN, angles_with_ties, emu_available, emu_op3d, opts2d, opts2d_as_3d, \
-    opts3d, rel_l2, shapes3d
+from helpers import CASES3D, OPS, VOLUME_IN, angles_with_ties, emu_available, emu_op3d, emu_view_spans, opts2d, \
+    opts2d_as_3d, opts3d, rel_l2, shapes3d
 
 pytestmark = pytest.mark.skipif(not emu_available(), reason="emulator not built (run __graft_entry__.build())")
 
@@ -109,3 +109,24 @@ def test_cell_gather_variants_agree(geo, variant):
         got = emu_op3d(op, x, ang, opts, cell_variant=variant)
         assert rel_l2(got, want) <= 1e-5
         assert rel_l2(got, base) <= 2e-6
+
+
+def test_span_classes_cover_fixed_and_general_paths():
+    """The benchmark's cone-flat geometry (512 x 64 detector at 1 mm, magnification 2, 1 mm
+    voxels) has two-pixel views near the axes and three-pixel views near the diagonals; a
+    coarse detector falls back to the general loop.  Each class agrees with the oracle."""
+    cfg2 = opts3d(256, 256, 64, 512, 64, 1.0, 1.0, 1.0, 1.0, 1.0, 500.0, 1000.0, 1)
+    spans = emu_view_spans(np.arange(720, dtype=np.float32) * np.float32(2 * np.pi / 720), cfg2)
+    assert set(np.unique(spans)) == {2, 3}
+    assert np.count_nonzero(spans == 2) > np.count_nonzero(spans == 3)
+    coarse = opts3d(20, 18, 6, 12, 5, 0.5, 0.5, 0.5, 4.0, 2.0, 500.0, 1000.0, 1, 0.1, 0.0, 0.0, 0.3, 0.1)
+    assert np.all(emu_view_spans(angles_with_ties(16), coarse) == 0)
+    # a small geometry with both fixed classes, through every cell operator
+    opts = opts3d(24, 20, 6, 30, 6, 1.0, 1.0, 1.0, 1.6, 1.0, 500.0, 1000.0, 1, 0.2, -0.1, 0.1, 0.2, 0.1)
+    ang = angles_with_ties(24)
+    sp = emu_view_spans(ang, opts)
+    assert {2, 3} <= set(np.unique(sp))
+    vol, _ = shapes3d(opts, 1, 24)
+    x = np.random.default_rng(8).random(vol).astype(np.float32)
+    for op in ("A", "BT", "BWT"):
+        assert rel_l2(emu_op3d(op, x, ang, opts), oracle.op3d(op, x.astype(np.float64), ang, opts)) <= 1e-5

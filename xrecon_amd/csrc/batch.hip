@@ -30,7 +30,7 @@ struct BatchDesc {
     VoxArgs va;  // nchunk is filled per launch from the launch's voxel chunk
     SinoArgs sa;
     int nb, any_y;
-    int narrow, big;  // Plan3d::all_narrow, Plan3d::big
+    int big;  // Plan3d::big
     unsigned long long ws_volx, ws_voly, ws_sino;  // byte offsets into the workspace
 };
 
@@ -82,8 +82,8 @@ __global__ __launch_bounds__(kBlock) void k_vol_prefix_batch(const BatchDesc *__
     const BatchDesc &d = D[i];
     const int64_t plane = (int64_t)d.ca.nx * d.ca.ny;
     const int64_t idx = (int64_t)bid * kBlock + threadIdx.x;
-    if (idx == 0)
-        reinterpret_cast<float2 *>(ws + d.ws_volx)[plane * (d.ca.nz + 1) * d.nb] = make_float2(0.0f, 0.0f);
+    if (idx < kVolPad)
+        reinterpret_cast<float2 *>(ws + d.ws_volx)[plane * (d.ca.nz + 1) * d.nb + idx] = make_float2(0.0f, 0.0f);
     if (idx >= plane * d.nb) return;
     const int64_t b = idx / plane;
     vol_prefix_body(io.in[i], reinterpret_cast<float2 *>(ws + d.ws_volx), plane, d.ca.nz, b, idx - b * plane);
@@ -101,8 +101,8 @@ __global__ __launch_bounds__(kBlock) void k_transpose_batch(const BatchDesc *__r
     const int tx_n = ceil_div(nx, 32), ty_n = ceil_div(ny, 32);
     const int bx = bid % tx_n, by = (bid / tx_n) % ty_n, p = bid / (tx_n * ty_n);
     const int64_t plane = (int64_t)nx * ny;
-    if (bid == 0 && threadIdx.x == 0)
-        reinterpret_cast<float2 *>(ws + d.ws_voly)[plane * (d.ca.nz + 1) * d.nb] = make_float2(0.0f, 0.0f);
+    if (bid == 0 && threadIdx.x < kVolPad)
+        reinterpret_cast<float2 *>(ws + d.ws_voly)[plane * (d.ca.nz + 1) * d.nb + threadIdx.x] = make_float2(0.0f, 0.0f);
     const float2 *src = reinterpret_cast<const float2 *>(ws + d.ws_volx) + (int64_t)p * plane;
     float2 *dst = reinterpret_cast<float2 *>(ws + d.ws_voly) + (int64_t)p * plane;
     const int i0 = bx * 32, j0 = by * 32;
@@ -137,9 +137,11 @@ __global__ __launch_bounds__(kBlock) void k_cell_batch(const BatchDesc *__restri
     const float2 *WY = reinterpret_cast<const float2 *>(ws + d.ws_voly);
     const int v = bz % a.nview, b = bz / a.nview;
     // the per-instance variant choice is uniform over the workgroup
-    if (d.big) cell_body<OPK, GEO, R, false, true>(d.hdr, d.lineC, d.col, WX, WY, io.out[i], a, c, r0, v, b);
-    else if (d.narrow) cell_body<OPK, GEO, R, true, false>(d.hdr, d.lineC, d.col, WX, WY, io.out[i], a, c, r0, v, b);
-    else cell_body<OPK, GEO, R, false, false>(d.hdr, d.lineC, d.col, WX, WY, io.out[i], a, c, r0, v, b);
+    const int span = d.big ? 0 : d.hdr[v].span;
+    if (d.big) cell_body<OPK, GEO, R, 0, true>(d.hdr, d.lineC, d.col, WX, WY, io.out[i], a, c, r0, v, b);
+    else if (span == 2) cell_body<OPK, GEO, R, 2, false>(d.hdr, d.lineC, d.col, WX, WY, io.out[i], a, c, r0, v, b);
+    else if (span == 3) cell_body<OPK, GEO, R, 3, false>(d.hdr, d.lineC, d.col, WX, WY, io.out[i], a, c, r0, v, b);
+    else cell_body<OPK, GEO, R, 0, false>(d.hdr, d.lineC, d.col, WX, WY, io.out[i], a, c, r0, v, b);
 }
 
 template <int MODE, int GEO>
@@ -266,7 +268,6 @@ int xr_batch3d_create(xr_plan3d *const *plans, int n, const int *num_batch, xr_b
         d.sa = xr::make_sino_args(p, num_batch[i]);
         d.nb = num_batch[i];
         d.any_y = p.any_y;
-        d.narrow = p.all_narrow;
         d.big = p.big;
         const size_t vb = xr::align256(xr::vol_prefix_bytes(p, num_batch[i]));
         d.ws_volx = off_cell;

@@ -74,9 +74,11 @@ __host__ __device__ inline int ceil_div(int64_t a, int64_t b) { return (int)((a 
 
 inline size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
 
-// Workspace of the cell gathers: z-prefix of the volume in one orientation, plus one
-// zeroed pad entry after the table (the two-pixel loads of the cell gather may read one
-// entry past the last pixel, with weight zero).
+// Workspace of the cell gathers: z-prefix of the volume in one orientation, plus
+// kVolPad zeroed pad entries after the table (the fixed-span loads of the cell gather may
+// read up to two entries past the last pixel, with weight zero).
+constexpr int kVolPad = 2;
+
 inline size_t vol_prefix_entries(const Plan3d &p, int nbatch)
 {
     return (size_t)nbatch * (p.vol.nz + 1) * (size_t)p.vol.nx * p.vol.ny;
@@ -84,7 +86,7 @@ inline size_t vol_prefix_entries(const Plan3d &p, int nbatch)
 
 inline size_t vol_prefix_bytes(const Plan3d &p, int nbatch)
 {
-    return (vol_prefix_entries(p, nbatch) + 1) * sizeof(float2);
+    return (vol_prefix_entries(p, nbatch) + kVolPad) * sizeof(float2);
 }
 
 // Workspace of the voxel gathers: row-prefix of the sinogram.
@@ -286,12 +288,14 @@ __host__ __device__ inline void sino_prefix_body(const ViewHdr *__restrict__ hdr
 
 // One thread: internal column c, rows r0..r0+R-1 of view v, batch b.
 //
-// NARROW: every view of the plan is narrow (a column overlaps at most two consecutive
-// pixels of a line), so each line takes the fixed two-pixel path.
+// SPAN = 2 or 3: every column of the view overlaps at most SPAN consecutive pixels of any
+// line (ViewHdr::span), so each line takes a fixed SPAN-pixel path with no pixel loop;
+// SPAN = 0 is the general loop.  The kernels pick the variant per view at entry (one
+// view per workgroup), outside the line loop.
 // BIG: one batch item's z-prefix table reaches 2^32 bytes or a z-plane 2^24 bytes, so
 // table offsets are taken in 64-bit arithmetic; otherwise they are 32-bit byte offsets
 // (one 24-bit multiply-add) over a wave-uniform line base.
-template <int OPK, int GEO, int R, bool NARROW, bool BIG>
+template <int OPK, int GEO, int R, int SPAN, bool BIG>
 __host__ __device__ inline void cell_body(const ViewHdr *__restrict__ hdr, const float4 *__restrict__ lineC,
                                           const float4 *__restrict__ colt, const float2 *__restrict__ WX,
                                           const float2 *__restrict__ WY, float *__restrict__ out, const CellArgs &a,
@@ -355,13 +359,12 @@ __host__ __device__ inline void cell_body(const ViewHdr *__restrict__ hdr, const
             if (BIG) return Wl + ((int64_t)o + m);
             return reinterpret_cast<const float2 *>(reinterpret_cast<const char *>(Wl) + (o + ((unsigned)m << 3)));
         };
-        if (NARROW) {
+        if (SPAN == 2) {
             // The column spans less than one pixel: it overlaps pixel m0 and possibly m0+1.
             // Both pixels' boundary lookups are issued together.
             const int m0 = t0 > 0.0f ? (int)t0 : 0;
-            const bool has1 = m0 + 1 < nalong;
             const float w0 = pixel_weight(fminf(t1, (float)(m0 + 1)) - fmaxf(t0, (float)m0), lc.w, L, m0);
-            const float w1 = has1 ? pixel_weight(fmaxf(t1 - (float)(m0 + 1), 0.0f), lc.w, L, m0 + 1) : 0.0f;
+            const float w1 = m0 + 1 < nalong ? pixel_weight(fmaxf(t1 - (float)(m0 + 1), 0.0f), lc.w, L, m0 + 1) : 0.0f;
             // Entries m0 and m0+1 are adjacent: one 16-byte load per boundary.  When
             // m0 is the last pixel the second half belongs to the next line (or to the
             // zeroed pad after the table) and is weighted by w1 = 0.
@@ -377,6 +380,37 @@ __host__ __device__ inline void cell_body(const ViewHdr *__restrict__ hdr, const
                 acc[q] = fmaf(w0, K0 - J0, fmaf(w1, K1 - J1, acc[q]));
                 J0 = K0;
                 J1 = K1;
+            }
+        } else if (SPAN == 3) {
+            // The column spans less than two pixels: pixels m0, m0+1 and possibly m0+2,
+            // read as one 16-byte and one 8-byte load per boundary (entries past the
+            // line's end are weighted by zero, as above).
+            const int m0 = t0 > 0.0f ? (int)t0 : 0;
+            const float w0 = pixel_weight(fminf(t1, (float)(m0 + 1)) - fmaxf(t0, (float)m0), lc.w, L, m0);
+            const float w1 = m0 + 1 < nalong
+                                 ? pixel_weight(fmaxf(fminf(t1, (float)(m0 + 2)) - (float)(m0 + 1), 0.0f), lc.w, L, m0 + 1)
+                                 : 0.0f;
+            const float w2 = m0 + 2 < nalong ? pixel_weight(fmaxf(t1 - (float)(m0 + 2), 0.0f), lc.w, L, m0 + 2) : 0.0f;
+            Pair2 e[R + 1];
+            float2 g[R + 1];
+#pragma unroll
+            for (int q = 0; q <= R; ++q) {
+                const float2 *p0 = entry(koff[q], m0);
+                e[q] = *reinterpret_cast<const Pair2 *>(p0);
+                g[q] = p0[2];
+            }
+            float J0 = fmaf(fr[0], e[0].y0, e[0].x0);
+            float J1 = fmaf(fr[0], e[0].y1, e[0].x1);
+            float J2 = fmaf(fr[0], g[0].y, g[0].x);
+#pragma unroll
+            for (int q = 0; q < R; ++q) {
+                const float K0 = fmaf(fr[q + 1], e[q + 1].y0, e[q + 1].x0);
+                const float K1 = fmaf(fr[q + 1], e[q + 1].y1, e[q + 1].x1);
+                const float K2 = fmaf(fr[q + 1], g[q + 1].y, g[q + 1].x);
+                acc[q] = fmaf(w0, K0 - J0, fmaf(w1, K1 - J1, fmaf(w2, K2 - J2, acc[q])));
+                J0 = K0;
+                J1 = K1;
+                J2 = K2;
             }
         } else {
             const int m0 = t0 > 0.0f ? (int)floorf(t0) : 0;

@@ -50,7 +50,7 @@ __global__ __launch_bounds__(kBlock) void k_vol_prefix(const float *__restrict__
 {
     const int64_t plane = (int64_t)nx * ny;
     const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx == 0) W[plane * (nz + 1) * nbatch] = make_float2(0.0f, 0.0f);  // pad entry
+    if (idx < kVolPad) W[plane * (nz + 1) * nbatch + idx] = make_float2(0.0f, 0.0f);  // pad entries
     if (idx >= plane * nbatch) return;
     const int64_t b = idx / plane;
     vol_prefix_body(f, W, plane, nz, b, idx - b * plane);
@@ -63,8 +63,8 @@ __global__ __launch_bounds__(kBlock) void k_plane_transpose(const float2 *__rest
 {
     __shared__ float2 tile[32][33];
     const int64_t plane = (int64_t)nx * ny;
-    if ((blockIdx.x | blockIdx.y | blockIdx.z | threadIdx.x) == 0)
-        WY[plane * gridDim.z] = make_float2(0.0f, 0.0f);  // pad entry after the last plane
+    if ((blockIdx.x | blockIdx.y | blockIdx.z) == 0 && threadIdx.x < kVolPad)
+        WY[plane * gridDim.z + threadIdx.x] = make_float2(0.0f, 0.0f);  // pad entries after the last plane
     const float2 *src = WX + (int64_t)blockIdx.z * plane;
     float2 *dst = WY + (int64_t)blockIdx.z * plane;
     const int i0 = blockIdx.x * 32, j0 = blockIdx.y * 32;
@@ -89,7 +89,7 @@ __global__ __launch_bounds__(kBlock) void k_sino_prefix(const ViewHdr *__restric
     sino_prefix_body<MODE, GEO>(hdr, colt, p, Q, a, blockIdx.z, blockIdx.y, c);
 }
 
-template <int OPK, int GEO, int R, bool NARROW, bool BIG>
+template <int OPK, int GEO, int R, bool BIG>
 __global__ __launch_bounds__(kBlock) void k_cell(const ViewHdr *__restrict__ hdr, const float4 *__restrict__ lineC,
                                                  const float4 *__restrict__ colt, const float2 *__restrict__ WX,
                                                  const float2 *__restrict__ WY, float *__restrict__ out, CellArgs a)
@@ -101,7 +101,10 @@ __global__ __launch_bounds__(kBlock) void k_cell(const ViewHdr *__restrict__ hdr
     const int v = blockIdx.z % a.nview;
     const int b = blockIdx.z / a.nview;
     if (c >= a.nu || r0 >= a.nv) return;
-    cell_body<OPK, GEO, R, NARROW, BIG>(hdr, lineC, colt, WX, WY, out, a, c, r0, v, b);
+    const int span = BIG ? 0 : hdr[v].span;
+    if (span == 2) cell_body<OPK, GEO, R, 2, false>(hdr, lineC, colt, WX, WY, out, a, c, r0, v, b);
+    else if (span == 3) cell_body<OPK, GEO, R, 3, false>(hdr, lineC, colt, WX, WY, out, a, c, r0, v, b);
+    else cell_body<OPK, GEO, R, 0, BIG>(hdr, lineC, colt, WX, WY, out, a, c, r0, v, b);
 }
 
 template <int OPK, int GEO, int KC>
@@ -125,22 +128,15 @@ void launch_cell(const Plan3d &p, int R, const float2 *WX, const float2 *WY, flo
     const CellArgs a = make_cell_args(p, nbatch);
     if (p.big) R = 8;  // tables of 4 GiB and more: one 64-bit general variant
     dim3 grid(ceil_div(a.nu, kWave), ceil_div(a.nv, R * kWavesPerBlock), nbatch * p.nview);
-#define XR_CELL(RR, NN, BB) \
-    hipLaunchKernelGGL((k_cell<OPK, GEO, RR, NN, BB>), grid, dim3(kBlock), 0, s, p.d_hdr, p.d_lineC, p.d_col, WX, WY, out, a)
-    if (p.big) XR_CELL(8, false, true);
-    else if (p.all_narrow) {
+#define XR_CELL(RR, BB) \
+    hipLaunchKernelGGL((k_cell<OPK, GEO, RR, BB>), grid, dim3(kBlock), 0, s, p.d_hdr, p.d_lineC, p.d_col, WX, WY, out, a)
+    if (p.big) XR_CELL(8, true);
+    else {
         switch (R) {
-        case 1: XR_CELL(1, true, false); break;
-        case 2: XR_CELL(2, true, false); break;
-        case 4: XR_CELL(4, true, false); break;
-        default: XR_CELL(8, true, false); break;
-        }
-    } else {
-        switch (R) {
-        case 1: XR_CELL(1, false, false); break;
-        case 2: XR_CELL(2, false, false); break;
-        case 4: XR_CELL(4, false, false); break;
-        default: XR_CELL(8, false, false); break;
+        case 1: XR_CELL(1, false); break;
+        case 2: XR_CELL(2, false); break;
+        case 4: XR_CELL(4, false); break;
+        default: XR_CELL(8, false); break;
         }
     }
 #undef XR_CELL

@@ -91,7 +91,6 @@ int build_tables(Plan3d &p, const float *angles, PlanTables &t, std::string &err
     p.xdrv.assign(V, 0);
     p.keep.assign(V, 0);
     p.any_x = p.any_y = 0;
-    p.all_narrow = V > 0 ? 1 : 0;
     p.big = ((int64_t)(nz + 1) * nx * ny * 8 >= ((int64_t)1 << 32) || (int64_t)nx * ny * 8 >= ((int64_t)1 << 24)) ? 1 : 0;
 
     std::vector<double> d(nu + 1), sig(nu);
@@ -225,8 +224,7 @@ int build_tables(Plan3d &p, const float *angles, PlanTables &t, std::string &err
             double wmax = 0.0, dmin = 1e300;
             for (int c = 0; c < nu; ++c) wmax = std::max(wmax, d[c + 1] - d[c]);
             for (int L = 0; L < h.nline; ++L) dmin = std::min(dmin, 1.0 / (double)t.lineC[(size_t)v * ML + L].y);
-            h.narrow = (wmax < 0.99 * dmin) ? 1 : 0;
-            if (!h.narrow) p.all_narrow = 0;
+            h.span = (wmax < 0.99 * dmin) ? 2 : (wmax < 1.99 * dmin) ? 3 : 0;
         }
 
         // Inverse map axis point -> fractional internal column, used by the voxel kernels

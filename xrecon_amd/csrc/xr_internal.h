@@ -34,7 +34,7 @@ struct alignas(16) ViewHdr {
     float tk0, tk1;  // fractional internal column of axis point a: tk0 + tk1 * g(T(a))
     float sx, sy;    // source position (cone) for the FDK weights
     float dlo, dhi;  // d_0 and d_Nu
-    int narrow;      // 1: every column spans at most one pixel width on every line of this view
+    int span;        // 2 / 3: every column overlaps at most 2 / 3 pixels of any line; 0: no bound
     float pad1;
 };
 static_assert(sizeof(ViewHdr) == 80, "ViewHdr layout");
@@ -51,7 +51,6 @@ struct Plan3d {
     int nview = 0;
     int maxline = 0;         // max(Nx, Ny)
     int any_x = 0, any_y = 0;
-    int all_narrow = 0;      // every view narrow: the cell gather takes the two-pixel path
     int big = 0;             // z-prefix table of one batch item >= 2^32 bytes or plane >= 2^24 bytes
     // Scalars shared by all views.
     float zb = 0, dz = 1;    // volume bottom z and slice thickness
