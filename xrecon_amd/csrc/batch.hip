@@ -58,7 +58,7 @@ __host__ __device__ inline int blocks_sino_prefix(const BatchDesc &d)
 }
 __host__ __device__ inline int blocks_voxel(const BatchDesc &d, int KC)
 {
-    return ceil_div(d.va.nx, kWave) * ceil_div(d.va.ny, kWavesPerBlock) * d.nb * ceil_div(d.va.nz, KC);
+    return ceil_div(d.va.nx, kVoxTile) * ceil_div(d.va.ny, kVoxTile) * d.nb * ceil_div(d.va.nz, KC);
 }
 
 // Maps the flat block index to (instance, local block); returns the instance or -1.
@@ -169,10 +169,10 @@ __global__ __launch_bounds__(kBlock) void k_voxel_batch(const BatchDesc *__restr
     const BatchDesc &d = D[i];
     VoxArgs a = d.va;
     a.nchunk = ceil_div(a.nz, KC);
-    const int gx = ceil_div(a.nx, kWave), gy = ceil_div(a.ny, kWavesPerBlock);
+    const int gx = ceil_div(a.nx, kVoxTile), gy = ceil_div(a.ny, kVoxTile);
     const int bx = bid % gx, by = (bid / gx) % gy, bz = bid / (gx * gy);
-    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
-    const int ii = bx * kWave + lane, j = by * kWavesPerBlock + wave;
+    int ii, j;
+    voxel_coords(bx, by, threadIdx.x, ii, j);
     if (ii >= a.nx || j >= a.ny) return;
     voxel_body<OPK, GEO, KC>(d.hdr, d.lineV, d.col, reinterpret_cast<const float2 *>(ws + d.ws_sino), io.out[i], a,
                              ii, j, (bz % a.nchunk) * KC, bz / a.nchunk);

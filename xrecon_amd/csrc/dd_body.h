@@ -23,6 +23,19 @@ constexpr int kWave = 64;
 constexpr int kBlock = 256;
 constexpr int kWavesPerBlock = kBlock / kWave;
 
+// Voxel-gather workgroup: a kVoxTile x kVoxTile block of voxel columns, each wave an
+// 8 x 8 tile of it.  The 64 lanes of a wave then project onto a compact detector patch
+// (a few cache lines per lookup) for views of either driving axis, where a 64 x 1 strip
+// would spread one lookup over ~128 columns.
+constexpr int kVoxTile = 16;
+
+__host__ __device__ inline void voxel_coords(int bx, int by, int tid, int &i, int &j)
+{
+    const int lane = tid & (kWave - 1), wave = tid / kWave;
+    i = bx * kVoxTile + (wave & 1) * 8 + (lane & 7);
+    j = by * kVoxTile + (wave >> 1) * 8 + (lane >> 3);
+}
+
 struct CellArgs {
     int nx, ny, nz, nu, nv, nview, maxline, nbatch;
     float v0, dvv, zoff, sod;

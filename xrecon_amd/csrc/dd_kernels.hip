@@ -112,10 +112,8 @@ __global__ __launch_bounds__(kBlock) void k_voxel(const ViewHdr *__restrict__ hd
                                                   const float4 *__restrict__ colt, const float2 *__restrict__ Q,
                                                   float *__restrict__ out, VoxArgs a)
 {
-    const int lane = threadIdx.x & (kWave - 1);
-    const int wave = threadIdx.x / kWave;
-    const int i = blockIdx.x * kWave + lane;
-    const int j = blockIdx.y * kWavesPerBlock + wave;
+    int i, j;
+    voxel_coords(blockIdx.x, blockIdx.y, threadIdx.x, i, j);
     const int chunk = blockIdx.z % a.nchunk;
     const int b = blockIdx.z / a.nchunk;
     if (i >= a.nx || j >= a.ny) return;
@@ -146,7 +144,7 @@ template <int OPK, int GEO>
 void launch_voxel(const Plan3d &p, int KC, const float2 *Q, float *out, int nbatch, hipStream_t s)
 {
     const VoxArgs a = make_vox_args(p, nbatch, KC);
-    dim3 grid(ceil_div(a.nx, kWave), ceil_div(a.ny, kWavesPerBlock), nbatch * a.nchunk);
+    dim3 grid(ceil_div(a.nx, kVoxTile), ceil_div(a.ny, kVoxTile), nbatch * a.nchunk);
     switch (KC) {
     case 1: hipLaunchKernelGGL((k_voxel<OPK, GEO, 1>), grid, dim3(kBlock), 0, s, p.d_hdr, p.d_lineV, p.d_col, Q, out, a); break;
     case 4: hipLaunchKernelGGL((k_voxel<OPK, GEO, 4>), grid, dim3(kBlock), 0, s, p.d_hdr, p.d_lineV, p.d_col, Q, out, a); break;
