@@ -118,8 +118,8 @@ __global__ __launch_bounds__(kBlock) void k_transpose_batch(const BatchDesc *__r
     }
 }
 
-template <int OPK, int GEO, int R>
-__global__ __launch_bounds__(kBlock) void k_cell_batch(const BatchDesc *__restrict__ D, int n, IOPtrs io,
+template <int OPK, int GEO, int R, bool BIG>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_cell_batch(const BatchDesc *__restrict__ D, int n, IOPtrs io,
                                                        char *__restrict__ ws)
 {
     int bid = blockIdx.x;
@@ -136,9 +136,10 @@ __global__ __launch_bounds__(kBlock) void k_cell_batch(const BatchDesc *__restri
     const float2 *WX = reinterpret_cast<const float2 *>(ws + d.ws_volx);
     const float2 *WY = reinterpret_cast<const float2 *>(ws + d.ws_voly);
     const int v = bz % a.nview, b = bz / a.nview;
-    // the per-instance variant choice is uniform over the workgroup
-    const int span = d.big ? 0 : d.hdr[v].span;
-    if (d.big) cell_body<OPK, GEO, R, 0, true>(d.hdr, d.lineC, d.col, WX, WY, io.out[i], a, c, r0, v, b);
+    // the per-view variant choice is uniform over the workgroup; a batch holding a big
+    // plan runs every instance through the 64-bit general path
+    const int span = BIG ? 0 : d.hdr[v].span;
+    if (BIG) cell_body<OPK, GEO, R, 0, true>(d.hdr, d.lineC, d.col, WX, WY, io.out[i], a, c, r0, v, b);
     else if (span == 2) cell_body<OPK, GEO, R, 2, false>(d.hdr, d.lineC, d.col, WX, WY, io.out[i], a, c, r0, v, b);
     else if (span == 3) cell_body<OPK, GEO, R, 3, false>(d.hdr, d.lineC, d.col, WX, WY, io.out[i], a, c, r0, v, b);
     else cell_body<OPK, GEO, R, 0, false>(d.hdr, d.lineC, d.col, WX, WY, io.out[i], a, c, r0, v, b);
@@ -179,13 +180,17 @@ __global__ __launch_bounds__(kBlock) void k_voxel_batch(const BatchDesc *__restr
 }
 
 template <int OPK, int GEO>
-void launch_cell_batch(int R, int grid, const BatchDesc *D, int n, const IOPtrs &io, char *ws, hipStream_t s)
+void launch_cell_batch(int R, bool big, int grid, const BatchDesc *D, int n, const IOPtrs &io, char *ws, hipStream_t s)
 {
+    if (big) {
+        hipLaunchKernelGGL((k_cell_batch<OPK, GEO, 8, true>), dim3(grid), dim3(kBlock), 0, s, D, n, io, ws);
+        return;
+    }
     switch (R) {
-    case 1: hipLaunchKernelGGL((k_cell_batch<OPK, GEO, 1>), dim3(grid), dim3(kBlock), 0, s, D, n, io, ws); break;
-    case 2: hipLaunchKernelGGL((k_cell_batch<OPK, GEO, 2>), dim3(grid), dim3(kBlock), 0, s, D, n, io, ws); break;
-    case 4: hipLaunchKernelGGL((k_cell_batch<OPK, GEO, 4>), dim3(grid), dim3(kBlock), 0, s, D, n, io, ws); break;
-    default: hipLaunchKernelGGL((k_cell_batch<OPK, GEO, 8>), dim3(grid), dim3(kBlock), 0, s, D, n, io, ws); break;
+    case 1: hipLaunchKernelGGL((k_cell_batch<OPK, GEO, 1, false>), dim3(grid), dim3(kBlock), 0, s, D, n, io, ws); break;
+    case 2: hipLaunchKernelGGL((k_cell_batch<OPK, GEO, 2, false>), dim3(grid), dim3(kBlock), 0, s, D, n, io, ws); break;
+    case 4: hipLaunchKernelGGL((k_cell_batch<OPK, GEO, 4, false>), dim3(grid), dim3(kBlock), 0, s, D, n, io, ws); break;
+    default: hipLaunchKernelGGL((k_cell_batch<OPK, GEO, 8, false>), dim3(grid), dim3(kBlock), 0, s, D, n, io, ws); break;
     }
 }
 
@@ -201,13 +206,13 @@ void launch_voxel_batch(int KC, int grid, const BatchDesc *D, int n, const IOPtr
 }
 
 template <int GEO>
-void dispatch_gather(const Route &rt, int R, int KC, int grid, const BatchDesc *D, int n, const IOPtrs &io, char *ws,
-                     hipStream_t s)
+void dispatch_gather(const Route &rt, int R, bool big, int KC, int grid, const BatchDesc *D, int n, const IOPtrs &io,
+                     char *ws, hipStream_t s)
 {
     if (rt.cell) {
-        if (rt.kop == CELL_A) launch_cell_batch<CELL_A, GEO>(R, grid, D, n, io, ws, s);
-        else if (rt.kop == CELL_BT) launch_cell_batch<CELL_BT, GEO>(R, grid, D, n, io, ws, s);
-        else launch_cell_batch<CELL_BWT, GEO>(R, grid, D, n, io, ws, s);
+        if (rt.kop == CELL_A) launch_cell_batch<CELL_A, GEO>(R, big, grid, D, n, io, ws, s);
+        else if (rt.kop == CELL_BT) launch_cell_batch<CELL_BT, GEO>(R, big, grid, D, n, io, ws, s);
+        else launch_cell_batch<CELL_BWT, GEO>(R, big, grid, D, n, io, ws, s);
     } else {
         if (rt.kop == VOX_B) launch_voxel_batch<VOX_B, GEO>(KC, grid, D, n, io, ws, s);
         else if (rt.kop == VOX_BW) launch_voxel_batch<VOX_BW, GEO>(KC, grid, D, n, io, ws, s);
@@ -341,6 +346,9 @@ int xr_batch3d_execute(const xr_batch3d *b, int op, const float *const *in, floa
     }
     if (!xr::valid_rows(R)) return fail(XR_EINVAL, "tile_rows must be 1, 2, 4 or 8");
     if (!xr::valid_chunk(KC)) return fail(XR_EINVAL, "voxel_chunk must be 1, 4, 8 or 16");
+    bool big = false;
+    for (const xr::BatchDesc &d : b->desc) big = big || d.big;
+    if (big) R = 8;  // tables of 4 GiB and more: the 64-bit general variant with 8-row tiles
     hipStream_t s = static_cast<hipStream_t>(stream);
     char *ws = static_cast<char *>(workspace);
     const xr::BatchDesc *D = b->d_desc;
@@ -367,9 +375,9 @@ int xr_batch3d_execute(const xr_batch3d *b, int op, const float *const *in, floa
         else if (kind == xr::GEO_ARC) xr::dispatch_sino_prefix<xr::GEO_ARC>(at, (int)g_pre, D, n, io, ws, s);
         else xr::dispatch_sino_prefix<xr::GEO_PAR>(at, (int)g_pre, D, n, io, ws, s);
     }
-    if (kind == xr::GEO_FLAT) xr::dispatch_gather<xr::GEO_FLAT>(rt, R, KC, (int)g_gather, D, n, io, ws, s);
-    else if (kind == xr::GEO_ARC) xr::dispatch_gather<xr::GEO_ARC>(rt, R, KC, (int)g_gather, D, n, io, ws, s);
-    else xr::dispatch_gather<xr::GEO_PAR>(rt, R, KC, (int)g_gather, D, n, io, ws, s);
+    if (kind == xr::GEO_FLAT) xr::dispatch_gather<xr::GEO_FLAT>(rt, R, big, KC, (int)g_gather, D, n, io, ws, s);
+    else if (kind == xr::GEO_ARC) xr::dispatch_gather<xr::GEO_ARC>(rt, R, big, KC, (int)g_gather, D, n, io, ws, s);
+    else xr::dispatch_gather<xr::GEO_PAR>(rt, R, big, KC, (int)g_gather, D, n, io, ws, s);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(XR_EHIP, std::string("batched launch failed: ") + hipGetErrorString(e));
     return XR_OK;
