@@ -180,6 +180,29 @@ def summarize_rocpd(paths):
             print(f"{short:70s} {len(d):5d} {sum(d) / len(d) / 1e3:10.2f} {min(d) / 1e3:10.2f} {max(d) / 1e3:10.2f}  {ctr}")
 
 
+def traffic_from_profiles(kernel: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 summaries: a line
+    "traffic <kernel> fetch_bytes=F write_bytes=W total_bytes=T source=<pmc runs>" in
+    profiles/*.txt, written from FETCH_SIZE / WRITE_SIZE passes scaled by the copy-kernel
+    calibration.  None when no summary covers this kernel."""
+    pdir = os.path.join(ROOT, "profiles")
+    if not os.path.isdir(pdir):
+        return None, None
+    for name in sorted(os.listdir(pdir)):
+        if not name.endswith(".txt"):
+            continue
+        with open(os.path.join(pdir, name)) as f:
+            for line in f:
+                if not line.startswith("traffic "):
+                    continue
+                head, _, rest = line[len("traffic "):].partition(" fetch_bytes=")
+                if head.strip() != kernel:
+                    continue
+                fields = dict(kv.split("=", 1) for kv in ("fetch_bytes=" + rest).split() if "=" in kv)
+                return float(fields["total_bytes"]), f"profiles/{name}"
+    return None, None
+
+
 def calibrate_copy():
     import torch
 
@@ -320,8 +343,12 @@ def main():
     value = vox_views / (ms_step * 1e-3)
     nnz = nnz_of(WORKLOAD)
     flops = 4.0 * nnz
-    dominant, t_dom = ("cell_gather(A)", t_cell) if t_cell >= t_vox else ("voxel_gather(B)", t_vox)
+    if t_cell >= t_vox:
+        dominant, t_dom, symbol = "cell_gather(A)", t_cell, f"k_cell<0, 1, {tune_a[0]}, false>"
+    else:
+        dominant, t_dom, symbol = "voxel_gather(B)", t_vox, f"k_voxel<0, 1, {tune_b[1]}>"
     achieved = flops / (t_dom * 1e-3) / 1e12
+    traffic, traffic_src = traffic_from_profiles(symbol)
 
     cpu = None
     if not args.no_cpu:
@@ -345,10 +372,12 @@ def main():
         "config": {"workload": "cone-flat A+B: volume 256x256x64 @1mm, flat detector 512x64 @1mm, "
                                "SOD 500 mm, SDD 1000 mm, 720 views over 2pi, B*C=1",
                    "voxel_views_per_operator": vox_views, "nnz_per_operator": nnz},
-        "roofline": {"bound": "valu", "kernel": dominant, "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
+        "roofline": {"bound": "valu", "kernel": dominant, "symbol": symbol, "achieved": achieved,
+                     "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS,
+                     "traffic": traffic, "traffic_source": traffic_src,
                      "note": "FP32 vector-ALU roof (packed FP32 peak); no dense contraction, so no MFMA; "
-                             "F = 4*nnz per launch"},
+                             "F = 4*nnz per launch; traffic = HBM bytes per launch (rocprofv3 FETCH_SIZE x2.0 "
+                             "+ WRITE_SIZE x1.0, copy-kernel calibrated)"},
         "tiles": {"A_tile_rows": tune_a[0], "B_voxel_chunk": tune_b[1]},
         "kernels_ms": {"cell_gather_A": t_cell, "voxel_gather_B": t_vox,
                        "prefix_passes": ms_step - t_cell - t_vox},
