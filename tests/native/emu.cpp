@@ -5,6 +5,7 @@
 // "thread" at a time on the CPU.  The CPU test suite uses it to check the kernel logic
 // against the oracle without a GPU; the GPU suite checks the compiled kernels.  It is
 // not part of the product library and nothing in xrecon_amd loads it.
+#include <algorithm>
 #include <cstring>
 #include <string>
 #include <thread>
@@ -64,9 +65,61 @@ void run_cell(const Plan3d &p, int R, const float2 *WX, const float2 *WY, float 
     else run_cell_R<OPK, GEO, 8>(p, WX, WY, out, nb, nt);
 }
 
+// Voxel-gather variant: 0 = the workgroup path the library launches (per-view block
+// staged in "LDS", lookups outside it from the table), 1 = every lookup from the table.
+int g_voxel_variant = 0;
+
+// Workgroup emulation of the staged voxel gather: per view, the block estimate, all
+// threads' staging shares, then every thread's gather -- the order the barriers impose.
+template <int OPK, int GEO, int KC>
+void run_voxel_wg_KC(const Plan3d &p, const float2 *Q, float *out, int nb, int nthreads)
+{
+    const VoxArgs a = make_vox_args(p, nb, KC);
+    const int gx = (a.nx + kVoxTile - 1) / kVoxTile, gy = (a.ny + kVoxTile - 1) / kVoxTile;
+    const int items = gx * gy * a.nchunk * nb;
+    std::vector<std::thread> th;
+    for (int t = 0; t < nthreads; ++t) {
+        th.emplace_back([&, t]() {
+            std::vector<float2> tile(kVoxLds);
+            std::vector<float> acc((size_t)kBlock * KC);
+            for (int it = t; it < items; it += nthreads) {
+                const int bx = it % gx, by = (it / gx) % gy, bz = it / (gx * gy);
+                const int k0 = (bz % a.nchunk) * KC, b = bz / a.nchunk;
+                const int i0 = bx * kVoxTile, j0 = by * kVoxTile;
+                const int i1 = imin(i0 + kVoxTile - 1, a.nx - 1), j1 = imin(j0 + kVoxTile - 1, a.ny - 1);
+                std::fill(acc.begin(), acc.end(), 0.0f);
+                for (int v = 0; v < a.nview; ++v) {
+                    const ViewHdr &h = p.d_hdr[v];
+                    const float4 *ct = p.d_col + (int64_t)v * (a.nu + 1);
+                    const float2 *Qv = view_table(Q, a, b, v);
+                    const VoxBox box = voxel_box<GEO, KC>(h, p.d_lineV, ct, a, v, i0, i1, j0, j1, k0);
+                    for (int tid = 0; tid < kBlock; ++tid) voxel_stage(Qv, box, tile.data(), a.nu, tid, kBlock);
+                    for (int tid = 0; tid < kBlock; ++tid) {
+                        int i, j;
+                        voxel_coords(bx, by, tid, i, j);
+                        if (i >= a.nx || j >= a.ny) continue;
+                        voxel_view<OPK, GEO, KC>(h, p.d_lineV, ct, Qv, a, v, i, j, k0, voxel_px(a, i), voxel_py(a, j),
+                                                 &acc[(size_t)tid * KC], box, tile.data());
+                    }
+                }
+                for (int tid = 0; tid < kBlock; ++tid) {
+                    int i, j;
+                    voxel_coords(bx, by, tid, i, j);
+                    if (i < a.nx && j < a.ny) voxel_store<KC>(out, a, i, j, k0, b, &acc[(size_t)tid * KC]);
+                }
+            }
+        });
+    }
+    for (auto &x : th) x.join();
+}
+
 template <int OPK, int GEO, int KC>
 void run_voxel_KC(const Plan3d &p, const float2 *Q, float *out, int nb, int nthreads)
 {
+    if (g_voxel_variant == 0) {
+        run_voxel_wg_KC<OPK, GEO, KC>(p, Q, out, nb, nthreads);
+        return;
+    }
     const VoxArgs a = make_vox_args(p, nb, KC);
     const int items = nb * a.nchunk * a.ny;
     std::vector<std::thread> th;
@@ -130,6 +183,8 @@ extern "C" {
 const char *emu_error(void) { return g_err.c_str(); }
 
 void emu_set_cell_variant(int v) { g_cell_variant = v; }
+
+void emu_set_voxel_variant(int v) { g_voxel_variant = v; }
 
 // Per-view span class of the cell gather (ViewHdr::span) for a geometry.
 int emu_view_spans(const int *vn, const float *vf, const int *gn, const float *gf, const float *angles, int nview,

@@ -130,3 +130,20 @@ def test_span_classes_cover_fixed_and_general_paths():
     x = np.random.default_rng(8).random(vol).astype(np.float32)
     for op in ("A", "BT", "BWT"):
         assert rel_l2(emu_op3d(op, x, ang, opts), oracle.op3d(op, x.astype(np.float64), ang, opts)) <= 1e-5
+
+
+@pytest.mark.parametrize("geo", sorted(CASES3D))
+def test_voxel_gather_staged_and_direct_agree(geo):
+    """The voxel gather with the per-view block staged in LDS (lookups outside the block
+    read the table) and with every lookup from the table give the oracle's results."""
+    opts = CASES3D[geo]
+    V = 16
+    ang = angles_with_ties(V)
+    _, sino = shapes3d(opts, 2, V)
+    y = np.random.default_rng(9).random(sino).astype(np.float32)
+    for op in ("B", "BW", "AT"):
+        want = oracle.op3d(op, y.astype(np.float64), ang, opts)
+        staged = emu_op3d(op, y, ang, opts)
+        direct = emu_op3d(op, y, ang, opts, voxel_variant=1)
+        assert rel_l2(staged, want) <= 1e-5
+        assert np.array_equal(staged, direct)

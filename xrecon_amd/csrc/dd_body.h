@@ -190,6 +190,9 @@ struct __attribute__((aligned(8))) Pair2 {
     float x0, y0, x1, y1;
 };
 
+__host__ __device__ inline int imin(int a, int b) { return a < b ? a : b; }
+__host__ __device__ inline int imax(int a, int b) { return a > b ? a : b; }
+
 // x - floor(x) for x >= 0: one v_fract_f32 on the device.
 __host__ __device__ inline float fractf(float x)
 {
@@ -468,103 +471,251 @@ __host__ __device__ inline void cell_body(const ViewHdr *__restrict__ hdr, const
 
 // ---------------------------------------------------------------- voxel gather
 
-// One thread: voxel column (i, j), slices k0..k0+KC-1, batch b.
-template <int OPK, int GEO, int KC>
-__host__ __device__ inline void voxel_body(const ViewHdr *__restrict__ hdr, const float4 *__restrict__ lineV,
-                                           const float4 *__restrict__ colt, const float2 *__restrict__ Q,
-                                           float *__restrict__ out, const VoxArgs &a, int i, int j, int k0, int b)
+// First column index to visit for axis point a: the closed-form inverse of the column map
+// (projective for flat and parallel, atan of a projective for arc).  A seed only: callers
+// correct it against the exact FP32 table.
+template <int GEO>
+__host__ __device__ inline float column_seed(const ViewHdr &h, float a)
 {
-    const float px = ((float)i - 0.5f * a.nx + 0.5f) * a.dx + a.shx;
-    const float py = (0.5f * a.ny - (float)j - 0.5f) * a.dy + a.shy;
+    const float T = seed_div(fmaf(h.tp1, a, h.tp0), fmaf(h.tq1, a, h.tq0));
+    return h.tk0 + h.tk1 * (GEO == GEO_ARC ? atanf(T) : T);
+}
+
+// Block of one view's row-prefix table staged in LDS for a voxel-gather workgroup:
+// columns c0 .. c0+nc-1 and rows r0 .. r0+nr-1, stored row-major with pitch nc.
+// nc = 0: nothing staged.
+struct VoxBox {
+    int c0, nc, r0, nr;
+};
+
+// Capacity of the staged block, in float2 entries (32 KiB of LDS per workgroup).
+constexpr int kVoxLds = 4096;
+
+// The block that voxel columns [i0, i1] x [j0, j1], slices [k0, k0 + KC) read from view v,
+// estimated from the tile's corner pixels (the axis map of a pixel edge is monotone in
+// the pixel and in the line, the depth magnification monotone in the line) with a
+// margin of one column and one row.  An estimate only: every lookup checks the block and
+// reads the table in HBM when it falls outside, so the block decides speed, not results.
+template <int GEO, int KC>
+__host__ __device__ inline VoxBox voxel_box(const ViewHdr &h, const float4 *__restrict__ lineV,
+                                            const float4 *__restrict__ ct, const VoxArgs &a, int v, int i0, int i1,
+                                            int j0, int j1, int k0)
+{
+    VoxBox bx{0, 0, 0, 0};
+    int La, Lb, mlo, mhi;
+    if (h.xdrv) { La = j0; Lb = j1; mlo = i0; mhi = i1; }
+    else { La = i0; Lb = i1; mlo = a.ny - 1 - j1; mhi = a.ny - 1 - j0; }
+    const float4 la = lineV[(int64_t)v * a.maxline + La], lb = lineV[(int64_t)v * a.maxline + Lb];
+    const float alo = fminf(fmaf((float)mlo, la.y, la.x), fmaf((float)mlo, lb.y, lb.x));
+    const float ahi = fmaxf(fmaf((float)(mhi + 1), la.y, la.x), fmaf((float)(mhi + 1), lb.y, lb.x));
+    if (!(ahi > h.dlo && alo < h.dhi)) return bx;
+    const int nu = a.nu;
+    const int c0 = imax((int)clampf(column_seed<GEO>(h, alo), 0.0f, (float)(nu - 1)) - 1, 0);
+    const int c1 = imin((int)clampf(column_seed<GEO>(h, ahi), 0.0f, (float)(nu - 1)) + 1, nu - 1);
+    if (c1 < c0) return bx;
+    // rho = zeta * z * kappa - v0/dvv over the corner lines, the end columns and the chunk's
+    // bottom and top slice boundaries.
+    const float zlo = fmaf((float)k0, a.dz, a.zb), zhi = fmaf((float)(k0 + KC), a.dz, a.zb);
+    const float ka = ct[c0].z, kb = ct[c1].z;
+    float lo = 3.0e38f, hi = -3.0e38f;
+    const float zs[2] = {zlo, zhi}, zes[2] = {la.z, lb.z}, ks[2] = {ka, kb};
+    for (int p = 0; p < 2; ++p)
+        for (int q = 0; q < 2; ++q)
+            for (int r = 0; r < 2; ++r) {
+                const float t = zes[p] * zs[q] * ks[r];
+                lo = fminf(lo, t);
+                hi = fmaxf(hi, t);
+            }
+    lo -= a.rv0;
+    hi -= a.rv0;
+    if (!(hi > -1.0f && lo < (float)a.nv + 1.0f)) return bx;
+    lo = fmaxf(lo, -1.0f);
+    hi = fminf(hi, (float)a.nv + 1.0f);
+    const int r0 = imax((int)floorf(lo) - 1, 0);
+    const int r1 = imin((int)floorf(hi) + 1, a.nv);
+    const int nc = c1 - c0 + 1;
+    int nr = r1 - r0 + 1;
+    if (nr <= 0 || nc > kVoxLds) return bx;
+    if (nc * nr > kVoxLds) nr = kVoxLds / nc;
+    bx.c0 = c0;
+    bx.nc = nc;
+    bx.r0 = r0;
+    bx.nr = nr;
+    return bx;
+}
+
+// Thread tid's share of copying the block from view v's row-prefix table Qv into LDS
+// (entries e = tid, tid + nthreads, ...; consecutive threads take consecutive columns).
+__host__ __device__ inline void voxel_stage(const float2 *__restrict__ Qv, const VoxBox &bx, float2 *__restrict__ lds,
+                                            int nu, int tid, int nthreads)
+{
+    const int E = bx.nc * bx.nr;
+    const float inv = 1.0f / (float)(bx.nc > 0 ? bx.nc : 1);
+    for (int e = tid; e < E; e += nthreads) {
+        // e / nc exactly: (e + 1/2) / nc stays >= 1/(2 nc) from an integer, far above the
+        // rounding of the product for E <= kVoxLds.
+        const int r = (int)(((float)e + 0.5f) * inv);
+        const int c = e - r * bx.nc;
+        lds[e] = Qv[(int64_t)(bx.r0 + r) * nu + bx.c0 + c];
+    }
+}
+
+// One view's contribution to the KC slice accumulators of voxel column (i, j).  With a
+// staged block (lds != nullptr), lookups inside it read LDS, the rest the table in HBM.
+template <int OPK, int GEO, int KC>
+__host__ __device__ inline void voxel_view(const ViewHdr &h, const float4 *__restrict__ lineV,
+                                           const float4 *__restrict__ ct, const float2 *__restrict__ Qv,
+                                           const VoxArgs &a, int v, int i, int j, int k0, float px, float py,
+                                           float *acc, const VoxBox &bx, const float2 *__restrict__ lds)
+{
     const float fnv = (float)a.nv;
     const int nu = a.nu;
+    int L, m;
+    if (h.xdrv) { L = j; m = i; } else { L = i; m = a.ny - 1 - j; }
+    const float4 lv = lineV[(int64_t)v * a.maxline + L];  // {P0, Delta, zeta, 1/(Delta zeta)}
+    const float a0 = fmaf((float)m, lv.y, lv.x);
+    const float a1 = a0 + lv.y;
+    if (!(a1 > h.dlo && a0 < h.dhi)) return;
 
-    float acc[KC];
-    for (int q = 0; q < KC; ++q) acc[q] = 0.0f;
-
-    for (int v = 0; v < a.nview; ++v) {
-        const ViewHdr h = hdr[v];
-        int L, m;
-        if (h.xdrv) { L = j; m = i; } else { L = i; m = a.ny - 1 - j; }
-        const float4 lv = lineV[(int64_t)v * a.maxline + L];  // {P0, Delta, zeta, 1/(Delta zeta)}
-        const float a0 = fmaf((float)m, lv.y, lv.x);
-        const float a1 = a0 + lv.y;
-        if (!(a1 > h.dlo && a0 < h.dhi)) continue;
-
-        float hB = 1.0f;
-        if (OPK != VOX_AT) {
-            // U * w / A_vox with A_vox = Delta * zeta * dz (include/cpu3d_common.h:80-85),
-            // times the row pitch dvv of the depth overlap measured in row units.
-            const float U = (OPK == VOX_BW) ? fdk_weight<GEO>(h.sx, h.sy, px, py, a.sod) : 1.0f;
-            hB = U * h.colw * a.hscale * lv.w;
-        }
-
-        // A column at or before the first one whose right boundary exceeds a0: closed-form
-        // inverse, corrected downwards against the table.  Columns between it and the
-        // footprint have no overlap and are skipped by the walk below, so the result is
-        // exact w.r.t. the FP32 boundaries whatever the seed's rounding.
-        const float T = seed_div(fmaf(h.tp1, a0, h.tp0), fmaf(h.tq1, a0, h.tq0));
-        const float tc = h.tk0 + h.tk1 * (GEO == GEO_ARC ? atanf(T) : T);
-        int c = (int)clampf(tc, 0.0f, (float)(nu - 1));
-        const float4 *ct = colt + (int64_t)v * (nu + 1);
-        float4 cc = ct[c], cn = ct[c + 1];
-        while (c > 0 && cc.x > a0) { --c; cn = cc; cc = ct[c]; }
-
-        const float zbz = lv.z * a.zb, zsz = lv.z * a.dz;
-        // Byte offsets below 2^32 within one view's table ((Nv+1)*Nu*8, checked at plan
-        // creation) over a wave-uniform base.
-        const char *Qv = reinterpret_cast<const char *>(Q + ((int64_t)b * a.nview + v) * (int64_t)(a.nv + 1) * nu);
-        const unsigned nu8 = (unsigned)nu * 8u;
-        while (c < nu && cc.x < a1) {
-            // prefetch the next column's table entry one iteration ahead
-            const float4 cnn = ct[c + 2 <= nu ? c + 2 : nu];
-            const float ou = fminf(a1, cn.x) - fmaxf(a0, cc.x);
-            if (ou > 0.0f) {
-                // Row coordinate of slice boundary k: rho_k = (zeta z_k / sigma - v0) / dvv.
-                const float kap = cc.z;
-                const float step = zsz * kap;
-                const float base = fmaf(zbz, kap, -a.rv0);
-                const float rlo = fmaf((float)k0, step, base);
-                const float rhi = fmaf((float)(k0 + KC), step, base);
-                if (rhi > 0.0f && rlo < fnv) {
-                    // B: ou * U w sigma dvv / (Delta zeta dz); A^T: ou (normalisation folded
-                    // into the prefix table).
-                    const float w = (OPK == VOX_AT) ? ou : ou * hB * cc.y;
-                    const unsigned c8 = (unsigned)c * 8u;
-                    // All KC+1 boundary lookups are independent: compute the rows, issue
-                    // the loads back to back, then consume.
-                    unsigned off[KC + 1];
-                    float fr[KC + 1];
-#pragma unroll
-                    for (int q = 0; q <= KC; ++q) {
-                        const float rho = clampf(fmaf((float)q, step, rlo), 0.0f, fnv);
-                        off[q] = mul24((unsigned)(int)rho, nu8) + c8;
-                        fr[q] = fractf(rho);
-                    }
-                    float2 e[KC + 1];
-#pragma unroll
-                    for (int q = 0; q <= KC; ++q) e[q] = *reinterpret_cast<const float2 *>(Qv + off[q]);
-                    float Ip = fmaf(fr[0], e[0].y, e[0].x);
-#pragma unroll
-                    for (int q = 0; q < KC; ++q) {
-                        const float In = fmaf(fr[q + 1], e[q + 1].y, e[q + 1].x);
-                        acc[q] = fmaf(w, In - Ip, acc[q]);
-                        Ip = In;
-                    }
-                }
-            }
-            ++c;
-            cc = cn;
-            cn = cnn;
-        }
+    float hB = 1.0f;
+    if (OPK != VOX_AT) {
+        // U * w / A_vox with A_vox = Delta * zeta * dz (include/cpu3d_common.h:80-85),
+        // times the row pitch dvv of the depth overlap measured in row units.
+        const float U = (OPK == VOX_BW) ? fdk_weight<GEO>(h.sx, h.sy, px, py, a.sod) : 1.0f;
+        hB = U * h.colw * a.hscale * lv.w;
     }
 
+    // A column at or before the first one whose right boundary exceeds a0: closed-form
+    // seed, corrected downwards against the table.  Columns between it and the footprint
+    // have no overlap and are skipped by the walk below, so the result is exact w.r.t. the
+    // FP32 boundaries whatever the seed's rounding.
+    int c = (int)clampf(column_seed<GEO>(h, a0), 0.0f, (float)(nu - 1));
+    float4 cc = ct[c], cn = ct[c + 1];
+    while (c > 0 && cc.x > a0) { --c; cn = cc; cc = ct[c]; }
+
+    const float zbz = lv.z * a.zb, zsz = lv.z * a.dz;
+    // Byte offsets below 2^32 within one view's table ((Nv+1)*Nu*8, checked at plan
+    // creation) over a wave-uniform base.
+    const char *Qb = reinterpret_cast<const char *>(Qv);
+    const unsigned nu8 = (unsigned)nu * 8u;
+    const unsigned nc8 = (unsigned)bx.nc * 8u;
+    while (c < nu && cc.x < a1) {
+        // prefetch the next column's table entry one iteration ahead
+        const float4 cnn = ct[c + 2 <= nu ? c + 2 : nu];
+        const float ou = fminf(a1, cn.x) - fmaxf(a0, cc.x);
+        if (ou > 0.0f) {
+            // Row coordinate of slice boundary k: rho_k = (zeta z_k / sigma - v0) / dvv.
+            const float kap = cc.z;
+            const float step = zsz * kap;
+            const float base = fmaf(zbz, kap, -a.rv0);
+            const float rlo = fmaf((float)k0, step, base);
+            const float rhi = fmaf((float)(k0 + KC), step, base);
+            if (rhi > 0.0f && rlo < fnv) {
+                // B: ou * U w sigma dvv / (Delta zeta dz); A^T: ou (normalisation folded
+                // into the prefix table).
+                const float w = (OPK == VOX_AT) ? ou : ou * hB * cc.y;
+                // All KC+1 boundary lookups are independent: compute the rows, issue the
+                // loads back to back, then consume.
+                int ri[KC + 1];
+                float fr[KC + 1];
+#pragma unroll
+                for (int q = 0; q <= KC; ++q) {
+                    const float rho = clampf(fmaf((float)q, step, rlo), 0.0f, fnv);
+                    ri[q] = (int)rho;
+                    fr[q] = fractf(rho);
+                }
+                float2 e[KC + 1];
+                // rows increase with q, so the first and last lookups bound the column's rows
+                const bool staged = lds != nullptr && (unsigned)(c - bx.c0) < (unsigned)bx.nc && ri[0] >= bx.r0 &&
+                                    ri[KC] < bx.r0 + bx.nr;
+                if (staged) {
+                    const char *Lb = reinterpret_cast<const char *>(lds);
+                    const unsigned cb = (unsigned)(c - bx.c0) * 8u;
+#pragma unroll
+                    for (int q = 0; q <= KC; ++q)
+                        e[q] = *reinterpret_cast<const float2 *>(Lb + (mul24((unsigned)(ri[q] - bx.r0), nc8) + cb));
+                } else {
+                    const unsigned c8 = (unsigned)c * 8u;
+#pragma unroll
+                    for (int q = 0; q <= KC; ++q)
+                        e[q] = *reinterpret_cast<const float2 *>(Qb + (mul24((unsigned)ri[q], nu8) + c8));
+                }
+                float Ip = fmaf(fr[0], e[0].y, e[0].x);
+#pragma unroll
+                for (int q = 0; q < KC; ++q) {
+                    const float In = fmaf(fr[q + 1], e[q + 1].y, e[q + 1].x);
+                    acc[q] = fmaf(w, In - Ip, acc[q]);
+                    Ip = In;
+                }
+            }
+        }
+        ++c;
+        cc = cn;
+        cn = cnn;
+    }
+}
+
+__host__ __device__ inline float voxel_px(const VoxArgs &a, int i) { return ((float)i - 0.5f * a.nx + 0.5f) * a.dx + a.shx; }
+__host__ __device__ inline float voxel_py(const VoxArgs &a, int j) { return (0.5f * a.ny - (float)j - 0.5f) * a.dy + a.shy; }
+
+__host__ __device__ inline const float2 *view_table(const float2 *Q, const VoxArgs &a, int b, int v)
+{
+    return Q + ((int64_t)b * a.nview + v) * (int64_t)(a.nv + 1) * a.nu;
+}
+
+template <int KC>
+__host__ __device__ inline void voxel_store(float *__restrict__ out, const VoxArgs &a, int i, int j, int k0, int b,
+                                            const float *acc)
+{
     float *o = out + (int64_t)b * a.nz * a.ny * a.nx + (int64_t)j * a.nx + i;
     for (int q = 0; q < KC; ++q) {
         const int k = k0 + q;
         if (k >= a.nz) break;
         o[(int64_t)k * a.ny * a.nx] = acc[q];
     }
+}
+
+// A workgroup of the voxel gather (device only): voxel columns [i0, i1] x [j0, j1] of the
+// tile, slices k0..k0+KC-1, batch b; this thread owns (i, j) when `active`.  Per view the
+// workgroup stages the block of the row-prefix table its footprint reads into LDS, then
+// every thread gathers from it.  Inactive threads take part in staging and barriers.
+template <int OPK, int GEO, int KC>
+__device__ inline void voxel_wg(const ViewHdr *__restrict__ hdr, const float4 *__restrict__ lineV,
+                                const float4 *__restrict__ colt, const float2 *__restrict__ Q,
+                                float *__restrict__ out, const VoxArgs &a, int i, int j, bool active, int i0, int i1,
+                                int j0, int j1, int k0, int b, float2 *__restrict__ tile)
+{
+    const float px = voxel_px(a, i), py = voxel_py(a, j);
+    float acc[KC];
+    for (int q = 0; q < KC; ++q) acc[q] = 0.0f;
+    for (int v = 0; v < a.nview; ++v) {
+        const ViewHdr h = hdr[v];
+        const float4 *ct = colt + (int64_t)v * (a.nu + 1);
+        const float2 *Qv = view_table(Q, a, b, v);
+        const VoxBox bx = voxel_box<GEO, KC>(h, lineV, ct, a, v, i0, i1, j0, j1, k0);
+        voxel_stage(Qv, bx, tile, a.nu, threadIdx.x, kBlock);
+        __syncthreads();
+        if (active) voxel_view<OPK, GEO, KC>(h, lineV, ct, Qv, a, v, i, j, k0, px, py, acc, bx, tile);
+        __syncthreads();
+    }
+    if (active) voxel_store<KC>(out, a, i, j, k0, b, acc);
+}
+
+// One thread, reading the tables in HBM only: voxel column (i, j), slices k0..k0+KC-1,
+// batch b.
+template <int OPK, int GEO, int KC>
+__host__ __device__ inline void voxel_body(const ViewHdr *__restrict__ hdr, const float4 *__restrict__ lineV,
+                                           const float4 *__restrict__ colt, const float2 *__restrict__ Q,
+                                           float *__restrict__ out, const VoxArgs &a, int i, int j, int k0, int b)
+{
+    const float px = voxel_px(a, i), py = voxel_py(a, j);
+    float acc[KC];
+    for (int q = 0; q < KC; ++q) acc[q] = 0.0f;
+    const VoxBox none{0, 0, 0, 0};
+    for (int v = 0; v < a.nview; ++v)
+        voxel_view<OPK, GEO, KC>(hdr[v], lineV, colt + (int64_t)v * (a.nu + 1), view_table(Q, a, b, v), a, v, i, j,
+                                 k0, px, py, acc, none, nullptr);
+    voxel_store<KC>(out, a, i, j, k0, b, acc);
 }
 
 }  // namespace xr
