@@ -161,7 +161,7 @@ void dispatch(const Plan3d &p, const Route &rt, int R, int KC, const float *in, 
         else run_cell<CELL_BWT, GEO>(p, R, WX.data(), WY.data(), out, nb, nt);
     } else {
         const SinoArgs sa = make_sino_args(p, nb);
-        std::vector<float2> Q((size_t)nb * p.nview * (p.geo.nv + 1) * p.geo.nu);
+        std::vector<float2> Q((size_t)nb * p.nview * (p.geo.nv + 1) * p.geo.nu + kSinoPad);
         for (int b = 0; b < nb; ++b)
             for (int v = 0; v < p.nview; ++v)
                 for (int c = 0; c < p.geo.nu; ++c) {

@@ -164,7 +164,7 @@ template <int OPK, int GEO, int KC>
 __global__ __launch_bounds__(kBlock) void k_voxel_batch(const BatchDesc *__restrict__ D, int n, IOPtrs io,
                                                         char *__restrict__ ws)
 {
-    __shared__ float2 tile[kVoxLds];
+    __shared__ __attribute__((aligned(16))) float2 tile[kVoxLds];
     int bid = blockIdx.x;
     const int i = find_instance(D, n, bid, [](const BatchDesc &d) { return blocks_voxel(d, KC); });
     if (i < 0) return;

@@ -102,10 +102,14 @@ inline size_t vol_prefix_bytes(const Plan3d &p, int nbatch)
     return (vol_prefix_entries(p, nbatch) + kVolPad) * sizeof(float2);
 }
 
-// Workspace of the voxel gathers: row-prefix of the sinogram.
+// Workspace of the voxel gathers: row-prefix of the sinogram, plus kSinoPad entries after
+// it (the paired staging loads may read one entry past a view's last column; it is
+// never used).
+constexpr int kSinoPad = 2;
+
 inline size_t sino_prefix_bytes(const Plan3d &p, int nbatch)
 {
-    return (size_t)nbatch * p.nview * (size_t)(p.geo.nv + 1) * p.geo.nu * sizeof(float2);
+    return ((size_t)nbatch * p.nview * (size_t)(p.geo.nv + 1) * p.geo.nu + kSinoPad) * sizeof(float2);
 }
 
 // Default tile shapes (the autotuner may choose others).
@@ -482,10 +486,10 @@ __host__ __device__ inline float column_seed(const ViewHdr &h, float a)
 }
 
 // Block of one view's row-prefix table staged in LDS for a voxel-gather workgroup:
-// columns c0 .. c0+nc-1 and rows r0 .. r0+nr-1, stored row-major with pitch nc.
-// nc = 0: nothing staged.
+// columns c0 .. c0+nc-1 and rows r0 .. r0+nr-1, stored row-major with an even pitch >= nc
+// (whole 16-byte pairs per row).  nc = 0: nothing staged.
 struct VoxBox {
-    int c0, nc, r0, nr;
+    int c0, nc, r0, nr, pitch;
 };
 
 // Capacity of the staged block, in float2 entries (32 KiB of LDS per workgroup).
@@ -501,7 +505,7 @@ __host__ __device__ inline VoxBox voxel_box(const ViewHdr &h, const float4 *__re
                                             const float4 *__restrict__ ct, const VoxArgs &a, int v, int i0, int i1,
                                             int j0, int j1, int k0)
 {
-    VoxBox bx{0, 0, 0, 0};
+    VoxBox bx{0, 0, 0, 0, 0};
     int La, Lb, mlo, mhi;
     if (h.xdrv) { La = j0; Lb = j1; mlo = i0; mhi = i1; }
     else { La = i0; Lb = i1; mlo = a.ny - 1 - j1; mhi = a.ny - 1 - j0; }
@@ -534,29 +538,48 @@ __host__ __device__ inline VoxBox voxel_box(const ViewHdr &h, const float4 *__re
     const int r0 = imax((int)floorf(lo) - 1, 0);
     const int r1 = imin((int)floorf(hi) + 1, a.nv);
     const int nc = c1 - c0 + 1;
+    const int pitch = (nc + 1) & ~1;
     int nr = r1 - r0 + 1;
-    if (nr <= 0 || nc > kVoxLds) return bx;
-    if (nc * nr > kVoxLds) nr = kVoxLds / nc;
+    if (nr <= 0 || pitch > kVoxLds) return bx;
+    if (pitch * nr > kVoxLds) nr = kVoxLds / pitch;
     bx.c0 = c0;
     bx.nc = nc;
     bx.r0 = r0;
     bx.nr = nr;
+    bx.pitch = pitch;
     return bx;
 }
 
-// Thread tid's share of copying the block from view v's row-prefix table Qv into LDS
-// (entries e = tid, tid + nthreads, ...; consecutive threads take consecutive columns).
+// Thread tid's share of copying the block from view v's row-prefix table Qv into LDS, in
+// 16-byte pairs of adjacent columns (pairs p = tid, tid + nthreads, ...; at most
+// kVoxLds / 2 / kBlock per thread with nthreads = kBlock).  All loads are issued before
+// the stores.  A pair may end one entry past the view's last column (the next row's first
+// entry, or the pad after the table); that entry is never looked up.
 __host__ __device__ inline void voxel_stage(const float2 *__restrict__ Qv, const VoxBox &bx, float2 *__restrict__ lds,
                                             int nu, int tid, int nthreads)
 {
-    const int E = bx.nc * bx.nr;
-    const float inv = 1.0f / (float)(bx.nc > 0 ? bx.nc : 1);
-    for (int e = tid; e < E; e += nthreads) {
-        // e / nc exactly: (e + 1/2) / nc stays >= 1/(2 nc) from an integer, far above the
-        // rounding of the product for E <= kVoxLds.
-        const int r = (int)(((float)e + 0.5f) * inv);
-        const int c = e - r * bx.nc;
-        lds[e] = Qv[(int64_t)(bx.r0 + r) * nu + bx.c0 + c];
+    constexpr int S = kVoxLds / 2 / kBlock;
+    const int half = bx.pitch >> 1;
+    const int E2 = bx.nr * half;
+    const float inv = 1.0f / (float)(half > 0 ? half : 1);
+    const char *Qb = reinterpret_cast<const char *>(Qv);
+    const unsigned nu8 = (unsigned)nu * 8u;
+    Pair2 t[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        const int p = tid + s * nthreads;
+        if (p < E2) {
+            // p / half exactly: (p + 1/2) / half stays >= 1/(2 half) from an integer, far
+            // above the rounding of the product for p < kVoxLds.
+            const int r = (int)(((float)p + 0.5f) * inv);
+            const int c = 2 * (p - r * half);
+            t[s] = *reinterpret_cast<const Pair2 *>(Qb + (mul24((unsigned)(bx.r0 + r), nu8) + (unsigned)(bx.c0 + c) * 8u));
+        }
+    }
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        const int p = tid + s * nthreads;
+        if (p < E2) *reinterpret_cast<Pair2 *>(lds + 2 * p) = t[s];
     }
 }
 
@@ -598,7 +621,7 @@ __host__ __device__ inline void voxel_view(const ViewHdr &h, const float4 *__res
     // creation) over a wave-uniform base.
     const char *Qb = reinterpret_cast<const char *>(Qv);
     const unsigned nu8 = (unsigned)nu * 8u;
-    const unsigned nc8 = (unsigned)bx.nc * 8u;
+    const unsigned nc8 = (unsigned)bx.pitch * 8u;
     while (c < nu && cc.x < a1) {
         // prefetch the next column's table entry one iteration ahead
         const float4 cnn = ct[c + 2 <= nu ? c + 2 : nu];
@@ -711,7 +734,7 @@ __host__ __device__ inline void voxel_body(const ViewHdr *__restrict__ hdr, cons
     const float px = voxel_px(a, i), py = voxel_py(a, j);
     float acc[KC];
     for (int q = 0; q < KC; ++q) acc[q] = 0.0f;
-    const VoxBox none{0, 0, 0, 0};
+    const VoxBox none{0, 0, 0, 0, 0};
     for (int v = 0; v < a.nview; ++v)
         voxel_view<OPK, GEO, KC>(hdr[v], lineV, colt + (int64_t)v * (a.nu + 1), view_table(Q, a, b, v), a, v, i, j,
                                  k0, px, py, acc, none, nullptr);

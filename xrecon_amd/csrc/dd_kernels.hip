@@ -112,7 +112,7 @@ __global__ __launch_bounds__(kBlock) void k_voxel(const ViewHdr *__restrict__ hd
                                                   const float4 *__restrict__ colt, const float2 *__restrict__ Q,
                                                   float *__restrict__ out, VoxArgs a)
 {
-    __shared__ float2 tile[kVoxLds];
+    __shared__ __attribute__((aligned(16))) float2 tile[kVoxLds];
     int i, j;
     voxel_coords(blockIdx.x, blockIdx.y, threadIdx.x, i, j);
     const int chunk = blockIdx.z % a.nchunk;
