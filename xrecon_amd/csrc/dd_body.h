@@ -551,35 +551,36 @@ __host__ __device__ inline VoxBox voxel_box(const ViewHdr &h, const float4 *__re
 }
 
 // Thread tid's share of copying the block from view v's row-prefix table Qv into LDS, in
-// 16-byte pairs of adjacent columns (pairs p = tid, tid + nthreads, ...; at most
-// kVoxLds / 2 / kBlock per thread with nthreads = kBlock).  All loads are issued before
-// the stores.  A pair may end one entry past the view's last column (the next row's first
-// entry, or the pad after the table); that entry is never looked up.
+// 16-byte pairs of adjacent columns: pair p = tid + s * nthreads goes to lds[2p, 2p+1]
+// (at most kVoxLds / 2 pairs, so s < kVoxLds / 2 / nthreads).  A pair may end one entry
+// past the view's last column (the next row's first entry, or the pad after the table);
+// that entry is never looked up.
+//
+// On the device the copies are direct global-to-LDS loads (no registers held): lane l of
+// the wave writes 16 bytes at the wave's LDS base + 16 l, which is where pair p lands
+// for p = s * nthreads + 64 * wave + l.  Lanes past the block load the table's first
+// pair into unused LDS.  The caller waits for the loads (s_waitcnt) before its barrier.
 __host__ __device__ inline void voxel_stage(const float2 *__restrict__ Qv, const VoxBox &bx, float2 *__restrict__ lds,
                                             int nu, int tid, int nthreads)
 {
-    constexpr int S = kVoxLds / 2 / kBlock;
     const int half = bx.pitch >> 1;
     const int E2 = bx.nr * half;
     const float inv = 1.0f / (float)(half > 0 ? half : 1);
     const char *Qb = reinterpret_cast<const char *>(Qv);
     const unsigned nu8 = (unsigned)nu * 8u;
-    Pair2 t[S];
-#pragma unroll
-    for (int s = 0; s < S; ++s) {
+    for (int s = 0; s * nthreads < E2; ++s) {
         const int p = tid + s * nthreads;
-        if (p < E2) {
-            // p / half exactly: (p + 1/2) / half stays >= 1/(2 half) from an integer, far
-            // above the rounding of the product for p < kVoxLds.
-            const int r = (int)(((float)p + 0.5f) * inv);
-            const int c = 2 * (p - r * half);
-            t[s] = *reinterpret_cast<const Pair2 *>(Qb + (mul24((unsigned)(bx.r0 + r), nu8) + (unsigned)(bx.c0 + c) * 8u));
-        }
-    }
-#pragma unroll
-    for (int s = 0; s < S; ++s) {
-        const int p = tid + s * nthreads;
-        if (p < E2) *reinterpret_cast<Pair2 *>(lds + 2 * p) = t[s];
+        // p / half exactly: (p + 1/2) / half stays >= 1/(2 half) from an integer, far
+        // above the rounding of the product for p < kVoxLds.
+        const int r = (int)(((float)p + 0.5f) * inv);
+        const int c = 2 * (p - r * half);
+        const unsigned off = p < E2 ? mul24((unsigned)(bx.r0 + r), nu8) + (unsigned)(bx.c0 + c) * 8u : 0u;
+#if defined(__HIP_DEVICE_COMPILE__)
+        float2 *wave_base = lds + 2 * (s * nthreads + (tid & ~(kWave - 1)));
+        __builtin_amdgcn_global_load_lds(Qb + off, (__attribute__((address_space(3))) void *)wave_base, 16, 0, 0);
+#else
+        if (p < E2) *reinterpret_cast<Pair2 *>(lds + 2 * p) = *reinterpret_cast<const Pair2 *>(Qb + off);
+#endif
     }
 }
 
@@ -717,6 +718,7 @@ __device__ inline void voxel_wg(const ViewHdr *__restrict__ hdr, const float4 *_
         const float2 *Qv = view_table(Q, a, b, v);
         const VoxBox bx = voxel_box<GEO, KC>(h, lineV, ct, a, v, i0, i1, j0, j1, k0);
         voxel_stage(Qv, bx, tile, a.nu, threadIdx.x, kBlock);
+        __builtin_amdgcn_s_waitcnt(0);  // the direct-to-LDS loads of this wave have landed
         __syncthreads();
         if (active) voxel_view<OPK, GEO, KC>(h, lineV, ct, Qv, a, v, i, j, k0, px, py, acc, bx, tile);
         __syncthreads();
