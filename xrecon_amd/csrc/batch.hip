@@ -164,7 +164,8 @@ template <int OPK, int GEO, int KC>
 __global__ __launch_bounds__(kBlock) void k_voxel_batch(const BatchDesc *__restrict__ D, int n, IOPtrs io,
                                                         char *__restrict__ ws)
 {
-    __shared__ __attribute__((aligned(16))) float2 tile[kVoxLds];
+    __shared__ __attribute__((aligned(16))) float2 tile0[kVoxLds];
+    __shared__ __attribute__((aligned(16))) float2 tile1[kVoxLds];
     int bid = blockIdx.x;
     const int i = find_instance(D, n, bid, [](const BatchDesc &d) { return blocks_voxel(d, KC); });
     if (i < 0) return;
@@ -178,7 +179,8 @@ __global__ __launch_bounds__(kBlock) void k_voxel_batch(const BatchDesc *__restr
     const int i0 = bx * kVoxTile, j0 = by * kVoxTile;
     const int i1 = imin(i0 + kVoxTile - 1, a.nx - 1), j1 = imin(j0 + kVoxTile - 1, a.ny - 1);
     voxel_wg<OPK, GEO, KC>(d.hdr, d.lineV, d.col, reinterpret_cast<const float2 *>(ws + d.ws_sino), io.out[i], a, ii,
-                           j, ii < a.nx && j < a.ny, i0, i1, j0, j1, (bz % a.nchunk) * KC, bz / a.nchunk, tile);
+                           j, ii < a.nx && j < a.ny, i0, i1, j0, j1, (bz % a.nchunk) * KC, bz / a.nchunk, tile0,
+                           tile1);
 }
 
 template <int OPK, int GEO>

@@ -112,7 +112,8 @@ __global__ __launch_bounds__(kBlock) void k_voxel(const ViewHdr *__restrict__ hd
                                                   const float4 *__restrict__ colt, const float2 *__restrict__ Q,
                                                   float *__restrict__ out, VoxArgs a)
 {
-    __shared__ __attribute__((aligned(16))) float2 tile[kVoxLds];
+    __shared__ __attribute__((aligned(16))) float2 tile0[kVoxLds];
+    __shared__ __attribute__((aligned(16))) float2 tile1[kVoxLds];
     int i, j;
     voxel_coords(blockIdx.x, blockIdx.y, threadIdx.x, i, j);
     const int chunk = blockIdx.z % a.nchunk;
@@ -120,7 +121,7 @@ __global__ __launch_bounds__(kBlock) void k_voxel(const ViewHdr *__restrict__ hd
     const int i0 = blockIdx.x * kVoxTile, j0 = blockIdx.y * kVoxTile;
     const int i1 = imin(i0 + kVoxTile - 1, a.nx - 1), j1 = imin(j0 + kVoxTile - 1, a.ny - 1);
     voxel_wg<OPK, GEO, KC>(hdr, lineV, colt, Q, out, a, i, j, i < a.nx && j < a.ny, i0, i1, j0, j1, chunk * KC, b,
-                           tile);
+                           tile0, tile1);
 }
 
 template <int OPK, int GEO>
