@@ -149,7 +149,7 @@ void xr_batch3d_destroy(xr_batch3d *batch);
 /* Workspace (device bytes) that xr_batch3d_execute needs for `op`. */
 size_t xr_batch3d_workspace_bytes(const xr_batch3d *batch, int op);
 /* in[i] / out[i]: instance i's device buffers in the layouts above.  `tile_rows`
- * (1, 2, 4, 8; 0 = default) and `voxel_chunk` (1, 4, 8, 16; 0 = default) apply to all
+ * (1, 2, 4, 8, 16; 0 = default) and `voxel_chunk` (1, 4, 8, 16; 0 = default) apply to all
  * instances of the launch. */
 int xr_batch3d_execute(const xr_batch3d *batch, int op, const float *const *in, float *const *out,
                        void *workspace, size_t workspace_bytes, int tile_rows, int voxel_chunk,

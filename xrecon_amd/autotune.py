@@ -2,7 +2,7 @@
 
 Two knobs exist, one per kernel family (``Plan.kernel_kind(op)``):
 
-* cell gathers (A, B^T, B_W^T): ``tile_rows`` in {1, 2, 4, 8} -- detector rows per
+* cell gathers (A, B^T, B_W^T): ``tile_rows`` in {1, 2, 4, 8, 16} -- detector rows per
   thread.  More rows amortise each line's column overlaps and share row boundaries,
   but cost registers and therefore resident waves.
 * voxel gathers (B, B_W, A^T): ``voxel_chunk`` in {1, 4, 8, 16} -- slices per thread.
@@ -24,7 +24,7 @@ from . import _backend as be
 
 CELL = 1
 VOXEL = 2
-TILE_ROWS = (1, 2, 4, 8)
+TILE_ROWS = (1, 2, 4, 8, 16)
 VOXEL_CHUNKS = (1, 4, 8, 16)
 
 _CACHE: Dict[tuple, Tuple[int, int]] = {}

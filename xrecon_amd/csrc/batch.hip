@@ -164,8 +164,7 @@ template <int OPK, int GEO, int KC>
 __global__ __launch_bounds__(kBlock) void k_voxel_batch(const BatchDesc *__restrict__ D, int n, IOPtrs io,
                                                         char *__restrict__ ws)
 {
-    __shared__ __attribute__((aligned(16))) float2 tile0[kVoxLds];
-    __shared__ __attribute__((aligned(16))) float2 tile1[kVoxLds];
+    __shared__ __attribute__((aligned(16))) float2 tile[kVoxLds];
     int bid = blockIdx.x;
     const int i = find_instance(D, n, bid, [](const BatchDesc &d) { return blocks_voxel(d, KC); });
     if (i < 0) return;
@@ -179,8 +178,7 @@ __global__ __launch_bounds__(kBlock) void k_voxel_batch(const BatchDesc *__restr
     const int i0 = bx * kVoxTile, j0 = by * kVoxTile;
     const int i1 = imin(i0 + kVoxTile - 1, a.nx - 1), j1 = imin(j0 + kVoxTile - 1, a.ny - 1);
     voxel_wg<OPK, GEO, KC>(d.hdr, d.lineV, d.col, reinterpret_cast<const float2 *>(ws + d.ws_sino), io.out[i], a, ii,
-                           j, ii < a.nx && j < a.ny, i0, i1, j0, j1, (bz % a.nchunk) * KC, bz / a.nchunk, tile0,
-                           tile1);
+                           j, ii < a.nx && j < a.ny, i0, i1, j0, j1, (bz % a.nchunk) * KC, bz / a.nchunk, tile);
 }
 
 template <int OPK, int GEO>
@@ -194,6 +192,7 @@ void launch_cell_batch(int R, bool big, int grid, const BatchDesc *D, int n, con
     case 1: hipLaunchKernelGGL((k_cell_batch<OPK, GEO, 1, false>), dim3(grid), dim3(kBlock), 0, s, D, n, io, ws); break;
     case 2: hipLaunchKernelGGL((k_cell_batch<OPK, GEO, 2, false>), dim3(grid), dim3(kBlock), 0, s, D, n, io, ws); break;
     case 4: hipLaunchKernelGGL((k_cell_batch<OPK, GEO, 4, false>), dim3(grid), dim3(kBlock), 0, s, D, n, io, ws); break;
+    case 16: hipLaunchKernelGGL((k_cell_batch<OPK, GEO, 16, false>), dim3(grid), dim3(kBlock), 0, s, D, n, io, ws); break;
     default: hipLaunchKernelGGL((k_cell_batch<OPK, GEO, 8, false>), dim3(grid), dim3(kBlock), 0, s, D, n, io, ws); break;
     }
 }
@@ -348,7 +347,7 @@ int xr_batch3d_execute(const xr_batch3d *b, int op, const float *const *in, floa
         KC = 1;
         for (const xr::Plan3d *p : b->plans) KC = std::max(KC, xr::default_chunk(*p));
     }
-    if (!xr::valid_rows(R)) return fail(XR_EINVAL, "tile_rows must be 1, 2, 4 or 8");
+    if (!xr::valid_rows(R)) return fail(XR_EINVAL, "tile_rows must be 1, 2, 4, 8 or 16");
     if (!xr::valid_chunk(KC)) return fail(XR_EINVAL, "voxel_chunk must be 1, 4, 8 or 16");
     bool big = false;
     for (const xr::BatchDesc &d : b->desc) big = big || d.big;

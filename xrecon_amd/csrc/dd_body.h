@@ -131,7 +131,7 @@ inline int default_chunk(const Plan3d &p)
     return 1;
 }
 
-inline bool valid_rows(int r) { return r == 1 || r == 2 || r == 4 || r == 8; }
+inline bool valid_rows(int r) { return r == 1 || r == 2 || r == 4 || r == 8 || r == 16; }
 inline bool valid_chunk(int k) { return k == 1 || k == 4 || k == 8 || k == 16; }
 
 inline CellArgs make_cell_args(const Plan3d &p, int nbatch)
@@ -492,7 +492,7 @@ struct VoxBox {
     int c0, nc, r0, nr, pitch;
 };
 
-// Capacity of one staged block, in float2 entries (20 KiB; a workgroup holds two).  The
+// Capacity of the staged block, in float2 entries (20 KiB of LDS per workgroup).  The
 // blocks of 16 x 16 voxel columns x 16 slices of a 1 mm volume seen by a 0.5 mm (at the
 // isocentre) detector stay below 2400 entries.
 constexpr int kVoxLds = 2560;
@@ -704,52 +704,29 @@ __host__ __device__ inline void voxel_store(float *__restrict__ out, const VoxAr
 // A workgroup of the voxel gather (device only): voxel columns [i0, i1] x [j0, j1] of the
 // tile, slices k0..k0+KC-1, batch b; this thread owns (i, j) when `active`.  Per view the
 // workgroup stages the block of the row-prefix table its footprint reads into LDS, then
-// every thread gathers from it.  Two LDS buffers alternate: the direct loads of view
-// v+1's block are in flight while view v is gathered, and one barrier per view both
-// publishes v+1's block and retires v's buffer.  (The view loop is unrolled by two so
-// that each gather reads one fixed buffer while the other is being written.)  Inactive
-// threads take part in staging and barriers.
+// every thread gathers from it.  Inactive threads take part in staging and barriers.
+// (A second, alternating buffer that overlaps the next view's staging with this view's
+// gather measured slower: the gather's own table loads must wait for the earlier-issued
+// staging loads anyway, and the doubled code and registers cost more than they save.)
 template <int OPK, int GEO, int KC>
 __device__ inline void voxel_wg(const ViewHdr *__restrict__ hdr, const float4 *__restrict__ lineV,
                                 const float4 *__restrict__ colt, const float2 *__restrict__ Q,
                                 float *__restrict__ out, const VoxArgs &a, int i, int j, bool active, int i0, int i1,
-                                int j0, int j1, int k0, int b, float2 *__restrict__ tile0, float2 *__restrict__ tile1)
+                                int j0, int j1, int k0, int b, float2 *__restrict__ tile)
 {
     const float px = voxel_px(a, i), py = voxel_py(a, j);
     float acc[KC];
     for (int q = 0; q < KC; ++q) acc[q] = 0.0f;
-    const int V = a.nview;
-    auto box_of = [&](int v) {
-        return voxel_box<GEO, KC>(hdr[v], lineV, colt + (int64_t)v * (a.nu + 1), a, v, i0, i1, j0, j1, k0);
-    };
-    auto gather = [&](int v, const VoxBox &bx, const float2 *tile) {
-        if (active)
-            voxel_view<OPK, GEO, KC>(hdr[v], lineV, colt + (int64_t)v * (a.nu + 1), view_table(Q, a, b, v), a, v, i,
-                                     j, k0, px, py, acc, bx, tile);
-    };
-    VoxBox cur = V > 0 ? box_of(0) : VoxBox{0, 0, 0, 0, 0};
-    if (V > 0) voxel_stage(view_table(Q, a, b, 0), cur, tile0, a.nu, threadIdx.x, kBlock);
-    __builtin_amdgcn_s_waitcnt(0);  // this wave's direct-to-LDS loads have landed
-    __syncthreads();
-    for (int v = 0; v < V; v += 2) {
-        VoxBox nxt{0, 0, 0, 0, 0};
-        if (v + 1 < V) {
-            nxt = box_of(v + 1);
-            voxel_stage(view_table(Q, a, b, v + 1), nxt, tile1, a.nu, threadIdx.x, kBlock);
-        }
-        gather(v, cur, tile0);
-        __builtin_amdgcn_s_waitcnt(0);
+    for (int v = 0; v < a.nview; ++v) {
+        const ViewHdr h = hdr[v];
+        const float4 *ct = colt + (int64_t)v * (a.nu + 1);
+        const float2 *Qv = view_table(Q, a, b, v);
+        const VoxBox bx = voxel_box<GEO, KC>(h, lineV, ct, a, v, i0, i1, j0, j1, k0);
+        voxel_stage(Qv, bx, tile, a.nu, threadIdx.x, kBlock);
+        __builtin_amdgcn_s_waitcnt(0);  // this wave's direct-to-LDS loads have landed
         __syncthreads();
-        if (v + 1 >= V) break;
-        cur = nxt;
-        if (v + 2 < V) {
-            nxt = box_of(v + 2);
-            voxel_stage(view_table(Q, a, b, v + 2), nxt, tile0, a.nu, threadIdx.x, kBlock);
-        }
-        gather(v + 1, cur, tile1);
-        __builtin_amdgcn_s_waitcnt(0);
+        if (active) voxel_view<OPK, GEO, KC>(h, lineV, ct, Qv, a, v, i, j, k0, px, py, acc, bx, tile);
         __syncthreads();
-        cur = nxt;
     }
     if (active) voxel_store<KC>(out, a, i, j, k0, b, acc);
 }

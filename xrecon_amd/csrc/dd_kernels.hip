@@ -112,8 +112,7 @@ __global__ __launch_bounds__(kBlock) void k_voxel(const ViewHdr *__restrict__ hd
                                                   const float4 *__restrict__ colt, const float2 *__restrict__ Q,
                                                   float *__restrict__ out, VoxArgs a)
 {
-    __shared__ __attribute__((aligned(16))) float2 tile0[kVoxLds];
-    __shared__ __attribute__((aligned(16))) float2 tile1[kVoxLds];
+    __shared__ __attribute__((aligned(16))) float2 tile[kVoxLds];
     int i, j;
     voxel_coords(blockIdx.x, blockIdx.y, threadIdx.x, i, j);
     const int chunk = blockIdx.z % a.nchunk;
@@ -121,7 +120,7 @@ __global__ __launch_bounds__(kBlock) void k_voxel(const ViewHdr *__restrict__ hd
     const int i0 = blockIdx.x * kVoxTile, j0 = blockIdx.y * kVoxTile;
     const int i1 = imin(i0 + kVoxTile - 1, a.nx - 1), j1 = imin(j0 + kVoxTile - 1, a.ny - 1);
     voxel_wg<OPK, GEO, KC>(hdr, lineV, colt, Q, out, a, i, j, i < a.nx && j < a.ny, i0, i1, j0, j1, chunk * KC, b,
-                           tile0, tile1);
+                           tile);
 }
 
 template <int OPK, int GEO>
@@ -138,6 +137,7 @@ void launch_cell(const Plan3d &p, int R, const float2 *WX, const float2 *WY, flo
         case 1: XR_CELL(1, false); break;
         case 2: XR_CELL(2, false); break;
         case 4: XR_CELL(4, false); break;
+        case 16: XR_CELL(16, false); break;
         default: XR_CELL(8, false); break;
         }
     }
@@ -232,7 +232,7 @@ int execute_phase(const Plan3d &p, int op, int phase, const float *in, float *ou
     if (tile_rows == 0) tile_rows = default_rows(p);
     if (voxel_chunk == 0) voxel_chunk = default_chunk(p);
     if (tile_rows != 1 && tile_rows != 2 && tile_rows != 4 && tile_rows != 8)
-        return fail(XR_EINVAL, "tile_rows must be 1, 2, 4 or 8");
+        return fail(XR_EINVAL, "tile_rows must be 1, 2, 4, 8 or 16");
     if (voxel_chunk != 1 && voxel_chunk != 4 && voxel_chunk != 8 && voxel_chunk != 16)
         return fail(XR_EINVAL, "voxel_chunk must be 1, 4, 8 or 16");
 
