@@ -62,6 +62,7 @@ void run_cell(const Plan3d &p, int R, const float2 *WX, const float2 *WY, float 
     if (R == 1) run_cell_R<OPK, GEO, 1>(p, WX, WY, out, nb, nt);
     else if (R == 2) run_cell_R<OPK, GEO, 2>(p, WX, WY, out, nb, nt);
     else if (R == 4) run_cell_R<OPK, GEO, 4>(p, WX, WY, out, nb, nt);
+    else if (R == 16) run_cell_R<OPK, GEO, 16>(p, WX, WY, out, nb, nt);
     else run_cell_R<OPK, GEO, 8>(p, WX, WY, out, nb, nt);
 }
 

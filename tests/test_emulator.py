@@ -48,7 +48,7 @@ def test_2d_bodies_vs_oracle(geo, op):
     assert rel_l2(got, want) <= 1e-5
 
 
-@pytest.mark.parametrize("R,KC", [(1, 1), (2, 4), (4, 8), (8, 16)])
+@pytest.mark.parametrize("R,KC", [(1, 1), (2, 4), (4, 8), (8, 16), (16, 16)])
 def test_tilings_give_same_results(R, KC):
     """Tile shapes change only the work split; results agree with the oracle for each."""
     opts = CASES3D["flat"]

@@ -231,10 +231,8 @@ int execute_phase(const Plan3d &p, int op, int phase, const float *in, float *ou
         return fail(XR_EWORKSPACE, "workspace of " + std::to_string(ws_bytes) + " bytes, need " + std::to_string(need));
     if (tile_rows == 0) tile_rows = default_rows(p);
     if (voxel_chunk == 0) voxel_chunk = default_chunk(p);
-    if (tile_rows != 1 && tile_rows != 2 && tile_rows != 4 && tile_rows != 8)
-        return fail(XR_EINVAL, "tile_rows must be 1, 2, 4, 8 or 16");
-    if (voxel_chunk != 1 && voxel_chunk != 4 && voxel_chunk != 8 && voxel_chunk != 16)
-        return fail(XR_EINVAL, "voxel_chunk must be 1, 4, 8 or 16");
+    if (!valid_rows(tile_rows)) return fail(XR_EINVAL, "tile_rows must be 1, 2, 4, 8 or 16");
+    if (!valid_chunk(voxel_chunk)) return fail(XR_EINVAL, "voxel_chunk must be 1, 4, 8 or 16");
 
     if (rt.cell) {
         float2 *WX = static_cast<float2 *>(ws);
