@@ -86,14 +86,12 @@ void run_voxel_wg_KC(const Plan3d &p, const float2 *Q, float *out, int nb, int n
             for (int it = t; it < items; it += nthreads) {
                 const int bx = it % gx, by = (it / gx) % gy, bz = it / (gx * gy);
                 const int k0 = (bz % a.nchunk) * KC, b = bz / a.nchunk;
-                const int i0 = bx * kVoxTile, j0 = by * kVoxTile;
-                const int i1 = imin(i0 + kVoxTile - 1, a.nx - 1), j1 = imin(j0 + kVoxTile - 1, a.ny - 1);
                 std::fill(acc.begin(), acc.end(), 0.0f);
                 for (int v = 0; v < a.nview; ++v) {
                     const ViewHdr &h = p.d_hdr[v];
                     const float4 *ct = p.d_col + (int64_t)v * (a.nu + 1);
                     const float2 *Qv = view_table(Q, a, b, v);
-                    const VoxBox box = voxel_box<GEO, KC>(h, p.d_lineV, ct, a, v, i0, i1, j0, j1, k0);
+                    const VoxBox box = voxel_box<KC>(p.d_vtile[((int64_t)v * gy + by) * gx + bx], a, k0);
                     for (int tid = 0; tid < kBlock; ++tid) voxel_stage(Qv, box, tile.data(), a.nu, tid, kBlock);
                     for (int tid = 0; tid < kBlock; ++tid) {
                         int i, j;
@@ -231,6 +229,7 @@ int emu_op3d(int op, const int *vn, const float *vf, const int *gn, const float 
     p.d_lineC = t.lineC.data();
     p.d_lineV = t.lineV.data();
     p.d_col = t.col.data();
+    p.d_vtile = t.vtile.data();
     const Route rt = route(p, op);
     if (rt.kop < 0) {
         g_err = "unknown op";

@@ -26,6 +26,7 @@ namespace {
 struct BatchDesc {
     const ViewHdr *hdr;
     const float4 *lineC, *lineV, *col;
+    const VoxTile *vtile;
     CellArgs ca;
     VoxArgs va;  // nchunk is filled per launch from the launch's voxel chunk
     SinoArgs sa;
@@ -175,10 +176,8 @@ __global__ __launch_bounds__(kBlock) void k_voxel_batch(const BatchDesc *__restr
     const int bx = bid % gx, by = (bid / gx) % gy, bz = bid / (gx * gy);
     int ii, j;
     voxel_coords(bx, by, threadIdx.x, ii, j);
-    const int i0 = bx * kVoxTile, j0 = by * kVoxTile;
-    const int i1 = imin(i0 + kVoxTile - 1, a.nx - 1), j1 = imin(j0 + kVoxTile - 1, a.ny - 1);
-    voxel_wg<OPK, GEO, KC>(d.hdr, d.lineV, d.col, reinterpret_cast<const float2 *>(ws + d.ws_sino), io.out[i], a, ii,
-                           j, ii < a.nx && j < a.ny, i0, i1, j0, j1, (bz % a.nchunk) * KC, bz / a.nchunk, tile);
+    voxel_wg<OPK, GEO, KC>(d.hdr, d.lineV, d.col, d.vtile, reinterpret_cast<const float2 *>(ws + d.ws_sino), io.out[i],
+                           a, ii, j, ii < a.nx && j < a.ny, by * gx + bx, (bz % a.nchunk) * KC, bz / a.nchunk, tile);
 }
 
 template <int OPK, int GEO>
@@ -271,6 +270,7 @@ int xr_batch3d_create(xr_plan3d *const *plans, int n, const int *num_batch, xr_b
         d.lineC = p.d_lineC;
         d.lineV = p.d_lineV;
         d.col = p.d_col;
+        d.vtile = p.d_vtile;
         d.ca = xr::make_cell_args(p, num_batch[i]);
         d.va = xr::make_vox_args(p, num_batch[i], 1);
         d.sa = xr::make_sino_args(p, num_batch[i]);

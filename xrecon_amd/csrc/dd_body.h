@@ -497,55 +497,54 @@ struct VoxBox {
 // isocentre) detector stay below 2400 entries.
 constexpr int kVoxLds = 2560;
 
-// The block that voxel columns [i0, i1] x [j0, j1], slices [k0, k0 + KC) read from view v,
-// estimated from the tile's corner pixels (the axis map of a pixel edge is monotone in
-// the pixel and in the line, the depth magnification monotone in the line) with a
-// margin of one column and one row.  An estimate only: every lookup checks the block and
-// reads the table in HBM when it falls outside, so the block decides speed, not results.
-template <int GEO, int KC>
-__host__ __device__ inline VoxBox voxel_box(const ViewHdr &h, const float4 *__restrict__ lineV,
-                                            const float4 *__restrict__ ct, const VoxArgs &a, int v, int i0, int i1,
-                                            int j0, int j1, int k0)
+// The tile estimate of VoxTile for voxel columns [i0, i1] x [j0, j1] on view v, from the
+// tile's corner pixels: the axis map of a pixel edge is monotone in the pixel and in the
+// line, and the depth magnification zeta monotone in the line.  Computed once per plan.
+template <int GEO>
+inline VoxTile voxel_tile(const ViewHdr &h, const float4 *lineV, const float4 *ct, int ny, int nu, int maxline, int v,
+                          int i0, int i1, int j0, int j1)
 {
-    VoxBox bx{0, 0, 0, 0, 0};
+    VoxTile t{0, 0, 0.0f, 0.0f};
     int La, Lb, mlo, mhi;
     if (h.xdrv) { La = j0; Lb = j1; mlo = i0; mhi = i1; }
-    else { La = i0; Lb = i1; mlo = a.ny - 1 - j1; mhi = a.ny - 1 - j0; }
-    const float4 la = lineV[(int64_t)v * a.maxline + La], lb = lineV[(int64_t)v * a.maxline + Lb];
+    else { La = i0; Lb = i1; mlo = ny - 1 - j1; mhi = ny - 1 - j0; }
+    const float4 la = lineV[(int64_t)v * maxline + La], lb = lineV[(int64_t)v * maxline + Lb];
     const float alo = fminf(fmaf((float)mlo, la.y, la.x), fmaf((float)mlo, lb.y, lb.x));
     const float ahi = fmaxf(fmaf((float)(mhi + 1), la.y, la.x), fmaf((float)(mhi + 1), lb.y, lb.x));
-    if (!(ahi > h.dlo && alo < h.dhi)) return bx;
-    const int nu = a.nu;
+    if (!(ahi > h.dlo && alo < h.dhi)) return t;
     const int c0 = imax((int)clampf(column_seed<GEO>(h, alo), 0.0f, (float)(nu - 1)) - 1, 0);
     const int c1 = imin((int)clampf(column_seed<GEO>(h, ahi), 0.0f, (float)(nu - 1)) + 1, nu - 1);
-    if (c1 < c0) return bx;
-    // rho = zeta * z * kappa - v0/dvv over the corner lines, the end columns and the chunk's
-    // bottom and top slice boundaries.
+    if (c1 < c0) return t;
+    const float zs[4] = {la.z * ct[c0].z, la.z * ct[c1].z, lb.z * ct[c0].z, lb.z * ct[c1].z};
+    t.c0 = c0;
+    t.nc = c1 - c0 + 1;
+    t.zk_lo = fminf(fminf(zs[0], zs[1]), fminf(zs[2], zs[3]));
+    t.zk_hi = fmaxf(fmaxf(zs[0], zs[1]), fmaxf(zs[2], zs[3]));
+    return t;
+}
+
+// The block a voxel-gather workgroup stages for slices [k0, k0 + KC): the tile's columns
+// and the rows rho = zeta kappa z - v0/dvv of its chunk's bottom and top boundaries over
+// the tile's depth-scale range, with a margin of one row, clipped to the LDS capacity.
+template <int KC>
+__host__ __device__ inline VoxBox voxel_box(const VoxTile &t, const VoxArgs &a, int k0)
+{
+    VoxBox bx{0, 0, 0, 0, 0};
+    if (t.nc <= 0) return bx;
     const float zlo = fmaf((float)k0, a.dz, a.zb), zhi = fmaf((float)(k0 + KC), a.dz, a.zb);
-    const float ka = ct[c0].z, kb = ct[c1].z;
-    float lo = 3.0e38f, hi = -3.0e38f;
-    const float zs[2] = {zlo, zhi}, zes[2] = {la.z, lb.z}, ks[2] = {ka, kb};
-    for (int p = 0; p < 2; ++p)
-        for (int q = 0; q < 2; ++q)
-            for (int r = 0; r < 2; ++r) {
-                const float t = zes[p] * zs[q] * ks[r];
-                lo = fminf(lo, t);
-                hi = fmaxf(hi, t);
-            }
-    lo -= a.rv0;
-    hi -= a.rv0;
+    float lo = fminf(zlo * t.zk_lo, zlo * t.zk_hi) - a.rv0;
+    float hi = fmaxf(zhi * t.zk_lo, zhi * t.zk_hi) - a.rv0;
     if (!(hi > -1.0f && lo < (float)a.nv + 1.0f)) return bx;
     lo = fmaxf(lo, -1.0f);
     hi = fminf(hi, (float)a.nv + 1.0f);
     const int r0 = imax((int)floorf(lo) - 1, 0);
     const int r1 = imin((int)floorf(hi) + 1, a.nv);
-    const int nc = c1 - c0 + 1;
-    const int pitch = (nc + 1) & ~1;
+    const int pitch = (t.nc + 1) & ~1;
     int nr = r1 - r0 + 1;
     if (nr <= 0 || pitch > kVoxLds) return bx;
     if (pitch * nr > kVoxLds) nr = kVoxLds / pitch;
-    bx.c0 = c0;
-    bx.nc = nc;
+    bx.c0 = t.c0;
+    bx.nc = t.nc;
     bx.r0 = r0;
     bx.nr = nr;
     bx.pitch = pitch;
@@ -567,22 +566,31 @@ __host__ __device__ inline void voxel_stage(const float2 *__restrict__ Qv, const
 {
     const int half = bx.pitch >> 1;
     const int E2 = bx.nr * half;
-    const float inv = 1.0f / (float)(half > 0 ? half : 1);
+    if (E2 <= 0) return;
     const char *Qb = reinterpret_cast<const char *>(Qv);
     const unsigned nu8 = (unsigned)nu * 8u;
+    // (row, pair) of p = tid, advanced by nthreads = dq rows + dc pairs per step.  The
+    // first division: (tid + 1/2) / half stays >= 1/(2 half) from an integer, far above
+    // the rounding of the product.
+    int r = (int)(((float)tid + 0.5f) * (1.0f / (float)half));
+    int c = tid - r * half;
+    const int dq = nthreads / half, dc = nthreads - dq * half;
+    const unsigned cb = (unsigned)bx.c0 * 8u;
     for (int s = 0; s * nthreads < E2; ++s) {
         const int p = tid + s * nthreads;
-        // p / half exactly: (p + 1/2) / half stays >= 1/(2 half) from an integer, far
-        // above the rounding of the product for p < kVoxLds.
-        const int r = (int)(((float)p + 0.5f) * inv);
-        const int c = 2 * (p - r * half);
-        const unsigned off = p < E2 ? mul24((unsigned)(bx.r0 + r), nu8) + (unsigned)(bx.c0 + c) * 8u : 0u;
+        const unsigned off = p < E2 ? mul24((unsigned)(bx.r0 + r), nu8) + cb + (unsigned)c * 16u : 0u;
 #if defined(__HIP_DEVICE_COMPILE__)
         float2 *wave_base = lds + 2 * (s * nthreads + (tid & ~(kWave - 1)));
         __builtin_amdgcn_global_load_lds(Qb + off, (__attribute__((address_space(3))) void *)wave_base, 16, 0, 0);
 #else
         if (p < E2) *reinterpret_cast<Pair2 *>(lds + 2 * p) = *reinterpret_cast<const Pair2 *>(Qb + off);
 #endif
+        r += dq;
+        c += dc;
+        if (c >= half) {
+            c -= half;
+            ++r;
+        }
     }
 }
 
@@ -655,11 +663,13 @@ __host__ __device__ inline void voxel_view(const ViewHdr &h, const float4 *__res
                 const bool staged = lds != nullptr && (unsigned)(c - bx.c0) < (unsigned)bx.nc && ri[0] >= bx.r0 &&
                                     ri[KC] < bx.r0 + bx.nr;
                 if (staged) {
+                    // byte offset (ri - r0) * pitch * 8 + (c - c0) * 8 as one 24-bit
+                    // multiply-add (unsigned wrap-around of the constant part is exact)
                     const char *Lb = reinterpret_cast<const char *>(lds);
-                    const unsigned cb = (unsigned)(c - bx.c0) * 8u;
+                    const unsigned cb = (unsigned)(c - bx.c0) * 8u - (unsigned)bx.r0 * nc8;
 #pragma unroll
                     for (int q = 0; q <= KC; ++q)
-                        e[q] = *reinterpret_cast<const float2 *>(Lb + (mul24((unsigned)(ri[q] - bx.r0), nc8) + cb));
+                        e[q] = *reinterpret_cast<const float2 *>(Lb + (mul24((unsigned)ri[q], nc8) + cb));
                 } else {
                     const unsigned c8 = (unsigned)c * 8u;
 #pragma unroll
@@ -701,8 +711,8 @@ __host__ __device__ inline void voxel_store(float *__restrict__ out, const VoxAr
     }
 }
 
-// A workgroup of the voxel gather (device only): voxel columns [i0, i1] x [j0, j1] of the
-// tile, slices k0..k0+KC-1, batch b; this thread owns (i, j) when `active`.  Per view the
+// A workgroup of the voxel gather (device only): the 16 x 16 tile tile_id of voxel
+// columns, slices k0..k0+KC-1, batch b; this thread owns (i, j) when `active`.  Per view the
 // workgroup stages the block of the row-prefix table its footprint reads into LDS, then
 // every thread gathers from it.  Inactive threads take part in staging and barriers.
 // (A second, alternating buffer that overlaps the next view's staging with this view's
@@ -710,18 +720,19 @@ __host__ __device__ inline void voxel_store(float *__restrict__ out, const VoxAr
 // staging loads anyway, and the doubled code and registers cost more than they save.)
 template <int OPK, int GEO, int KC>
 __device__ inline void voxel_wg(const ViewHdr *__restrict__ hdr, const float4 *__restrict__ lineV,
-                                const float4 *__restrict__ colt, const float2 *__restrict__ Q,
-                                float *__restrict__ out, const VoxArgs &a, int i, int j, bool active, int i0, int i1,
-                                int j0, int j1, int k0, int b, float2 *__restrict__ tile)
+                                const float4 *__restrict__ colt, const VoxTile *__restrict__ vtile,
+                                const float2 *__restrict__ Q, float *__restrict__ out, const VoxArgs &a, int i, int j,
+                                bool active, int tile_id, int k0, int b, float2 *__restrict__ tile)
 {
     const float px = voxel_px(a, i), py = voxel_py(a, j);
     float acc[KC];
     for (int q = 0; q < KC; ++q) acc[q] = 0.0f;
+    const int ntile = ceil_div(a.nx, kVoxTile) * ceil_div(a.ny, kVoxTile);
     for (int v = 0; v < a.nview; ++v) {
         const ViewHdr h = hdr[v];
         const float4 *ct = colt + (int64_t)v * (a.nu + 1);
         const float2 *Qv = view_table(Q, a, b, v);
-        const VoxBox bx = voxel_box<GEO, KC>(h, lineV, ct, a, v, i0, i1, j0, j1, k0);
+        const VoxBox bx = voxel_box<KC>(vtile[(int64_t)v * ntile + tile_id], a, k0);
         voxel_stage(Qv, bx, tile, a.nu, threadIdx.x, kBlock);
         __builtin_amdgcn_s_waitcnt(0);  // this wave's direct-to-LDS loads have landed
         __syncthreads();

@@ -109,18 +109,16 @@ __global__ __launch_bounds__(kBlock) void k_cell(const ViewHdr *__restrict__ hdr
 
 template <int OPK, int GEO, int KC>
 __global__ __launch_bounds__(kBlock) void k_voxel(const ViewHdr *__restrict__ hdr, const float4 *__restrict__ lineV,
-                                                  const float4 *__restrict__ colt, const float2 *__restrict__ Q,
-                                                  float *__restrict__ out, VoxArgs a)
+                                                  const float4 *__restrict__ colt, const VoxTile *__restrict__ vtile,
+                                                  const float2 *__restrict__ Q, float *__restrict__ out, VoxArgs a)
 {
     __shared__ __attribute__((aligned(16))) float2 tile[kVoxLds];
     int i, j;
     voxel_coords(blockIdx.x, blockIdx.y, threadIdx.x, i, j);
     const int chunk = blockIdx.z % a.nchunk;
     const int b = blockIdx.z / a.nchunk;
-    const int i0 = blockIdx.x * kVoxTile, j0 = blockIdx.y * kVoxTile;
-    const int i1 = imin(i0 + kVoxTile - 1, a.nx - 1), j1 = imin(j0 + kVoxTile - 1, a.ny - 1);
-    voxel_wg<OPK, GEO, KC>(hdr, lineV, colt, Q, out, a, i, j, i < a.nx && j < a.ny, i0, i1, j0, j1, chunk * KC, b,
-                           tile);
+    voxel_wg<OPK, GEO, KC>(hdr, lineV, colt, vtile, Q, out, a, i, j, i < a.nx && j < a.ny,
+                           blockIdx.y * gridDim.x + blockIdx.x, chunk * KC, b, tile);
 }
 
 template <int OPK, int GEO>
@@ -150,10 +148,10 @@ void launch_voxel(const Plan3d &p, int KC, const float2 *Q, float *out, int nbat
     const VoxArgs a = make_vox_args(p, nbatch, KC);
     dim3 grid(ceil_div(a.nx, kVoxTile), ceil_div(a.ny, kVoxTile), nbatch * a.nchunk);
     switch (KC) {
-    case 1: hipLaunchKernelGGL((k_voxel<OPK, GEO, 1>), grid, dim3(kBlock), 0, s, p.d_hdr, p.d_lineV, p.d_col, Q, out, a); break;
-    case 4: hipLaunchKernelGGL((k_voxel<OPK, GEO, 4>), grid, dim3(kBlock), 0, s, p.d_hdr, p.d_lineV, p.d_col, Q, out, a); break;
-    case 8: hipLaunchKernelGGL((k_voxel<OPK, GEO, 8>), grid, dim3(kBlock), 0, s, p.d_hdr, p.d_lineV, p.d_col, Q, out, a); break;
-    default: hipLaunchKernelGGL((k_voxel<OPK, GEO, 16>), grid, dim3(kBlock), 0, s, p.d_hdr, p.d_lineV, p.d_col, Q, out, a); break;
+    case 1: hipLaunchKernelGGL((k_voxel<OPK, GEO, 1>), grid, dim3(kBlock), 0, s, p.d_hdr, p.d_lineV, p.d_col, p.d_vtile, Q, out, a); break;
+    case 4: hipLaunchKernelGGL((k_voxel<OPK, GEO, 4>), grid, dim3(kBlock), 0, s, p.d_hdr, p.d_lineV, p.d_col, p.d_vtile, Q, out, a); break;
+    case 8: hipLaunchKernelGGL((k_voxel<OPK, GEO, 8>), grid, dim3(kBlock), 0, s, p.d_hdr, p.d_lineV, p.d_col, p.d_vtile, Q, out, a); break;
+    default: hipLaunchKernelGGL((k_voxel<OPK, GEO, 16>), grid, dim3(kBlock), 0, s, p.d_hdr, p.d_lineV, p.d_col, p.d_vtile, Q, out, a); break;
     }
 }
 

@@ -11,7 +11,7 @@
 #include <cmath>
 #include <cstring>
 
-#include "xr_internal.h"
+#include "dd_body.h"
 
 namespace xr {
 
@@ -252,6 +252,22 @@ int build_tables(Plan3d &p, const float *angles, PlanTables &t, std::string &err
             h.tk0 = (float)(nu * 0.5 - sk * shu / du);
             h.tk1 = (float)(sk * (xdrv ? cb : sb) / du);
         }
+    }
+
+    // Voxel-gather staging estimates per (view, 16 x 16 tile of voxel columns).
+    const int gx = ceil_div(nx, kVoxTile), gy = ceil_div(ny, kVoxTile);
+    t.vtile.assign((size_t)V * gx * gy, VoxTile{0, 0, 0.0f, 0.0f});
+    for (int v = 0; v < V; ++v) {
+        const float4 *ct = t.col.data() + (size_t)v * (nu + 1);
+        for (int by = 0; by < gy; ++by)
+            for (int bx = 0; bx < gx; ++bx) {
+                const int i0 = bx * kVoxTile, j0 = by * kVoxTile;
+                const int i1 = std::min(i0 + kVoxTile - 1, nx - 1), j1 = std::min(j0 + kVoxTile - 1, ny - 1);
+                VoxTile &vt = t.vtile[((size_t)v * gy + by) * gx + bx];
+                if (p.kind == GEO_FLAT) vt = voxel_tile<GEO_FLAT>(t.hdr[v], t.lineV.data(), ct, ny, nu, ML, v, i0, i1, j0, j1);
+                else if (p.kind == GEO_ARC) vt = voxel_tile<GEO_ARC>(t.hdr[v], t.lineV.data(), ct, ny, nu, ML, v, i0, i1, j0, j1);
+                else vt = voxel_tile<GEO_PAR>(t.hdr[v], t.lineV.data(), ct, ny, nu, ML, v, i0, i1, j0, j1);
+            }
     }
     return XR_OK;
 }

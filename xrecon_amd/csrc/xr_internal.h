@@ -39,6 +39,17 @@ struct alignas(16) ViewHdr {
 };
 static_assert(sizeof(ViewHdr) == 80, "ViewHdr layout");
 
+// Per (view, 16 x 16 tile of voxel columns): the detector columns c0 .. c0+nc-1 the tile's
+// footprint reaches on that view (with a one-column margin), and the range of the depth
+// scale zeta * kappa over the tile's lines and those columns, from which a voxel-gather
+// workgroup derives the rows its slice chunk reads.  nc = 0: the tile misses the
+// detector.  An estimate that decides what is staged in LDS, never a result.
+struct VoxTile {
+    int c0, nc;
+    float zk_lo, zk_hi;
+};
+static_assert(sizeof(VoxTile) == 16, "VoxTile layout");
+
 // Geometry family of a plan; selects the column inverse g() and the weights.
 enum GeoKind : int { GEO_PAR = 0, GEO_FLAT = 1, GEO_ARC = 2 };
 
@@ -65,12 +76,14 @@ struct Plan3d {
     float4 *d_lineC = nullptr;  // [nview][maxline] {toff, invd, iz, lam}: cell kernels
     float4 *d_lineV = nullptr;  // [nview][maxline] {P0, Delta, zeta, 1/(Delta zeta)}: voxel kernels
     float4 *d_col = nullptr;    // [nview][Nu+1]    {d, sigma, kappa, W}
+    VoxTile *d_vtile = nullptr; // [nview][ceil(Ny/16)][ceil(Nx/16)]: voxel-gather staging estimates
 };
 
 // Host-side tables before upload (kept separate so tests can inspect the builder).
 struct PlanTables {
     std::vector<ViewHdr> hdr;
     std::vector<float4> lineC, lineV, col;
+    std::vector<VoxTile> vtile;
 };
 
 // plan.cpp
