@@ -50,6 +50,7 @@ def parse():
                     help="only print a per-kernel summary of rocprofv3 SQLite result files")
     ap.add_argument("--verbose", action="store_true", help="print autotuner timings")
     ap.add_argument("--no-batched", action="store_true", help="skip the 16-instance batched section")
+    ap.add_argument("--no-configs", action="store_true", help="skip the 2D fan-flat and cone-arc sections")
     return ap.parse_args()
 
 
@@ -178,6 +179,99 @@ def summarize_rocpd(paths):
             ctr = " ".join(f"{c}={sum(v) / len(v):.1f}" for c, v in sorted(pmc[name].items()))
             short = name.replace("(anonymous namespace)::", "").split("(")[0][-70:]
             print(f"{short:70s} {len(d):5d} {sum(d) / len(d) / 1e3:10.2f} {min(d) / 1e3:10.2f} {max(d) / 1e3:10.2f}  {ctr}")
+
+
+def _time_gpu(fn, reps, stream):
+    """Average ms of fn() over `reps` calls, HIP events on `stream` (after one warm-up)."""
+    import torch
+
+    fn()
+    torch.cuda.synchronize()
+    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    s.record(stream)
+    for _ in range(reps):
+        fn()
+    e.record(stream)
+    torch.cuda.synchronize()
+    return s.elapsed_time(e) / reps
+
+
+def bench_cfg1(dev, reps, with_cpu):
+    """2D fan-flat: 256^2 image at 1 mm, 256 detectors at 2 mm (1 mm virtual), SOD 500 mm,
+    SDD 1000 mm, 720 views, 10 seeded random ellipses; projection A then FBP (cosine
+    weight, ramp filter by rFFT, weighted backprojection).  Unit: pixel-views/s."""
+    import torch
+
+    import xrecon_amd
+    from xrecon_amd.phantom import random_ellipses
+
+    W = H = 256
+    Nu, V = 256, 720
+    proj = xrecon_amd.Projector2d(W, Nu, 1.0, 2.0, 500.0, 1000.0, scan_type="flat").to(dev)
+    angles = (torch.arange(V, dtype=torch.float32) * np.float32(2 * np.pi / V)).to(dev)
+    img = random_ellipses((W, H), (1.0, 1.0), count=10, semi_axes=(5.0, 60.0), seed=1001)
+    x = torch.from_numpy(img).to(dev).view(1, 1, H, W)
+    stream = torch.cuda.current_stream(dev)
+    sino = proj.projection(x, angles)
+    t_a = _time_gpu(lambda: proj.projection(x, angles), reps, stream)
+    t_f = _time_gpu(lambda: proj.filtered_backprojection(sino, angles), reps, stream)
+    pv = W * H * V
+    out = {"workload": "2D fan-flat A + FBP: 256^2 @1mm, 256 detectors @2mm (1mm virtual), SOD 500, SDD 1000, "
+                       "720 views, 10 random ellipses (seed 1001)",
+           "value": pv / ((t_a + t_f) * 1e-3), "unit": "pixel-views/s", "ms_A": t_a, "ms_FBP": t_f,
+           "path": "Projector2d API (plan cache, per-call allocation)"}
+    if with_cpu:
+        import oracle
+
+        if oracle.ref_available():
+            o = [float(v) for v in proj._host_options]
+            a = angles.cpu().numpy()
+            oracle.ref_op2d("A", img[None], a[:4], o)
+            t0 = time.perf_counter()
+            y = oracle.ref_op2d("A", img[None], a, o)
+            t1 = time.perf_counter()
+            oracle.ref_op2d("BW", y, a, o)
+            t2 = time.perf_counter()
+            out["cpu_baseline"] = {"value": pv / (t2 - t0), "unit": "pixel-views/s", "cores": os.cpu_count(),
+                                   "kind": "reference", "sample": f"all 720 views: A {t1 - t0:.3f}s + weighted B "
+                                   f"{t2 - t1:.3f}s (filtering not included)"}
+    return out
+
+
+def bench_cfg5(dev, reps):
+    """Cone-arc: 512^2 x 256 at 0.7 mm, 736 x 128 detector (1.2 mm / 1.0 mm), SOD 595 mm,
+    SDD 1085.6 mm (du = 1.105e-3 rad), 1152 views, batch 4, seeded random ellipsoids.
+    A + B and the adjoint pair A + A^T with the default tiles (no autotuning at this size)."""
+    import torch
+
+    import xrecon_amd
+    from xrecon_amd import _backend as be
+    from xrecon_amd.phantom import random_ellipsoids
+
+    W = H = 512
+    D, Nu, Nv, V, NB = 256, 736, 128, 1152, 4
+    proj = xrecon_amd.Projector3d((W, H, D), (Nu, Nv), 0.7, (1.2, 1.0), 595.0, 1085.6, scan_type="arc")
+    angles = torch.arange(V, dtype=torch.float32) * np.float32(2 * np.pi / V)
+    ph = random_ellipsoids((W, H, D), (0.7, 0.7, 0.7), count=16, semi_axes=(5.0, 120.0), seed=1005)
+    x = torch.from_numpy(ph).to(dev).view(1, 1, D, H, W).expand(NB, 1, D, H, W).contiguous()
+    del ph
+    plan = proj.plan(angles, dev)
+    sid = torch.cuda.current_stream(dev).cuda_stream
+    stream = torch.cuda.current_stream(dev)
+    A, B, AT = be.OP_PROJECTION, be.OP_BACKPROJECTION, be.OP_PROJECTION_T
+    sino = torch.empty((NB, 1, V, Nv, Nu), device=dev)
+    rec = torch.empty_like(x)
+    ws = torch.empty(max(plan.workspace_bytes(A, NB), plan.workspace_bytes(B, NB), plan.workspace_bytes(AT, NB)),
+                     dtype=torch.uint8, device=dev)
+    t_a = _time_gpu(lambda: plan.execute_into(A, x, sino, ws, sid, 0, 0, 0), reps, stream)
+    t_b = _time_gpu(lambda: plan.execute_into(B, sino, rec, ws, sid, 0, 0, 0), reps, stream)
+    t_at = _time_gpu(lambda: plan.execute_into(AT, sino, rec, ws, sid, 0, 0, 0), reps, stream)
+    vv = W * H * D * V * NB
+    return {"workload": "cone-arc 512^2x256 @0.7mm, 736x128 detector (1.2/1.0 mm), SOD 595, SDD 1085.6, "
+                        "1152 views, B=4, 16 random ellipsoids (seed 1005)",
+            "voxel_views_per_operator": vv, "value": vv / ((t_a + t_b) * 1e-3), "unit": "voxel-views/s",
+            "ms_A": t_a, "ms_B": t_b, "ms_AT": t_at, "adjoint_value": vv / ((t_a + t_at) * 1e-3),
+            "tiles": "defaults (8 rows, 16 slices)"}
 
 
 def traffic_from_profiles(kernel: str):
@@ -355,6 +449,12 @@ def main():
         cpu = cpu_baseline(list(proj._host_options), phantom, angles.numpy(), args.cpu_views)
 
     batched = None if args.no_batched else bench_batched(dev, max(2, K // 2), args.verbose)
+    configs = None
+    if not args.no_configs:
+        configs = {"cfg1_fan_flat_2d": bench_cfg1(dev, max(3, K), not args.no_cpu),
+                   "cfg5_cone_arc": bench_cfg5(dev, 2)}
+        torch.cuda.empty_cache()
+    compulsory = 4.0 * (vox_views / wl["V"] + wl["V"] * wl["Nv"] * wl["Nu"])
 
     out = {
         "metric": METRIC,
@@ -374,7 +474,11 @@ def main():
                    "voxel_views_per_operator": vox_views, "nnz_per_operator": nnz},
         "roofline": {"bound": "valu", "kernel": dominant, "symbol": symbol, "achieved": achieved,
                      "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS,
+                     "peak_nonpacked": FP32_PEAK_TFLOPS / 2, "frac_nonpacked": achieved / (FP32_PEAK_TFLOPS / 2),
                      "traffic": traffic, "traffic_source": traffic_src,
+                     "hbm_frac": None if traffic is None else traffic / (t_dom * 1e-3) / 8e12,
+                     "compulsory_bytes": compulsory,
+                     "compulsory_hbm_frac": compulsory / (t_dom * 1e-3) / 8e12,
                      "note": "FP32 vector-ALU roof (packed FP32 peak); no dense contraction, so no MFMA; "
                              "F = 4*nnz per launch; traffic = HBM bytes per launch (rocprofv3 FETCH_SIZE x2.0 "
                              "+ WRITE_SIZE x1.0, copy-kernel calibrated)"},
@@ -385,6 +489,7 @@ def main():
                     "pair": "A + A^T"},
         "cpu_baseline": cpu,
         "batched": batched,
+        "configs": configs,
     }
     print(json.dumps(out), flush=True)
 
