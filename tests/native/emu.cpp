@@ -115,7 +115,7 @@ void run_voxel_wg_KC(const Plan3d &p, const float2 *Q, float *out, int nb, int n
 template <int OPK, int GEO, int KC>
 void run_voxel_KC(const Plan3d &p, const float2 *Q, float *out, int nb, int nthreads)
 {
-    if (g_voxel_variant == 0) {
+    if (g_voxel_variant == 0 && KC >= kVoxStageMin) {
         run_voxel_wg_KC<OPK, GEO, KC>(p, Q, out, nb, nthreads);
         return;
     }

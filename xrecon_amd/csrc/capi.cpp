@@ -50,6 +50,9 @@ int validate(const xr_volume3d *vol, const xr_geometry3d *geo, int num_view)
         return fail(XR_EINVAL, "detector too large for 32-bit row indexing");
     if ((int64_t)vol->nx * vol->ny >= ((int64_t)1 << 24))
         return fail(XR_EINVAL, "volume plane too large (Nx*Ny must stay below 2^24)");
+    // 24-bit index arithmetic in the gathers: rows, slices and byte strides of a row.
+    if (geo->nu >= (1 << 21) || geo->nv >= (1 << 24) || vol->nz >= (1 << 24))
+        return fail(XR_EINVAL, "detector or volume dimension too large for 24-bit indexing");
     return XR_OK;
 }
 

@@ -188,6 +188,20 @@ __host__ __device__ inline unsigned mul24(unsigned a, unsigned b)
 #endif
 }
 
+// a * b + c for a, b below 2^24: one full-rate v_mad_u32_u24 on the device.  Written as
+// an instruction because the compiler otherwise turns some of these (runtime operands it
+// cannot bound) into a mask and a quarter-rate v_mul_lo_u32.
+__host__ __device__ inline unsigned mad24(unsigned a, unsigned b, unsigned c)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    unsigned r;
+    asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    return r;
+#else
+    return a * b + c;
+#endif
+}
+
 // Two consecutive float2 table entries (pixels m and m+1 of a line) in one 16-byte load.
 // The entries are only 8-byte aligned, which gfx950 vector memory accesses allow.
 struct __attribute__((aligned(8))) Pair2 {
@@ -578,7 +592,7 @@ __host__ __device__ inline void voxel_stage(const float2 *__restrict__ Qv, const
     const unsigned cb = (unsigned)bx.c0 * 8u;
     for (int s = 0; s * nthreads < E2; ++s) {
         const int p = tid + s * nthreads;
-        const unsigned off = p < E2 ? mul24((unsigned)(bx.r0 + r), nu8) + cb + (unsigned)c * 16u : 0u;
+        const unsigned off = p < E2 ? mad24((unsigned)(bx.r0 + r), nu8, cb + (unsigned)c * 16u) : 0u;
 #if defined(__HIP_DEVICE_COMPILE__)
         float2 *wave_base = lds + 2 * (s * nthreads + (tid & ~(kWave - 1)));
         __builtin_amdgcn_global_load_lds(Qb + off, (__attribute__((address_space(3))) void *)wave_base, 16, 0, 0);
@@ -669,7 +683,7 @@ __host__ __device__ inline void voxel_view(const ViewHdr &h, const float4 *__res
                     const unsigned cb = (unsigned)(c - bx.c0) * 8u - (unsigned)bx.r0 * nc8;
 #pragma unroll
                     for (int q = 0; q <= KC; ++q)
-                        e[q] = *reinterpret_cast<const float2 *>(Lb + (mul24((unsigned)ri[q], nc8) + cb));
+                        e[q] = *reinterpret_cast<const float2 *>(Lb + mad24((unsigned)ri[q], nc8, cb));
                 } else {
                     const unsigned c8 = (unsigned)c * 8u;
 #pragma unroll
@@ -711,6 +725,23 @@ __host__ __device__ inline void voxel_store(float *__restrict__ out, const VoxAr
     }
 }
 
+// One thread, reading the tables in HBM only: voxel column (i, j), slices k0..k0+KC-1,
+// batch b.
+template <int OPK, int GEO, int KC>
+__host__ __device__ inline void voxel_body(const ViewHdr *__restrict__ hdr, const float4 *__restrict__ lineV,
+                                           const float4 *__restrict__ colt, const float2 *__restrict__ Q,
+                                           float *__restrict__ out, const VoxArgs &a, int i, int j, int k0, int b)
+{
+    const float px = voxel_px(a, i), py = voxel_py(a, j);
+    float acc[KC];
+    for (int q = 0; q < KC; ++q) acc[q] = 0.0f;
+    const VoxBox none{0, 0, 0, 0, 0};
+    for (int v = 0; v < a.nview; ++v)
+        voxel_view<OPK, GEO, KC>(hdr[v], lineV, colt + (int64_t)v * (a.nu + 1), view_table(Q, a, b, v), a, v, i, j,
+                                 k0, px, py, acc, none, nullptr);
+    voxel_store<KC>(out, a, i, j, k0, b, acc);
+}
+
 // A workgroup of the voxel gather (device only): the 16 x 16 tile tile_id of voxel
 // columns, slices k0..k0+KC-1, batch b; this thread owns (i, j) when `active`.  Per view the
 // workgroup stages the block of the row-prefix table its footprint reads into LDS, then
@@ -718,12 +749,20 @@ __host__ __device__ inline void voxel_store(float *__restrict__ out, const VoxAr
 // (A second, alternating buffer that overlaps the next view's staging with this view's
 // gather measured slower: the gather's own table loads must wait for the earlier-issued
 // staging loads anyway, and the doubled code and registers cost more than they save.)
+// Chunks of fewer than kVoxStageMin slices read the table directly: a staged block then
+// serves too few lookups to repay its copy and the two barriers per view.
+constexpr int kVoxStageMin = 8;
+
 template <int OPK, int GEO, int KC>
 __device__ inline void voxel_wg(const ViewHdr *__restrict__ hdr, const float4 *__restrict__ lineV,
                                 const float4 *__restrict__ colt, const VoxTile *__restrict__ vtile,
                                 const float2 *__restrict__ Q, float *__restrict__ out, const VoxArgs &a, int i, int j,
                                 bool active, int tile_id, int k0, int b, float2 *__restrict__ tile)
 {
+    if (KC < kVoxStageMin) {
+        if (active) voxel_body<OPK, GEO, KC>(hdr, lineV, colt, Q, out, a, i, j, k0, b);
+        return;
+    }
     const float px = voxel_px(a, i), py = voxel_py(a, j);
     float acc[KC];
     for (int q = 0; q < KC; ++q) acc[q] = 0.0f;
@@ -740,23 +779,6 @@ __device__ inline void voxel_wg(const ViewHdr *__restrict__ hdr, const float4 *_
         __syncthreads();
     }
     if (active) voxel_store<KC>(out, a, i, j, k0, b, acc);
-}
-
-// One thread, reading the tables in HBM only: voxel column (i, j), slices k0..k0+KC-1,
-// batch b.
-template <int OPK, int GEO, int KC>
-__host__ __device__ inline void voxel_body(const ViewHdr *__restrict__ hdr, const float4 *__restrict__ lineV,
-                                           const float4 *__restrict__ colt, const float2 *__restrict__ Q,
-                                           float *__restrict__ out, const VoxArgs &a, int i, int j, int k0, int b)
-{
-    const float px = voxel_px(a, i), py = voxel_py(a, j);
-    float acc[KC];
-    for (int q = 0; q < KC; ++q) acc[q] = 0.0f;
-    const VoxBox none{0, 0, 0, 0, 0};
-    for (int v = 0; v < a.nview; ++v)
-        voxel_view<OPK, GEO, KC>(hdr[v], lineV, colt + (int64_t)v * (a.nu + 1), view_table(Q, a, b, v), a, v, i, j,
-                                 k0, px, py, acc, none, nullptr);
-    voxel_store<KC>(out, a, i, j, k0, b, acc);
 }
 
 }  // namespace xr
