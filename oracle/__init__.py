@@ -122,7 +122,7 @@ def op3d(op: str, x: np.ndarray, angles: np.ndarray, options: Sequence[float], n
     out = np.zeros(out_shape, dtype=np.float64)
     vn, vf, gn, gf = split3d(options)
     lib.oracle_op3d(OPS[op], _ptr(x), _ptr(out), _ptr(angles), _ptr(vn), _ptr(vf), _ptr(gn), _ptr(gf),
-                    nbatch, nview, nthreads or os.cpu_count() or 1)
+                    nbatch, nview, nthreads or min(16, os.cpu_count() or 1))
     return out
 
 
@@ -143,7 +143,7 @@ def op2d(op: str, x: np.ndarray, angles: np.ndarray, options: Sequence[float], n
     out = np.zeros(out_shape, dtype=np.float64)
     vn, vf, gn, gf = split2d(options)
     lib.oracle_op2d(OPS[op], _ptr(x), _ptr(out), _ptr(angles), _ptr(vn), _ptr(vf), _ptr(gn), _ptr(gf),
-                    nbatch, nview, nthreads or os.cpu_count() or 1)
+                    nbatch, nview, nthreads or min(16, os.cpu_count() or 1))
     return out
 
 
@@ -154,7 +154,7 @@ def count3d(angles: np.ndarray, options: Sequence[float], nthreads: int | None =
     vn, vf, gn, gf = split3d(options)
     dummy = np.zeros(1, dtype=np.float64)
     return int(lib.oracle_op3d(6, _ptr(dummy), _ptr(dummy), _ptr(angles), _ptr(vn), _ptr(vf), _ptr(gn),
-                               _ptr(gf), 1, angles.shape[0], nthreads or os.cpu_count() or 1))
+                               _ptr(gf), 1, angles.shape[0], nthreads or min(16, os.cpu_count() or 1)))
 
 
 def view3d(angle: float, options: Sequence[float]):
