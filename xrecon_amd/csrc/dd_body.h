@@ -566,37 +566,44 @@ __host__ __device__ inline VoxBox voxel_box(const VoxTile &t, const VoxArgs &a, 
 }
 
 // Thread tid's share of copying the block from view v's row-prefix table Qv into LDS, in
-// 16-byte pairs of adjacent columns: wave w copies rows w, w + nwaves, ..., lane l the
-// pairs l, l + 64, ... of a row (pair p of row r goes to lds[r * pitch + 2p, +1]).  A
-// pair may end one entry past the view's last column (the next row's first entry, or the
-// pad after the table); that entry is never looked up.
+// 16-byte pairs of adjacent columns: pair p = tid + s * nthreads goes to lds[2p, 2p+1]
+// (at most kVoxLds / 2 pairs, so s < kVoxLds / 2 / nthreads).  A pair may end one entry
+// past the view's last column (the next row's first entry, or the pad after the table);
+// that entry is never looked up.
 //
 // On the device the copies are direct global-to-LDS loads (no registers held): lane l of
-// the wave writes 16 bytes at the wave's LDS base + 16 l, which is where its pair lands;
-// lanes past the row's pairs are masked off.  The caller waits for the loads
-// (s_waitcnt) before its barrier.
+// the wave writes 16 bytes at the wave's LDS base + 16 l, which is where pair p lands
+// for p = s * nthreads + 64 * wave + l.  Lanes past the block load the table's first
+// pair into unused LDS.  The caller waits for the loads (s_waitcnt) before its barrier.
 __host__ __device__ inline void voxel_stage(const float2 *__restrict__ Qv, const VoxBox &bx, float2 *__restrict__ lds,
                                             int nu, int tid, int nthreads)
 {
     const int half = bx.pitch >> 1;
-    if (bx.nr <= 0 || half <= 0) return;
-    const int lane = tid & (kWave - 1), wave = tid / kWave, nwaves = nthreads / kWave;
+    const int E2 = bx.nr * half;
+    if (E2 <= 0) return;
     const char *Qb = reinterpret_cast<const char *>(Qv);
     const unsigned nu8 = (unsigned)nu * 8u;
-    for (int p0 = 0; p0 < half; p0 += kWave) {
-        const int p = p0 + lane;
-        const unsigned coff = (unsigned)(bx.c0 + 2 * p) * 8u;
-        for (int r = wave; r < bx.nr; r += nwaves) {
-            if (p < half) {
-                const unsigned off = mad24((unsigned)(bx.r0 + r), nu8, coff);
+    // (row, pair) of p = tid, advanced by nthreads = dq rows + dc pairs per step.  The
+    // first division: (tid + 1/2) / half stays >= 1/(2 half) from an integer, far above
+    // the rounding of the product.
+    int r = (int)(((float)tid + 0.5f) * (1.0f / (float)half));
+    int c = tid - r * half;
+    const int dq = nthreads / half, dc = nthreads - dq * half;
+    const unsigned cb = (unsigned)bx.c0 * 8u;
+    for (int s = 0; s * nthreads < E2; ++s) {
+        const int p = tid + s * nthreads;
+        const unsigned off = p < E2 ? mad24((unsigned)(bx.r0 + r), nu8, cb + (unsigned)c * 16u) : 0u;
 #if defined(__HIP_DEVICE_COMPILE__)
-                float2 *wave_base = lds + r * bx.pitch + 2 * p0;
-                __builtin_amdgcn_global_load_lds(Qb + off, (__attribute__((address_space(3))) void *)wave_base, 16,
-                                                 0, 0);
+        float2 *wave_base = lds + 2 * (s * nthreads + (tid & ~(kWave - 1)));
+        __builtin_amdgcn_global_load_lds(Qb + off, (__attribute__((address_space(3))) void *)wave_base, 16, 0, 0);
 #else
-                *reinterpret_cast<Pair2 *>(lds + r * bx.pitch + 2 * p) = *reinterpret_cast<const Pair2 *>(Qb + off);
+        if (p < E2) *reinterpret_cast<Pair2 *>(lds + 2 * p) = *reinterpret_cast<const Pair2 *>(Qb + off);
 #endif
-            }
+        r += dq;
+        c += dc;
+        if (c >= half) {
+            c -= half;
+            ++r;
         }
     }
 }
