@@ -127,7 +127,8 @@ void run_voxel_KC(const Plan3d &p, const float2 *Q, float *out, int nb, int nthr
             for (int it = t; it < items; it += nthreads) {
                 const int j = it % a.ny, rest = it / a.ny;
                 const int chunk = rest % a.nchunk, b = rest / a.nchunk;
-                for (int i = 0; i < a.nx; ++i) voxel_body<OPK, GEO, KC>(p.d_hdr, p.d_lineV, p.d_col, Q, out, a, i, j, chunk * KC, b);
+                for (int i = 0; i < a.nx; ++i)
+                    voxel_body<OPK, GEO, KC>(p.d_hdr, p.d_lineV, p.d_col, Q, out, a, i, j, chunk * KC, b, 0, a.nview);
             }
         });
     }

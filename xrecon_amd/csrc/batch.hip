@@ -33,6 +33,7 @@ struct BatchDesc {
     int nb, any_y;
     int big;  // Plan3d::big
     unsigned long long ws_volx, ws_voly, ws_sino;  // byte offsets into the workspace
+    unsigned long long ws_part;                    // view-set partial volumes (va.vsplit > 1)
 };
 
 struct IOPtrs {
@@ -59,7 +60,11 @@ __host__ __device__ inline int blocks_sino_prefix(const BatchDesc &d)
 }
 __host__ __device__ inline int blocks_voxel(const BatchDesc &d, int KC)
 {
-    return ceil_div(d.va.nx, kVoxTile) * ceil_div(d.va.ny, kVoxTile) * d.nb * ceil_div(d.va.nz, KC);
+    return ceil_div(d.va.nx, kVoxTile) * ceil_div(d.va.ny, kVoxTile) * d.nb * ceil_div(d.va.nz, KC) * d.va.vsplit;
+}
+__host__ __device__ inline int blocks_split_sum(const BatchDesc &d)
+{
+    return d.va.vsplit > 1 ? ceil_div((int64_t)d.nb * d.va.nz * d.va.ny * d.va.nx, kBlock) : 0;
 }
 
 // Maps the flat block index to (instance, local block); returns the instance or -1.
@@ -174,10 +179,33 @@ __global__ __launch_bounds__(kBlock) void k_voxel_batch(const BatchDesc *__restr
     a.nchunk = ceil_div(a.nz, KC);
     const int gx = ceil_div(a.nx, kVoxTile), gy = ceil_div(a.ny, kVoxTile);
     const int bx = bid % gx, by = (bid / gx) % gy, bz = bid / (gx * gy);
+    const int set = bz % a.vsplit, rest = bz / a.vsplit;
+    const int v0 = set * a.vper, v1 = imin(a.nview, v0 + a.vper);
+    float *dst = a.vsplit > 1
+                     ? reinterpret_cast<float *>(ws + d.ws_part) + (int64_t)set * d.nb * a.nz * a.ny * a.nx
+                     : io.out[i];
     int ii, j;
     voxel_coords(bx, by, threadIdx.x, ii, j);
-    voxel_wg<OPK, GEO, KC>(d.hdr, d.lineV, d.col, d.vtile, reinterpret_cast<const float2 *>(ws + d.ws_sino), io.out[i],
-                           a, ii, j, ii < a.nx && j < a.ny, by * gx + bx, (bz % a.nchunk) * KC, bz / a.nchunk, tile);
+    voxel_wg<OPK, GEO, KC>(d.hdr, d.lineV, d.col, d.vtile, reinterpret_cast<const float2 *>(ws + d.ws_sino), dst, a, ii,
+                           j, ii < a.nx && j < a.ny, by * gx + bx, (rest % a.nchunk) * KC, rest / a.nchunk, v0, v1,
+                           tile);
+}
+
+// Sum of the view-set partial volumes of every split instance, in set order.
+__global__ __launch_bounds__(kBlock) void k_split_sum_batch(const BatchDesc *__restrict__ D, int n, IOPtrs io,
+                                                            const char *__restrict__ ws)
+{
+    int bid = blockIdx.x;
+    const int i = find_instance(D, n, bid, [](const BatchDesc &d) { return blocks_split_sum(d); });
+    if (i < 0) return;
+    const BatchDesc &d = D[i];
+    const int64_t total = (int64_t)d.nb * d.va.nz * d.va.ny * d.va.nx;
+    const int64_t e = (int64_t)bid * kBlock + threadIdx.x;
+    if (e >= total) return;
+    const float *part = reinterpret_cast<const float *>(ws + d.ws_part);
+    float acc = part[e];
+    for (int s = 1; s < d.va.vsplit; ++s) acc += part[(int64_t)s * total + e];
+    io.out[i][e] = acc;
 }
 
 template <int OPK, int GEO>
@@ -283,6 +311,8 @@ int xr_batch3d_create(xr_plan3d *const *plans, int n, const int *num_batch, xr_b
         off_cell += vb * (p.any_y ? 2 : 1);
         d.ws_sino = off_vox;
         off_vox += xr::align256(xr::sino_prefix_bytes(p, num_batch[i]));
+        d.ws_part = off_vox;
+        off_vox += xr::align256(xr::split_partial_bytes(p, num_batch[i]));
         b->desc.push_back(d);
         b->plans.push_back(&p);
     }
@@ -355,7 +385,7 @@ int xr_batch3d_execute(const xr_batch3d *b, int op, const float *const *in, floa
     hipStream_t s = static_cast<hipStream_t>(stream);
     char *ws = static_cast<char *>(workspace);
     const xr::BatchDesc *D = b->d_desc;
-    int64_t g_pre = 0, g_tr = 0, g_gather = 0;
+    int64_t g_pre = 0, g_tr = 0, g_gather = 0, g_sum = 0;
     for (const xr::BatchDesc &d : b->desc) {
         if (rt.cell) {
             g_pre += xr::blocks_vol_prefix(d);
@@ -364,9 +394,10 @@ int xr_batch3d_execute(const xr_batch3d *b, int op, const float *const *in, floa
         } else {
             g_pre += xr::blocks_sino_prefix(d);
             g_gather += xr::blocks_voxel(d, KC);
+            g_sum += xr::blocks_split_sum(d);
         }
     }
-    if (g_pre >= INT32_MAX || g_tr >= INT32_MAX || g_gather >= INT32_MAX)
+    if (g_pre >= INT32_MAX || g_tr >= INT32_MAX || g_gather >= INT32_MAX || g_sum >= INT32_MAX)
         return fail(XR_EINVAL, "batch too large for one launch");
     const int kind = p0.kind;
     if (rt.cell) {
@@ -381,6 +412,7 @@ int xr_batch3d_execute(const xr_batch3d *b, int op, const float *const *in, floa
     if (kind == xr::GEO_FLAT) xr::dispatch_gather<xr::GEO_FLAT>(rt, R, big, KC, (int)g_gather, D, n, io, ws, s);
     else if (kind == xr::GEO_ARC) xr::dispatch_gather<xr::GEO_ARC>(rt, R, big, KC, (int)g_gather, D, n, io, ws, s);
     else xr::dispatch_gather<xr::GEO_PAR>(rt, R, big, KC, (int)g_gather, D, n, io, ws, s);
+    if (g_sum > 0) hipLaunchKernelGGL(xr::k_split_sum_batch, dim3((int)g_sum), dim3(xr::kBlock), 0, s, D, n, io, ws);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(XR_EHIP, std::string("batched launch failed: ") + hipGetErrorString(e));
     return XR_OK;

@@ -49,6 +49,8 @@ struct VoxArgs {
     float dx, dy, shx, shy;
     float dvv;
     float hscale;  // dvv / dz
+    int vsplit;    // view sets: set s sums views [s * vper, min(V, (s + 1) * vper))
+    int vper;
 };
 
 struct SinoArgs {
@@ -112,6 +114,18 @@ inline size_t sino_prefix_bytes(const Plan3d &p, int nbatch)
     return ((size_t)nbatch * p.nview * (size_t)(p.geo.nv + 1) * p.geo.nu + kSinoPad) * sizeof(float2);
 }
 
+// View splitting of the voxel gathers.  A plan whose volume offers fewer than
+// kVoxMinGroups workgroups (16 x 16 voxel columns x default slice chunk) splits its views
+// into up to 16 sets, each summed by its own workgroups into a partial volume; a second
+// pass adds the partials in set order.  Deterministic, and identical in the batched path.
+constexpr int kVoxMinGroups = 1024;
+
+inline size_t split_partial_bytes(const Plan3d &p, int nbatch)
+{
+    if (p.vsplit <= 1) return 0;
+    return (size_t)p.vsplit * nbatch * (size_t)p.vol.nz * p.vol.ny * p.vol.nx * sizeof(float);
+}
+
 // Default tile shapes (the autotuner may choose others).
 inline int default_rows(const Plan3d &p)
 {
@@ -155,6 +169,8 @@ inline VoxArgs make_vox_args(const Plan3d &p, int nbatch, int KC)
     a.dx = p.vol.dx; a.dy = p.vol.dy; a.shx = p.vol.shift_x; a.shy = p.vol.shift_y;
     a.dvv = p.dvv;
     a.hscale = p.dvv / p.dz;
+    a.vsplit = p.vsplit;
+    a.vper = (p.nview + p.vsplit - 1) / p.vsplit;
     return a;
 }
 
@@ -726,24 +742,26 @@ __host__ __device__ inline void voxel_store(float *__restrict__ out, const VoxAr
 }
 
 // One thread, reading the tables in HBM only: voxel column (i, j), slices k0..k0+KC-1,
-// batch b.
+// batch b, views [v0, v1).
 template <int OPK, int GEO, int KC>
 __host__ __device__ inline void voxel_body(const ViewHdr *__restrict__ hdr, const float4 *__restrict__ lineV,
                                            const float4 *__restrict__ colt, const float2 *__restrict__ Q,
-                                           float *__restrict__ out, const VoxArgs &a, int i, int j, int k0, int b)
+                                           float *__restrict__ out, const VoxArgs &a, int i, int j, int k0, int b,
+                                           int v0, int v1)
 {
     const float px = voxel_px(a, i), py = voxel_py(a, j);
     float acc[KC];
     for (int q = 0; q < KC; ++q) acc[q] = 0.0f;
     const VoxBox none{0, 0, 0, 0, 0};
-    for (int v = 0; v < a.nview; ++v)
+    for (int v = v0; v < v1; ++v)
         voxel_view<OPK, GEO, KC>(hdr[v], lineV, colt + (int64_t)v * (a.nu + 1), view_table(Q, a, b, v), a, v, i, j,
                                  k0, px, py, acc, none, nullptr);
     voxel_store<KC>(out, a, i, j, k0, b, acc);
 }
 
 // A workgroup of the voxel gather (device only): the 16 x 16 tile tile_id of voxel
-// columns, slices k0..k0+KC-1, batch b; this thread owns (i, j) when `active`.  Per view the
+// columns, slices k0..k0+KC-1, batch b, views [v0, v1); this thread owns (i, j) when
+// `active`.  Per view the
 // workgroup stages the block of the row-prefix table its footprint reads into LDS, then
 // every thread gathers from it.  Inactive threads take part in staging and barriers.
 // (A second, alternating buffer that overlaps the next view's staging with this view's
@@ -757,17 +775,18 @@ template <int OPK, int GEO, int KC>
 __device__ inline void voxel_wg(const ViewHdr *__restrict__ hdr, const float4 *__restrict__ lineV,
                                 const float4 *__restrict__ colt, const VoxTile *__restrict__ vtile,
                                 const float2 *__restrict__ Q, float *__restrict__ out, const VoxArgs &a, int i, int j,
-                                bool active, int tile_id, int k0, int b, float2 *__restrict__ tile)
+                                bool active, int tile_id, int k0, int b, int v0, int v1,
+                                float2 *__restrict__ tile)
 {
     if (KC < kVoxStageMin) {
-        if (active) voxel_body<OPK, GEO, KC>(hdr, lineV, colt, Q, out, a, i, j, k0, b);
+        if (active) voxel_body<OPK, GEO, KC>(hdr, lineV, colt, Q, out, a, i, j, k0, b, v0, v1);
         return;
     }
     const float px = voxel_px(a, i), py = voxel_py(a, j);
     float acc[KC];
     for (int q = 0; q < KC; ++q) acc[q] = 0.0f;
     const int ntile = ceil_div(a.nx, kVoxTile) * ceil_div(a.ny, kVoxTile);
-    for (int v = 0; v < a.nview; ++v) {
+    for (int v = v0; v < v1; ++v) {
         const ViewHdr h = hdr[v];
         const float4 *ct = colt + (int64_t)v * (a.nu + 1);
         const float2 *Qv = view_table(Q, a, b, v);

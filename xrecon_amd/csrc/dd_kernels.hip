@@ -110,15 +110,31 @@ __global__ __launch_bounds__(kBlock) void k_cell(const ViewHdr *__restrict__ hdr
 template <int OPK, int GEO, int KC>
 __global__ __launch_bounds__(kBlock) void k_voxel(const ViewHdr *__restrict__ hdr, const float4 *__restrict__ lineV,
                                                   const float4 *__restrict__ colt, const VoxTile *__restrict__ vtile,
-                                                  const float2 *__restrict__ Q, float *__restrict__ out, VoxArgs a)
+                                                  const float2 *__restrict__ Q, float *__restrict__ out,
+                                                  float *__restrict__ part, VoxArgs a)
 {
     __shared__ __attribute__((aligned(16))) float2 tile[kVoxLds];
     int i, j;
     voxel_coords(blockIdx.x, blockIdx.y, threadIdx.x, i, j);
-    const int chunk = blockIdx.z % a.nchunk;
-    const int b = blockIdx.z / a.nchunk;
-    voxel_wg<OPK, GEO, KC>(hdr, lineV, colt, vtile, Q, out, a, i, j, i < a.nx && j < a.ny,
-                           blockIdx.y * gridDim.x + blockIdx.x, chunk * KC, b, tile);
+    const int set = blockIdx.z % a.vsplit, rest = blockIdx.z / a.vsplit;
+    const int chunk = rest % a.nchunk;
+    const int b = rest / a.nchunk;
+    const int v0 = set * a.vper, v1 = imin(a.nview, v0 + a.vper);
+    // one view set: its own partial volume; no split: the output itself
+    float *dst = a.vsplit > 1 ? part + (int64_t)set * a.nbatch * a.nz * a.ny * a.nx : out;
+    voxel_wg<OPK, GEO, KC>(hdr, lineV, colt, vtile, Q, dst, a, i, j, i < a.nx && j < a.ny,
+                           blockIdx.y * gridDim.x + blockIdx.x, chunk * KC, b, v0, v1, tile);
+}
+
+// out[e] = sum over sets s = 0..nset-1 of part[s * n + e], in set order.
+__global__ __launch_bounds__(kBlock) void k_split_sum(const float *__restrict__ part, float *__restrict__ out,
+                                                      int64_t n, int nset)
+{
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+        float acc = part[e];
+        for (int s = 1; s < nset; ++s) acc += part[(int64_t)s * n + e];
+        out[e] = acc;
+    }
 }
 
 template <int OPK, int GEO>
@@ -143,15 +159,15 @@ void launch_cell(const Plan3d &p, int R, const float2 *WX, const float2 *WY, flo
 }
 
 template <int OPK, int GEO>
-void launch_voxel(const Plan3d &p, int KC, const float2 *Q, float *out, int nbatch, hipStream_t s)
+void launch_voxel(const Plan3d &p, int KC, const float2 *Q, float *out, float *part, int nbatch, hipStream_t s)
 {
     const VoxArgs a = make_vox_args(p, nbatch, KC);
-    dim3 grid(ceil_div(a.nx, kVoxTile), ceil_div(a.ny, kVoxTile), nbatch * a.nchunk);
+    dim3 grid(ceil_div(a.nx, kVoxTile), ceil_div(a.ny, kVoxTile), nbatch * a.nchunk * a.vsplit);
     switch (KC) {
-    case 1: hipLaunchKernelGGL((k_voxel<OPK, GEO, 1>), grid, dim3(kBlock), 0, s, p.d_hdr, p.d_lineV, p.d_col, p.d_vtile, Q, out, a); break;
-    case 4: hipLaunchKernelGGL((k_voxel<OPK, GEO, 4>), grid, dim3(kBlock), 0, s, p.d_hdr, p.d_lineV, p.d_col, p.d_vtile, Q, out, a); break;
-    case 8: hipLaunchKernelGGL((k_voxel<OPK, GEO, 8>), grid, dim3(kBlock), 0, s, p.d_hdr, p.d_lineV, p.d_col, p.d_vtile, Q, out, a); break;
-    default: hipLaunchKernelGGL((k_voxel<OPK, GEO, 16>), grid, dim3(kBlock), 0, s, p.d_hdr, p.d_lineV, p.d_col, p.d_vtile, Q, out, a); break;
+    case 1: hipLaunchKernelGGL((k_voxel<OPK, GEO, 1>), grid, dim3(kBlock), 0, s, p.d_hdr, p.d_lineV, p.d_col, p.d_vtile, Q, out, part, a); break;
+    case 4: hipLaunchKernelGGL((k_voxel<OPK, GEO, 4>), grid, dim3(kBlock), 0, s, p.d_hdr, p.d_lineV, p.d_col, p.d_vtile, Q, out, part, a); break;
+    case 8: hipLaunchKernelGGL((k_voxel<OPK, GEO, 8>), grid, dim3(kBlock), 0, s, p.d_hdr, p.d_lineV, p.d_col, p.d_vtile, Q, out, part, a); break;
+    default: hipLaunchKernelGGL((k_voxel<OPK, GEO, 16>), grid, dim3(kBlock), 0, s, p.d_hdr, p.d_lineV, p.d_col, p.d_vtile, Q, out, part, a); break;
     }
 }
 
@@ -165,11 +181,12 @@ void launch_cell_geo(const Plan3d &p, int kop, int R, const float2 *WX, const fl
 }
 
 template <int GEO>
-void launch_voxel_geo(const Plan3d &p, int kop, int KC, const float2 *Q, float *out, int nb, hipStream_t s)
+void launch_voxel_geo(const Plan3d &p, int kop, int KC, const float2 *Q, float *out, float *part, int nb,
+                      hipStream_t s)
 {
-    if (kop == VOX_B) launch_voxel<VOX_B, GEO>(p, KC, Q, out, nb, s);
-    else if (kop == VOX_BW) launch_voxel<VOX_BW, GEO>(p, KC, Q, out, nb, s);
-    else launch_voxel<VOX_AT, GEO>(p, KC, Q, out, nb, s);
+    if (kop == VOX_B) launch_voxel<VOX_B, GEO>(p, KC, Q, out, part, nb, s);
+    else if (kop == VOX_BW) launch_voxel<VOX_BW, GEO>(p, KC, Q, out, part, nb, s);
+    else launch_voxel<VOX_AT, GEO>(p, KC, Q, out, part, nb, s);
 }
 
 template <int GEO>
@@ -196,7 +213,7 @@ size_t workspace_bytes(const Plan3d &p, int op, int nbatch)
     const Route rt = route(p, op);
     if (rt.kop < 0) return 0;
     if (rt.cell) return align256(vol_prefix_bytes(p, nbatch)) * (p.any_y ? 2 : 1);
-    return align256(sino_prefix_bytes(p, nbatch));
+    return align256(sino_prefix_bytes(p, nbatch)) + align256(split_partial_bytes(p, nbatch));
 }
 
 int execute(const Plan3d &p, int op, const float *in, float *out, int nbatch, void *ws, size_t ws_bytes,
@@ -259,9 +276,17 @@ int execute_phase(const Plan3d &p, int op, int phase, const float *in, float *ou
             else launch_sino_prefix<GEO_PAR>(p, at, in, Q, nbatch, s);
         }
         if (do_gather) {
-            if (p.kind == GEO_FLAT) launch_voxel_geo<GEO_FLAT>(p, rt.kop, voxel_chunk, Q, out, nbatch, s);
-            else if (p.kind == GEO_ARC) launch_voxel_geo<GEO_ARC>(p, rt.kop, voxel_chunk, Q, out, nbatch, s);
-            else launch_voxel_geo<GEO_PAR>(p, rt.kop, voxel_chunk, Q, out, nbatch, s);
+            float *part = p.vsplit > 1
+                              ? reinterpret_cast<float *>(static_cast<char *>(ws) + align256(sino_prefix_bytes(p, nbatch)))
+                              : nullptr;
+            if (p.kind == GEO_FLAT) launch_voxel_geo<GEO_FLAT>(p, rt.kop, voxel_chunk, Q, out, part, nbatch, s);
+            else if (p.kind == GEO_ARC) launch_voxel_geo<GEO_ARC>(p, rt.kop, voxel_chunk, Q, out, part, nbatch, s);
+            else launch_voxel_geo<GEO_PAR>(p, rt.kop, voxel_chunk, Q, out, part, nbatch, s);
+            if (p.vsplit > 1) {
+                const int64_t n = (int64_t)nbatch * p.vol.nz * p.vol.ny * p.vol.nx;
+                hipLaunchKernelGGL(k_split_sum, dim3((unsigned)std::min<int64_t>(ceil_div(n, kBlock), 8192)),
+                                   dim3(kBlock), 0, s, part, out, n, p.vsplit);
+            }
         }
     }
     const hipError_t e = hipGetLastError();

@@ -254,8 +254,15 @@ int build_tables(Plan3d &p, const float *angles, PlanTables &t, std::string &err
         }
     }
 
-    // Voxel-gather staging estimates per (view, 16 x 16 tile of voxel columns).
+    // Voxel gathers on volumes too small to fill the GPU split their views into sets.
     const int gx = ceil_div(nx, kVoxTile), gy = ceil_div(ny, kVoxTile);
+    {
+        const int64_t groups = (int64_t)gx * gy * ceil_div(nz, default_chunk(p));
+        int sp = groups >= kVoxMinGroups ? 1 : (int)std::min<int64_t>(16, ceil_div(kVoxMinGroups, groups));
+        p.vsplit = std::max(1, std::min(sp, V));
+    }
+
+    // Voxel-gather staging estimates per (view, 16 x 16 tile of voxel columns).
     t.vtile.assign((size_t)V * gx * gy, VoxTile{0, 0, 0.0f, 0.0f});
     for (int v = 0; v < V; ++v) {
         const float4 *ct = t.col.data() + (size_t)v * (nu + 1);

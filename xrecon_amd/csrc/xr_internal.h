@@ -63,6 +63,7 @@ struct Plan3d {
     int maxline = 0;         // max(Nx, Ny)
     int any_x = 0, any_y = 0;
     int big = 0;             // z-prefix table of one batch item >= 2^32 bytes or plane >= 2^24 bytes
+    int vsplit = 1;          // voxel gathers: views split over this many workgroup sets (summed after)
     // Scalars shared by all views.
     float zb = 0, dz = 1;    // volume bottom z and slice thickness
     float v0 = 0, dvv = 1;   // first row boundary and row pitch on the iso plane
