@@ -17,30 +17,80 @@ using namespace xr;
 
 namespace {
 
-// Cell-gather variant: 0 = the one the library launches (fixed-span paths by view,
-// 64-bit offsets for big plans), 1 = the general 32-bit path, 2 = the general 64-bit path.
+// Cell-gather variant (see run_cell_RV): 0 = as launched, 1 = general 32-bit, 2 = general
+// 64-bit, 3 = fixed spans without staging.
 int g_cell_variant = 0;
 
-template <int OPK, int GEO, int R, bool FIXED, bool BIG>
+// One wave of the staged fixed-span cell gather, lanes in lockstep per line: the block
+// estimate, every lane's staging share, then every lane's gather -- the order the wave's
+// own wait imposes on the device.
+template <int OPK, int GEO, int R, int SPAN>
+void emu_cell_wave(const Plan3d &p, const CellArgs &a, const float2 *WX, const float2 *WY, float *out, int cg, int r0,
+                   int v, int b)
+{
+    const ViewHdr &h = p.d_hdr[v];
+    const float4 *ct = p.d_col + (int64_t)v * (a.nu + 1);
+    std::vector<CellLane<R>> st(kWave);
+    std::vector<char> ok(kWave);
+    for (int lane = 0; lane < kWave; ++lane) {
+        const int c = cg * kWave + lane;
+        ok[lane] = c < a.nu;
+        cell_init<R>(st[lane], ct, a, ok[lane] ? c : cg * kWave, r0);
+    }
+    const float2 *W = cell_table(WX, WY, h, a, b);
+    const float4 *lt = p.d_lineC + (int64_t)v * a.maxline;
+    const float dfirst = ct[cg * kWave].x, dlast = ct[imin(a.nu, cg * kWave + kWave)].x;
+    const float2 sg = p.d_csig[(int64_t)v * ceil_div(a.nu, kWave) + cg];
+    std::vector<float2> lds(cell_lds_cap<R>());
+    for (int L = 0; L < h.nline; ++L) {
+        const float4 lc = lt[L];
+        const CellBox bx = cell_box<R, SPAN>(a, lc, dfirst, dlast, sg.x, sg.y, r0, h.nalong);
+        if (bx.pitch == 0) continue;
+        for (int lane = 0; lane < kWave; ++lane) cell_stage(W + (int64_t)L * h.nalong, bx, lds.data(), a.plane8, lane);
+        for (int lane = 0; lane < kWave; ++lane)
+            if (ok[lane]) cell_line<OPK, GEO, R, SPAN, false>(st[lane], h, a, lc, L, W, bx, lds.data());
+    }
+    for (int lane = 0; lane < kWave; ++lane)
+        if (ok[lane]) cell_store<OPK, GEO, R>(st[lane], h, a, out, cg * kWave + lane, r0, v, b);
+}
+
+// MODE 0: what the library launches (fixed-span views staged per wave, the rest through
+// the general loop; big plans 64-bit general); 1: general loop, 32-bit offsets, for every
+// view; 2: general loop, 64-bit offsets; 3: fixed-span paths read from the table (no
+// staging).
+template <int OPK, int GEO, int R, int MODE>
 void run_cell_RV(const Plan3d &p, const float2 *WX, const float2 *WY, float *out, int nb, int nthreads)
 {
     const CellArgs a = make_cell_args(p, nb);
     const int ngroups = (a.nv + R - 1) / R;
+    const int ncg = ceil_div(a.nu, kWave);
     const int items = nb * p.nview;
     std::vector<std::thread> th;
     for (int t = 0; t < nthreads; ++t) {
         th.emplace_back([&, t]() {
             for (int it = t; it < items; it += nthreads) {
                 const int v = it % p.nview, b = it / p.nview;
-                const int span = FIXED ? p.d_hdr[v].span : 0;
-                for (int g = 0; g < ngroups; ++g)
-                    for (int c = 0; c < a.nu; ++c)
+                const int span = (MODE == 0 || MODE == 3) && !p.big ? p.d_hdr[v].span : 0;
+                for (int g = 0; g < ngroups; ++g) {
+                    if (MODE == 0 && span == 2) {
+                        for (int cg = 0; cg < ncg; ++cg) emu_cell_wave<OPK, GEO, R, 2>(p, a, WX, WY, out, cg, g * R, v, b);
+                        continue;
+                    }
+                    if (MODE == 0 && span == 3) {
+                        for (int cg = 0; cg < ncg; ++cg) emu_cell_wave<OPK, GEO, R, 3>(p, a, WX, WY, out, cg, g * R, v, b);
+                        continue;
+                    }
+                    for (int c = 0; c < a.nu; ++c) {
                         if (span == 2)
                             cell_body<OPK, GEO, R, 2, false>(p.d_hdr, p.d_lineC, p.d_col, WX, WY, out, a, c, g * R, v, b);
                         else if (span == 3)
                             cell_body<OPK, GEO, R, 3, false>(p.d_hdr, p.d_lineC, p.d_col, WX, WY, out, a, c, g * R, v, b);
+                        else if (MODE == 2 || p.big)
+                            cell_body<OPK, GEO, R, 0, true>(p.d_hdr, p.d_lineC, p.d_col, WX, WY, out, a, c, g * R, v, b);
                         else
-                            cell_body<OPK, GEO, R, 0, BIG>(p.d_hdr, p.d_lineC, p.d_col, WX, WY, out, a, c, g * R, v, b);
+                            cell_body<OPK, GEO, R, 0, false>(p.d_hdr, p.d_lineC, p.d_col, WX, WY, out, a, c, g * R, v, b);
+                    }
+                }
             }
         });
     }
@@ -50,10 +100,10 @@ void run_cell_RV(const Plan3d &p, const float2 *WX, const float2 *WY, float *out
 template <int OPK, int GEO, int R>
 void run_cell_R(const Plan3d &p, const float2 *WX, const float2 *WY, float *out, int nb, int nt)
 {
-    const int var = g_cell_variant ? g_cell_variant : (p.big ? 2 : 0);
-    if (var == 2) run_cell_RV<OPK, GEO, R, false, true>(p, WX, WY, out, nb, nt);
-    else if (var == 1) run_cell_RV<OPK, GEO, R, false, false>(p, WX, WY, out, nb, nt);
-    else run_cell_RV<OPK, GEO, R, true, false>(p, WX, WY, out, nb, nt);
+    if (g_cell_variant == 1) run_cell_RV<OPK, GEO, R, 1>(p, WX, WY, out, nb, nt);
+    else if (g_cell_variant == 2) run_cell_RV<OPK, GEO, R, 2>(p, WX, WY, out, nb, nt);
+    else if (g_cell_variant == 3) run_cell_RV<OPK, GEO, R, 3>(p, WX, WY, out, nb, nt);
+    else run_cell_RV<OPK, GEO, R, 0>(p, WX, WY, out, nb, nt);
 }
 
 template <int OPK, int GEO>
@@ -231,6 +281,7 @@ int emu_op3d(int op, const int *vn, const float *vf, const int *gn, const float 
     p.d_lineV = t.lineV.data();
     p.d_col = t.col.data();
     p.d_vtile = t.vtile.data();
+    p.d_csig = t.csig.data();
     const Route rt = route(p, op);
     if (rt.kop < 0) {
         g_err = "unknown op";

@@ -27,6 +27,7 @@ struct BatchDesc {
     const ViewHdr *hdr;
     const float4 *lineC, *lineV, *col;
     const VoxTile *vtile;
+    const float2 *csig;
     CellArgs ca;
     VoxArgs va;  // nchunk is filled per launch from the launch's voxel chunk
     SinoArgs sa;
@@ -128,6 +129,7 @@ template <int OPK, int GEO, int R, bool BIG>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_cell_batch(const BatchDesc *__restrict__ D, int n, IOPtrs io,
                                                        char *__restrict__ ws)
 {
+    __shared__ __attribute__((aligned(16))) float2 lds[kWavesPerBlock][cell_lds_cap<R>()];
     int bid = blockIdx.x;
     const int i = find_instance(D, n, bid, [](const BatchDesc &d) { return blocks_cell(d, R); });
     if (i < 0) return;
@@ -138,17 +140,24 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
     const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
     const int c = bx * kWave + lane;
     const int r0 = (by * kWavesPerBlock + wave) * R;
-    if (c >= a.nu || r0 >= a.nv) return;
+    if (r0 >= a.nv) return;  // the whole wave
     const float2 *WX = reinterpret_cast<const float2 *>(ws + d.ws_volx);
     const float2 *WY = reinterpret_cast<const float2 *>(ws + d.ws_voly);
     const int v = bz % a.nview, b = bz / a.nview;
     // the per-view variant choice is uniform over the workgroup; a batch holding a big
     // plan runs every instance through the 64-bit general path
     const int span = BIG ? 0 : d.hdr[v].span;
-    if (BIG) cell_body<OPK, GEO, R, 0, true>(d.hdr, d.lineC, d.col, WX, WY, io.out[i], a, c, r0, v, b);
-    else if (span == 2) cell_body<OPK, GEO, R, 2, false>(d.hdr, d.lineC, d.col, WX, WY, io.out[i], a, c, r0, v, b);
-    else if (span == 3) cell_body<OPK, GEO, R, 3, false>(d.hdr, d.lineC, d.col, WX, WY, io.out[i], a, c, r0, v, b);
-    else cell_body<OPK, GEO, R, 0, false>(d.hdr, d.lineC, d.col, WX, WY, io.out[i], a, c, r0, v, b);
+    if (BIG) {
+        if (c < a.nu) cell_body<OPK, GEO, R, 0, true>(d.hdr, d.lineC, d.col, WX, WY, io.out[i], a, c, r0, v, b);
+    } else if (span == 2) {
+        cell_wave<OPK, GEO, R, 2>(d.hdr, d.lineC, d.col, d.csig, WX, WY, io.out[i], a, c, c < a.nu, bx, r0, v, b,
+                                  lds[wave]);
+    } else if (span == 3) {
+        cell_wave<OPK, GEO, R, 3>(d.hdr, d.lineC, d.col, d.csig, WX, WY, io.out[i], a, c, c < a.nu, bx, r0, v, b,
+                                  lds[wave]);
+    } else if (c < a.nu) {
+        cell_body<OPK, GEO, R, 0, false>(d.hdr, d.lineC, d.col, WX, WY, io.out[i], a, c, r0, v, b);
+    }
 }
 
 template <int MODE, int GEO>
@@ -299,6 +308,7 @@ int xr_batch3d_create(xr_plan3d *const *plans, int n, const int *num_batch, xr_b
         d.lineV = p.d_lineV;
         d.col = p.d_col;
         d.vtile = p.d_vtile;
+        d.csig = p.d_csig;
         d.ca = xr::make_cell_args(p, num_batch[i]);
         d.va = xr::make_vox_args(p, num_batch[i], 1);
         d.sa = xr::make_sino_args(p, num_batch[i]);

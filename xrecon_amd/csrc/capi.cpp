@@ -88,9 +88,11 @@ void free_tables(Plan3d &p)
     if (p.d_lineV) (void)hipFree(p.d_lineV);
     if (p.d_col) (void)hipFree(p.d_col);
     if (p.d_vtile) (void)hipFree(p.d_vtile);
+    if (p.d_csig) (void)hipFree(p.d_csig);
     p.d_hdr = nullptr;
     p.d_lineC = p.d_lineV = p.d_col = nullptr;
     p.d_vtile = nullptr;
+    p.d_csig = nullptr;
 }
 
 }  // namespace
@@ -125,7 +127,8 @@ int xr_plan3d_create(const xr_volume3d *vol, const xr_geometry3d *geo, const flo
     if (rc) return fail(rc, err);
     DeviceGuard g(device);
     if ((rc = upload(&p.d_hdr, t.hdr)) || (rc = upload(&p.d_lineC, t.lineC)) || (rc = upload(&p.d_lineV, t.lineV)) ||
-        (rc = upload(&p.d_col, t.col)) || (rc = upload(&p.d_vtile, t.vtile))) {
+        (rc = upload(&p.d_col, t.col)) || (rc = upload(&p.d_vtile, t.vtile)) ||
+        (rc = upload(&p.d_csig, t.csig))) {
         free_tables(p);
         return rc;
     }

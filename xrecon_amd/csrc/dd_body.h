@@ -336,41 +336,131 @@ __host__ __device__ inline void sino_prefix_body(const ViewHdr *__restrict__ hdr
 
 // ---------------------------------------------------------------- cell gather
 
-// One thread: internal column c, rows r0..r0+R-1 of view v, batch b.
-//
-// SPAN = 2 or 3: every column of the view overlaps at most SPAN consecutive pixels of any
-// line (ViewHdr::span), so each line takes a fixed SPAN-pixel path with no pixel loop;
-// SPAN = 0 is the general loop.  The kernels pick the variant per view at entry (one
-// view per workgroup), outside the line loop.
-// BIG: one batch item's z-prefix table reaches 2^32 bytes or a z-plane 2^24 bytes, so
-// table offsets are taken in 64-bit arithmetic; otherwise they are 32-bit byte offsets
-// (one 24-bit multiply-add) over a wave-uniform line base.
-template <int OPK, int GEO, int R, int SPAN, bool BIG>
-__host__ __device__ inline void cell_body(const ViewHdr *__restrict__ hdr, const float4 *__restrict__ lineC,
-                                          const float4 *__restrict__ colt, const float2 *__restrict__ WX,
-                                          const float2 *__restrict__ WY, float *__restrict__ out, const CellArgs &a,
-                                          int c, int r0, int v, int b)
-{
-    const ViewHdr h = hdr[v];
-    const float4 *ct = colt + (int64_t)v * (a.nu + 1);
-    const float4 cc = ct[c];
-    const float d0 = cc.x, d1 = ct[c + 1].x;
+// Per-thread state of the cell gather: internal column c (boundaries d0, d1, table entry
+// cc), rows r0..r0+R-1 (row boundary heights hv on the axis), accumulators.
+template <int R>
+struct CellLane {
+    float4 cc;
+    float d0, d1;
     float hv[R + 1];
-    for (int q = 0; q <= R; ++q) hv[q] = cc.y * (a.v0 + (float)(r0 + q) * a.dvv);
-
-    const int nline = h.nline, nalong = h.nalong;
-    const int plane = nline * nalong;
-    const float2 *W = (h.xdrv ? WX : WY) + (int64_t)b * (int64_t)(a.nz + 1) * plane;
-    const float4 *lt = lineC + (int64_t)v * a.maxline;
-    const float fnz = (float)a.nz, fal = (float)nalong;
-
     float acc[R];
-    for (int q = 0; q < R; ++q) acc[q] = 0.0f;
+};
 
-    // Weight of pixel m of line L: its column overlap ou (pixel units) times the
-    // per-line factor of the operator.
-    auto pixel_weight = [&](float ou, float lam, int L, int m) -> float {
-        if (OPK == CELL_A) return ou * lam;
+template <int R>
+__host__ __device__ inline void cell_init(CellLane<R> &st, const float4 *__restrict__ ct, const CellArgs &a, int c,
+                                          int r0)
+{
+    st.cc = ct[c];
+    st.d0 = st.cc.x;
+    st.d1 = ct[c + 1].x;
+    for (int q = 0; q <= R; ++q) st.hv[q] = st.cc.y * (a.v0 + (float)(r0 + q) * a.dvv);
+    for (int q = 0; q < R; ++q) st.acc[q] = 0.0f;
+}
+
+// Block of z-prefix entries a wave stages in LDS for one line: planes Z0 .. Z0+NZ-1,
+// pixels P0 .. P0+pitch-1 (row-major, pitch even).  pitch = 0: nothing staged.
+struct CellBox {
+    int P0, pitch, Z0, NZ;
+};
+
+// Capacity of one wave's staged block in float2 entries: 8 KiB for 8-row tiles (16 waves
+// per CU fit), 12 KiB for 16-row tiles (at most 12 waves per CU by registers).
+template <int R>
+__host__ __device__ constexpr int cell_lds_cap()
+{
+    return R <= 8 ? 1024 : 1536;
+}
+
+// The block a wave needs on the line with table coefficients lc: pixels from the first
+// column's left boundary to SPAN-1 past the last column's right boundary, planes over
+// the wave's rows at the extremes of the column depth scale sigma of its 64 columns.
+// Lanes whose lookups leave it read the table in HBM, so it decides speed, not results.
+template <int R, int SPAN>
+__host__ __device__ inline CellBox cell_box(const CellArgs &a, const float4 &lc, float dfirst, float dlast,
+                                            float sig_lo, float sig_hi, int r0, int nalong)
+{
+    CellBox bx{0, 0, 0, 0};
+    const float t0 = fmaf(dfirst, lc.y, lc.x), t1 = fmaf(dlast, lc.y, lc.x);
+    const float fal = (float)nalong;
+    if (!(t1 > 0.0f && t0 < fal)) return bx;
+    const int P0 = t0 > 0.0f ? (int)t0 : 0;
+    const int Pend = imin((t1 < fal ? (int)t1 : nalong) + SPAN, nalong + 1);
+    const int pitch = (Pend - P0 + 1) & ~1;
+    const float va = fmaf((float)r0, a.dvv, a.v0), vb = fmaf((float)(r0 + R), a.dvv, a.v0);
+    const float hmin = fminf(fminf(sig_lo * va, sig_lo * vb), fminf(sig_hi * va, sig_hi * vb));
+    const float hmax = fmaxf(fmaxf(sig_lo * va, sig_lo * vb), fmaxf(sig_hi * va, sig_hi * vb));
+    const float fnz = (float)a.nz;
+    const int Z0 = (int)clampf(fmaf(hmin, lc.z, a.zoff), 0.0f, fnz);
+    int NZ = (int)clampf(fmaf(hmax, lc.z, a.zoff), 0.0f, fnz) - Z0 + 1;
+    constexpr int cap = cell_lds_cap<R>();
+    if (pitch <= 0 || pitch > cap) return bx;
+    if (NZ * pitch > cap) NZ = cap / pitch;
+    bx.P0 = P0;
+    bx.pitch = pitch;
+    bx.Z0 = Z0;
+    bx.NZ = NZ;
+    return bx;
+}
+
+// Lane `lane`'s share of copying a wave's block from the line's z-prefix entries (Wl =
+// entry (plane 0, this line, pixel 0); plane stride plane8 bytes) into the wave's LDS
+// block, in 16-byte pixel pairs: pair p = lane + 64 s goes to lds[2p, 2p+1].  A pair may
+// end one entry past the line (the next line's first entry, or the pad after the table);
+// that entry is only ever weighted by zero.  On the device the copies are direct
+// global-to-LDS loads (lane l writes the wave's LDS base + 16 l); the caller waits for
+// them (s_waitcnt) before reading the block.
+__host__ __device__ inline void cell_stage(const float2 *__restrict__ Wl, const CellBox &bx, float2 *__restrict__ lds,
+                                           unsigned plane8, int lane)
+{
+    const int half = bx.pitch >> 1;
+    const int E2 = bx.NZ * half;
+    if (E2 <= 0) return;
+    // (plane, pair) of p = lane, advanced by 64 = dq planes + dc pairs per step; the
+    // divisions by half are exact (quotients stay >= 1/(2 half) from integers).
+    const float inv = 1.0f / (float)half;
+    int r = (int)(((float)lane + 0.5f) * inv);
+    int c = lane - r * half;
+    const int dq = (int)(((float)kWave + 0.5f) * inv), dc = kWave - dq * half;
+    const char *Wb = reinterpret_cast<const char *>(Wl);
+    for (int s = 0; s * kWave < E2; ++s) {
+        const int p = lane + s * kWave;
+        const unsigned off = p < E2 ? mad24((unsigned)(bx.Z0 + r), plane8, (unsigned)(bx.P0 + 2 * c) * 8u) : 0u;
+#if defined(__HIP_DEVICE_COMPILE__)
+        __builtin_amdgcn_global_load_lds(Wb + off, (__attribute__((address_space(3))) void *)(lds + 2 * s * kWave), 16,
+                                         0, 0);
+#else
+        if (p < E2) *reinterpret_cast<Pair2 *>(lds + 2 * p) = *reinterpret_cast<const Pair2 *>(Wb + off);
+#endif
+        r += dq;
+        c += dc;
+        if (c >= half) {
+            c -= half;
+            ++r;
+        }
+    }
+}
+
+// One line L (table coefficients lc) of the cell gather for this thread.  W: the batch
+// item's z-prefix table in the view's orientation.  SPAN = 2 or 3: every column of the
+// view overlaps at most SPAN consecutive pixels of any line (ViewHdr::span) and the line
+// takes a fixed SPAN-pixel path, reading the wave's staged block `lds` when given (and
+// when the lane's lookups fall in it); SPAN = 0: the general pixel loop.  BIG: 64-bit
+// table offsets (entries), otherwise 32-bit byte offsets over the wave-uniform line base.
+template <int OPK, int GEO, int R, int SPAN, bool BIG>
+__host__ __device__ inline void cell_line(CellLane<R> &st, const ViewHdr &h, const CellArgs &a, const float4 &lc,
+                                          int L, const float2 *__restrict__ W, const CellBox &bx,
+                                          const float2 *__restrict__ lds)
+{
+    const int nalong = h.nalong;
+    const float fal = (float)nalong;
+    // Column boundaries in pixel units along the line.
+    const float t0 = fmaf(st.d0, lc.y, lc.x);
+    const float t1 = fmaf(st.d1, lc.y, lc.x);
+    if (!(t1 > 0.0f && t0 < fal)) return;
+
+    // Weight of pixel m: its column overlap ou (pixel units) times the operator's factor.
+    auto pixel_weight = [&](float ou, int m) -> float {
+        if (OPK == CELL_A) return ou * lc.w;
         if (OPK == CELL_BWT) {
             float px, py;
             if (h.xdrv) {
@@ -385,102 +475,104 @@ __host__ __device__ inline void cell_body(const ViewHdr *__restrict__ hdr, const
         return ou;
     };
 
-    float4 lnext = lt[0];
-    for (int L = 0; L < nline; ++L) {
-        const float4 lc = lnext;  // {toff, invd, iz, lam}
-        lnext = lt[L + 1 < nline ? L + 1 : L];
-        // Column boundaries in pixel units along the line.
-        const float t0 = fmaf(d0, lc.y, lc.x);
-        const float t1 = fmaf(d1, lc.y, lc.x);
-        if (!(t1 > 0.0f && t0 < fal)) continue;
-        // Row boundaries in voxel units along z at this line's magnification, and the
-        // offsets of their z-planes (entries if BIG, else bytes).
-        unsigned koff[R + 1];
-        float fr[R + 1];
+    // Row boundaries in voxel units along z at this line's magnification.
+    const float fnz = (float)a.nz;
+    int ki[R + 1];
+    float fr[R + 1];
 #pragma unroll
-        for (int q = 0; q <= R; ++q) {
-            const float z = clampf(fmaf(hv[q], lc.z, a.zoff), 0.0f, fnz);
-            koff[q] = mul24((unsigned)(int)z, BIG ? (unsigned)plane : a.plane8);
-            fr[q] = fractf(z);
-        }
-        const float2 *Wl = W + (int64_t)L * nalong;
-        // Entry m of z-plane offset o on this line.
-        auto entry = [&](unsigned o, int m) -> const float2 * {
-            if (BIG) return Wl + ((int64_t)o + m);
-            return reinterpret_cast<const float2 *>(reinterpret_cast<const char *>(Wl) + (o + ((unsigned)m << 3)));
-        };
+    for (int q = 0; q <= R; ++q) {
+        const float z = clampf(fmaf(st.hv[q], lc.z, a.zoff), 0.0f, fnz);
+        ki[q] = (int)z;
+        fr[q] = fractf(z);
+    }
+    const int plane = h.nline * nalong;
+    const float2 *Wl = W + (int64_t)L * nalong;
+    const char *Wb = reinterpret_cast<const char *>(Wl);
+
+    if (SPAN == 2 || SPAN == 3) {
+        const int m0 = t0 > 0.0f ? (int)t0 : 0;
+        // Pixel weights: m0 and m0+1 (two-pixel path: the column spans less than one
+        // pixel), and m0+2 (three-pixel path: less than two).  Entries past the line's end
+        // are weighted by zero.
+        float w0, w1, w2 = 0.0f;
+        w0 = pixel_weight(fminf(t1, (float)(m0 + 1)) - fmaxf(t0, (float)m0), m0);
         if (SPAN == 2) {
-            // The column spans less than one pixel: it overlaps pixel m0 and possibly m0+1.
-            // Both pixels' boundary lookups are issued together.
-            const int m0 = t0 > 0.0f ? (int)t0 : 0;
-            const float w0 = pixel_weight(fminf(t1, (float)(m0 + 1)) - fmaxf(t0, (float)m0), lc.w, L, m0);
-            const float w1 = m0 + 1 < nalong ? pixel_weight(fmaxf(t1 - (float)(m0 + 1), 0.0f), lc.w, L, m0 + 1) : 0.0f;
-            // Entries m0 and m0+1 are adjacent: one 16-byte load per boundary.  When
-            // m0 is the last pixel the second half belongs to the next line (or to the
-            // zeroed pad after the table) and is weighted by w1 = 0.
-            Pair2 e[R + 1];
-#pragma unroll
-            for (int q = 0; q <= R; ++q) e[q] = *reinterpret_cast<const Pair2 *>(entry(koff[q], m0));
-            float J0 = fmaf(fr[0], e[0].y0, e[0].x0);
-            float J1 = fmaf(fr[0], e[0].y1, e[0].x1);
-#pragma unroll
-            for (int q = 0; q < R; ++q) {
-                const float K0 = fmaf(fr[q + 1], e[q + 1].y0, e[q + 1].x0);
-                const float K1 = fmaf(fr[q + 1], e[q + 1].y1, e[q + 1].x1);
-                acc[q] = fmaf(w0, K0 - J0, fmaf(w1, K1 - J1, acc[q]));
-                J0 = K0;
-                J1 = K1;
-            }
-        } else if (SPAN == 3) {
-            // The column spans less than two pixels: pixels m0, m0+1 and possibly m0+2,
-            // read as one 16-byte and one 8-byte load per boundary (entries past the
-            // line's end are weighted by zero, as above).
-            const int m0 = t0 > 0.0f ? (int)t0 : 0;
-            const float w0 = pixel_weight(fminf(t1, (float)(m0 + 1)) - fmaxf(t0, (float)m0), lc.w, L, m0);
-            const float w1 = m0 + 1 < nalong
-                                 ? pixel_weight(fmaxf(fminf(t1, (float)(m0 + 2)) - (float)(m0 + 1), 0.0f), lc.w, L, m0 + 1)
+            w1 = m0 + 1 < nalong ? pixel_weight(fmaxf(t1 - (float)(m0 + 1), 0.0f), m0 + 1) : 0.0f;
+        } else {
+            w1 = m0 + 1 < nalong ? pixel_weight(fmaxf(fminf(t1, (float)(m0 + 2)) - (float)(m0 + 1), 0.0f), m0 + 1)
                                  : 0.0f;
-            const float w2 = m0 + 2 < nalong ? pixel_weight(fmaxf(t1 - (float)(m0 + 2), 0.0f), lc.w, L, m0 + 2) : 0.0f;
-            Pair2 e[R + 1];
-            float2 g[R + 1];
+            w2 = m0 + 2 < nalong ? pixel_weight(fmaxf(t1 - (float)(m0 + 2), 0.0f), m0 + 2) : 0.0f;
+        }
+        // Lookups: entries m0, m0+1 (one 16-byte pair) and m0+2 per row boundary, from the
+        // wave's staged block when they all fall in it, else from the table.
+        Pair2 e[R + 1];
+        float2 g[R + 1];
+        const bool staged = !BIG && lds != nullptr && bx.pitch > 0 && ki[0] >= bx.Z0 && ki[R] < bx.Z0 + bx.NZ &&
+                            m0 + SPAN - 1 < bx.P0 + bx.pitch;
+        if (staged) {
+            const unsigned p8 = (unsigned)bx.pitch * 8u;
+            const unsigned base = (unsigned)(m0 - bx.P0) * 8u - (unsigned)bx.Z0 * p8;
+            const char *Lb = reinterpret_cast<const char *>(lds);
 #pragma unroll
             for (int q = 0; q <= R; ++q) {
-                const float2 *p0 = entry(koff[q], m0);
-                e[q] = *reinterpret_cast<const Pair2 *>(p0);
-                g[q] = p0[2];
-            }
-            float J0 = fmaf(fr[0], e[0].y0, e[0].x0);
-            float J1 = fmaf(fr[0], e[0].y1, e[0].x1);
-            float J2 = fmaf(fr[0], g[0].y, g[0].x);
-#pragma unroll
-            for (int q = 0; q < R; ++q) {
-                const float K0 = fmaf(fr[q + 1], e[q + 1].y0, e[q + 1].x0);
-                const float K1 = fmaf(fr[q + 1], e[q + 1].y1, e[q + 1].x1);
-                const float K2 = fmaf(fr[q + 1], g[q + 1].y, g[q + 1].x);
-                acc[q] = fmaf(w0, K0 - J0, fmaf(w1, K1 - J1, fmaf(w2, K2 - J2, acc[q])));
-                J0 = K0;
-                J1 = K1;
-                J2 = K2;
+                const char *pq = Lb + mad24((unsigned)ki[q], p8, base);
+                e[q] = *reinterpret_cast<const Pair2 *>(pq);
+                if (SPAN == 3) g[q] = *reinterpret_cast<const float2 *>(pq + 16);
             }
         } else {
-            const int m0 = t0 > 0.0f ? (int)floorf(t0) : 0;
-            const int m1 = t1 < fal ? (int)ceilf(t1) : nalong;
-            for (int m = m0; m < m1; ++m) {
-                const float w = pixel_weight(fminf(t1, (float)(m + 1)) - fmaxf(t0, (float)m), lc.w, L, m);
-                float2 e[R + 1];
 #pragma unroll
-                for (int q = 0; q <= R; ++q) e[q] = *entry(koff[q], m);
-                float Jp = fmaf(fr[0], e[0].y, e[0].x);
+            for (int q = 0; q <= R; ++q) {
+                const float2 *pq;
+                if (BIG) pq = Wl + ((int64_t)mul24((unsigned)ki[q], (unsigned)plane) + m0);
+                else pq = reinterpret_cast<const float2 *>(Wb + mad24((unsigned)ki[q], a.plane8, (unsigned)m0 * 8u));
+                e[q] = *reinterpret_cast<const Pair2 *>(pq);
+                if (SPAN == 3) g[q] = pq[2];
+            }
+        }
+        float J0 = fmaf(fr[0], e[0].y0, e[0].x0);
+        float J1 = fmaf(fr[0], e[0].y1, e[0].x1);
+        float J2 = SPAN == 3 ? fmaf(fr[0], g[0].y, g[0].x) : 0.0f;
 #pragma unroll
-                for (int q = 0; q < R; ++q) {
-                    const float Jn = fmaf(fr[q + 1], e[q + 1].y, e[q + 1].x);
-                    acc[q] = fmaf(w, Jn - Jp, acc[q]);
-                    Jp = Jn;
-                }
+        for (int q = 0; q < R; ++q) {
+            const float K0 = fmaf(fr[q + 1], e[q + 1].y0, e[q + 1].x0);
+            const float K1 = fmaf(fr[q + 1], e[q + 1].y1, e[q + 1].x1);
+            float t = fmaf(w1, K1 - J1, st.acc[q]);
+            if (SPAN == 3) {
+                const float K2 = fmaf(fr[q + 1], g[q + 1].y, g[q + 1].x);
+                t = fmaf(w2, K2 - J2, t);
+                J2 = K2;
+            }
+            st.acc[q] = fmaf(w0, K0 - J0, t);
+            J0 = K0;
+            J1 = K1;
+        }
+    } else {
+        const int m0 = t0 > 0.0f ? (int)floorf(t0) : 0;
+        const int m1 = t1 < fal ? (int)ceilf(t1) : nalong;
+        for (int m = m0; m < m1; ++m) {
+            const float w = pixel_weight(fminf(t1, (float)(m + 1)) - fmaxf(t0, (float)m), m);
+            float2 e[R + 1];
+#pragma unroll
+            for (int q = 0; q <= R; ++q) {
+                if (BIG) e[q] = Wl[(int64_t)mul24((unsigned)ki[q], (unsigned)plane) + m];
+                else e[q] = *reinterpret_cast<const float2 *>(Wb + mad24((unsigned)ki[q], a.plane8, (unsigned)m * 8u));
+            }
+            float Jp = fmaf(fr[0], e[0].y, e[0].x);
+#pragma unroll
+            for (int q = 0; q < R; ++q) {
+                const float Jn = fmaf(fr[q + 1], e[q + 1].y, e[q + 1].x);
+                st.acc[q] = fmaf(w, Jn - Jp, st.acc[q]);
+                Jp = Jn;
             }
         }
     }
+}
 
+// Writes the thread's R detector cells (output column order restored).
+template <int OPK, int GEO, int R>
+__host__ __device__ inline void cell_store(const CellLane<R> &st, const ViewHdr &h, const CellArgs &a,
+                                           float *__restrict__ out, int c, int r0, int v, int b)
+{
     const int col = h.keep ? c : (a.nu - 1 - c);
     float *o = out + ((int64_t)b * a.nview + v) * a.nv * a.nu + col;
     for (int q = 0; q < R; ++q) {
@@ -489,18 +581,75 @@ __host__ __device__ inline void cell_body(const ViewHdr *__restrict__ hdr, const
         float s;
         if (OPK == CELL_A) {
             // Delta / (W_c H_cr cos theta_cr) with H = 1/kappa (src/cpu/forward3d.cpp:330-344)
-            s = h.delta * cc.z / cc.w;
+            s = h.delta * st.cc.z / st.cc.w;
             if (GEO != GEO_PAR) {
-                const float dbp = 0.5f * (d0 + d1) - h.P;
-                const float hb = cc.y * (a.v0 + ((float)r + 0.5f) * a.dvv);
+                const float dbp = 0.5f * (st.d0 + st.d1) - h.P;
+                const float hb = st.cc.y * (a.v0 + ((float)r + 0.5f) * a.dvv);
                 const float cosw = fabsf(h.S) / sqrtf(dbp * dbp + hb * hb + h.S * h.S);
                 s /= cosw;
             }
         } else {
             s = h.colw;
         }
-        o[(int64_t)r * a.nu] = acc[q] * s;
+        o[(int64_t)r * a.nu] = st.acc[q] * s;
     }
+}
+
+__host__ __device__ inline const float2 *cell_table(const float2 *WX, const float2 *WY, const ViewHdr &h,
+                                                    const CellArgs &a, int b)
+{
+    const int64_t plane = (int64_t)h.nline * h.nalong;
+    return (h.xdrv ? WX : WY) + (int64_t)b * (int64_t)(a.nz + 1) * plane;
+}
+
+// One thread, reading the tables in HBM only: internal column c, rows r0..r0+R-1 of view
+// v, batch b.  SPAN / BIG as for cell_line.
+template <int OPK, int GEO, int R, int SPAN, bool BIG>
+__host__ __device__ inline void cell_body(const ViewHdr *__restrict__ hdr, const float4 *__restrict__ lineC,
+                                          const float4 *__restrict__ colt, const float2 *__restrict__ WX,
+                                          const float2 *__restrict__ WY, float *__restrict__ out, const CellArgs &a,
+                                          int c, int r0, int v, int b)
+{
+    const ViewHdr h = hdr[v];
+    CellLane<R> st;
+    cell_init<R>(st, colt + (int64_t)v * (a.nu + 1), a, c, r0);
+    const float2 *W = cell_table(WX, WY, h, a, b);
+    const float4 *lt = lineC + (int64_t)v * a.maxline;
+    const CellBox none{0, 0, 0, 0};
+    for (int L = 0; L < h.nline; ++L) cell_line<OPK, GEO, R, SPAN, BIG>(st, h, a, lt[L], L, W, none, nullptr);
+    cell_store<OPK, GEO, R>(st, h, a, out, c, r0, v, b);
+}
+
+// A wave of the cell gather with per-line LDS staging (device only): columns
+// cg*64 .. cg*64+63 (this lane: c, valid when `lane_ok`), rows r0..r0+R-1 of view v,
+// batch b, fixed span SPAN (2 or 3).  Per line the wave copies the block of the z-prefix
+// table its lanes read into its own LDS block `lds` (no other wave touches it, so no
+// barrier: the wave waits for its own loads), then every lane gathers from it.
+template <int OPK, int GEO, int R, int SPAN>
+__device__ inline void cell_wave(const ViewHdr *__restrict__ hdr, const float4 *__restrict__ lineC,
+                                 const float4 *__restrict__ colt, const float2 *__restrict__ csig,
+                                 const float2 *__restrict__ WX, const float2 *__restrict__ WY,
+                                 float *__restrict__ out, const CellArgs &a, int c, bool lane_ok, int cg, int r0,
+                                 int v, int b, float2 *__restrict__ lds)
+{
+    const ViewHdr h = hdr[v];
+    const float4 *ct = colt + (int64_t)v * (a.nu + 1);
+    CellLane<R> st;
+    cell_init<R>(st, ct, a, lane_ok ? c : cg * kWave, r0);
+    const float2 *W = cell_table(WX, WY, h, a, b);
+    const float4 *lt = lineC + (int64_t)v * a.maxline;
+    const float dfirst = ct[cg * kWave].x, dlast = ct[imin(a.nu, cg * kWave + kWave)].x;
+    const float2 sg = csig[(int64_t)v * ceil_div(a.nu, kWave) + cg];
+    const int lane = threadIdx.x & (kWave - 1);
+    for (int L = 0; L < h.nline; ++L) {
+        const float4 lc = lt[L];
+        const CellBox bx = cell_box<R, SPAN>(a, lc, dfirst, dlast, sg.x, sg.y, r0, h.nalong);
+        if (bx.pitch == 0) continue;  // no column of the wave meets this line
+        cell_stage(W + (int64_t)L * h.nalong, bx, lds, a.plane8, lane);
+        __builtin_amdgcn_s_waitcnt(0);  // this wave's direct-to-LDS loads have landed
+        if (lane_ok) cell_line<OPK, GEO, R, SPAN, false>(st, h, a, lc, L, W, bx, lds);
+    }
+    if (lane_ok) cell_store<OPK, GEO, R>(st, h, a, out, c, r0, v, b);
 }
 
 // ---------------------------------------------------------------- voxel gather

@@ -91,20 +91,26 @@ __global__ __launch_bounds__(kBlock) void k_sino_prefix(const ViewHdr *__restric
 
 template <int OPK, int GEO, int R, bool BIG>
 __global__ __launch_bounds__(kBlock) void k_cell(const ViewHdr *__restrict__ hdr, const float4 *__restrict__ lineC,
-                                                 const float4 *__restrict__ colt, const float2 *__restrict__ WX,
-                                                 const float2 *__restrict__ WY, float *__restrict__ out, CellArgs a)
+                                                 const float4 *__restrict__ colt, const float2 *__restrict__ csig,
+                                                 const float2 *__restrict__ WX, const float2 *__restrict__ WY,
+                                                 float *__restrict__ out, CellArgs a)
 {
+    __shared__ __attribute__((aligned(16))) float2 lds[kWavesPerBlock][cell_lds_cap<R>()];
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = threadIdx.x / kWave;
     const int c = blockIdx.x * kWave + lane;
     const int r0 = (blockIdx.y * kWavesPerBlock + wave) * R;
     const int v = blockIdx.z % a.nview;
     const int b = blockIdx.z / a.nview;
-    if (c >= a.nu || r0 >= a.nv) return;
+    if (r0 >= a.nv) return;  // the whole wave
+    // the per-view variant choice is uniform over the workgroup (one view per workgroup)
     const int span = BIG ? 0 : hdr[v].span;
-    if (span == 2) cell_body<OPK, GEO, R, 2, false>(hdr, lineC, colt, WX, WY, out, a, c, r0, v, b);
-    else if (span == 3) cell_body<OPK, GEO, R, 3, false>(hdr, lineC, colt, WX, WY, out, a, c, r0, v, b);
-    else cell_body<OPK, GEO, R, 0, BIG>(hdr, lineC, colt, WX, WY, out, a, c, r0, v, b);
+    if (span == 2)
+        cell_wave<OPK, GEO, R, 2>(hdr, lineC, colt, csig, WX, WY, out, a, c, c < a.nu, blockIdx.x, r0, v, b, lds[wave]);
+    else if (span == 3)
+        cell_wave<OPK, GEO, R, 3>(hdr, lineC, colt, csig, WX, WY, out, a, c, c < a.nu, blockIdx.x, r0, v, b, lds[wave]);
+    else if (c < a.nu)
+        cell_body<OPK, GEO, R, 0, BIG>(hdr, lineC, colt, WX, WY, out, a, c, r0, v, b);
 }
 
 template <int OPK, int GEO, int KC>
@@ -144,7 +150,8 @@ void launch_cell(const Plan3d &p, int R, const float2 *WX, const float2 *WY, flo
     if (p.big) R = 8;  // tables of 4 GiB and more: one 64-bit general variant
     dim3 grid(ceil_div(a.nu, kWave), ceil_div(a.nv, R * kWavesPerBlock), nbatch * p.nview);
 #define XR_CELL(RR, BB) \
-    hipLaunchKernelGGL((k_cell<OPK, GEO, RR, BB>), grid, dim3(kBlock), 0, s, p.d_hdr, p.d_lineC, p.d_col, WX, WY, out, a)
+    hipLaunchKernelGGL((k_cell<OPK, GEO, RR, BB>), grid, dim3(kBlock), 0, s, p.d_hdr, p.d_lineC, p.d_col, p.d_csig, WX, \
+                       WY, out, a)
     if (p.big) XR_CELL(8, true);
     else {
         switch (R) {

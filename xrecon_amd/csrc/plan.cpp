@@ -262,6 +262,20 @@ int build_tables(Plan3d &p, const float *angles, PlanTables &t, std::string &err
         p.vsplit = std::max(1, std::min(sp, V));
     }
 
+    // Cell-gather staging: depth-scale range of each wave's 64 columns per view.
+    const int ncg = ceil_div(nu, kWave);
+    t.csig.assign((size_t)V * ncg, float2{0.0f, 0.0f});
+    for (int v = 0; v < V; ++v)
+        for (int g = 0; g < ncg; ++g) {
+            float lo = 3.0e38f, hi = -3.0e38f;
+            for (int c = g * kWave; c < std::min(nu, (g + 1) * kWave); ++c) {
+                const float sg = t.col[(size_t)v * (nu + 1) + c].y;
+                lo = std::min(lo, sg);
+                hi = std::max(hi, sg);
+            }
+            t.csig[(size_t)v * ncg + g] = float2{lo, hi};
+        }
+
     // Voxel-gather staging estimates per (view, 16 x 16 tile of voxel columns).
     t.vtile.assign((size_t)V * gx * gy, VoxTile{0, 0, 0.0f, 0.0f});
     for (int v = 0; v < V; ++v) {

@@ -78,6 +78,7 @@ struct Plan3d {
     float4 *d_lineV = nullptr;  // [nview][maxline] {P0, Delta, zeta, 1/(Delta zeta)}: voxel kernels
     float4 *d_col = nullptr;    // [nview][Nu+1]    {d, sigma, kappa, W}
     VoxTile *d_vtile = nullptr; // [nview][ceil(Ny/16)][ceil(Nx/16)]: voxel-gather staging estimates
+    float2 *d_csig = nullptr;   // [nview][ceil(Nu/64)] {min, max} of sigma over each 64-column group
 };
 
 // Host-side tables before upload (kept separate so tests can inspect the builder).
@@ -85,6 +86,7 @@ struct PlanTables {
     std::vector<ViewHdr> hdr;
     std::vector<float4> lineC, lineV, col;
     std::vector<VoxTile> vtile;
+    std::vector<float2> csig;
 };
 
 // plan.cpp
