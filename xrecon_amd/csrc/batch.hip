@@ -129,7 +129,9 @@ template <int OPK, int GEO, int R, bool BIG>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_cell_batch(const BatchDesc *__restrict__ D, int n, IOPtrs io,
                                                        char *__restrict__ ws)
 {
+#if XR_CELL_STAGED
     __shared__ __attribute__((aligned(16))) float2 lds[kWavesPerBlock][cell_lds_cap<R>()];
+#endif
     int bid = blockIdx.x;
     const int i = find_instance(D, n, bid, [](const BatchDesc &d) { return blocks_cell(d, R); });
     if (i < 0) return;
@@ -150,11 +152,19 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
     if (BIG) {
         if (c < a.nu) cell_body<OPK, GEO, R, 0, true>(d.hdr, d.lineC, d.col, WX, WY, io.out[i], a, c, r0, v, b);
     } else if (span == 2) {
+#if XR_CELL_STAGED
         cell_wave<OPK, GEO, R, 2>(d.hdr, d.lineC, d.col, d.csig, WX, WY, io.out[i], a, c, c < a.nu, bx, r0, v, b,
                                   lds[wave]);
+#else
+        if (c < a.nu) cell_body<OPK, GEO, R, 2, false>(d.hdr, d.lineC, d.col, WX, WY, io.out[i], a, c, r0, v, b);
+#endif
     } else if (span == 3) {
+#if XR_CELL_STAGED
         cell_wave<OPK, GEO, R, 3>(d.hdr, d.lineC, d.col, d.csig, WX, WY, io.out[i], a, c, c < a.nu, bx, r0, v, b,
                                   lds[wave]);
+#else
+        if (c < a.nu) cell_body<OPK, GEO, R, 3, false>(d.hdr, d.lineC, d.col, WX, WY, io.out[i], a, c, r0, v, b);
+#endif
     } else if (c < a.nu) {
         cell_body<OPK, GEO, R, 0, false>(d.hdr, d.lineC, d.col, WX, WY, io.out[i], a, c, r0, v, b);
     }

@@ -620,6 +620,14 @@ __host__ __device__ inline void cell_body(const ViewHdr *__restrict__ hdr, const
     cell_store<OPK, GEO, R>(st, h, a, out, c, r0, v, b);
 }
 
+// Whether the cell kernels stage each line's block in LDS per wave (cell_wave) or read
+// the table directly (cell_body).  Staging as written waits for each line's copy before
+// gathering it, which exposes one memory round trip per line and measured 2.7x slower
+// (10.9 ms against 4.0 ms for cfg2 A); it stays here, tested through the emulator, off.
+#ifndef XR_CELL_STAGED
+#define XR_CELL_STAGED 0
+#endif
+
 // A wave of the cell gather with per-line LDS staging (device only): columns
 // cg*64 .. cg*64+63 (this lane: c, valid when `lane_ok`), rows r0..r0+R-1 of view v,
 // batch b, fixed span SPAN (2 or 3).  Per line the wave copies the block of the z-prefix

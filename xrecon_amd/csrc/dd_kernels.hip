@@ -95,7 +95,9 @@ __global__ __launch_bounds__(kBlock) void k_cell(const ViewHdr *__restrict__ hdr
                                                  const float2 *__restrict__ WX, const float2 *__restrict__ WY,
                                                  float *__restrict__ out, CellArgs a)
 {
+#if XR_CELL_STAGED
     __shared__ __attribute__((aligned(16))) float2 lds[kWavesPerBlock][cell_lds_cap<R>()];
+#endif
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = threadIdx.x / kWave;
     const int c = blockIdx.x * kWave + lane;
@@ -105,12 +107,20 @@ __global__ __launch_bounds__(kBlock) void k_cell(const ViewHdr *__restrict__ hdr
     if (r0 >= a.nv) return;  // the whole wave
     // the per-view variant choice is uniform over the workgroup (one view per workgroup)
     const int span = BIG ? 0 : hdr[v].span;
+#if XR_CELL_STAGED
     if (span == 2)
         cell_wave<OPK, GEO, R, 2>(hdr, lineC, colt, csig, WX, WY, out, a, c, c < a.nu, blockIdx.x, r0, v, b, lds[wave]);
     else if (span == 3)
         cell_wave<OPK, GEO, R, 3>(hdr, lineC, colt, csig, WX, WY, out, a, c, c < a.nu, blockIdx.x, r0, v, b, lds[wave]);
     else if (c < a.nu)
         cell_body<OPK, GEO, R, 0, BIG>(hdr, lineC, colt, WX, WY, out, a, c, r0, v, b);
+#else
+    (void)csig;
+    if (c >= a.nu) return;
+    if (span == 2) cell_body<OPK, GEO, R, 2, false>(hdr, lineC, colt, WX, WY, out, a, c, r0, v, b);
+    else if (span == 3) cell_body<OPK, GEO, R, 3, false>(hdr, lineC, colt, WX, WY, out, a, c, r0, v, b);
+    else cell_body<OPK, GEO, R, 0, BIG>(hdr, lineC, colt, WX, WY, out, a, c, r0, v, b);
+#endif
 }
 
 template <int OPK, int GEO, int KC>
