@@ -839,22 +839,11 @@ __host__ __device__ inline void voxel_view(const ViewHdr &h, const float4 *__res
                 // loads back to back, then consume.
                 int ri[KC + 1];
                 float fr[KC + 1];
-                // rho increases with q: when its ends lie on the detector no boundary needs
-                // the clamp (the common case away from the top and bottom rows)
-                if (rlo >= 0.0f && fmaf((float)KC, step, rlo) <= fnv) {
 #pragma unroll
-                    for (int q = 0; q <= KC; ++q) {
-                        const float rho = fmaf((float)q, step, rlo);
-                        ri[q] = (int)rho;
-                        fr[q] = fractf(rho);
-                    }
-                } else {
-#pragma unroll
-                    for (int q = 0; q <= KC; ++q) {
-                        const float rho = clampf(fmaf((float)q, step, rlo), 0.0f, fnv);
-                        ri[q] = (int)rho;
-                        fr[q] = fractf(rho);
-                    }
+                for (int q = 0; q <= KC; ++q) {
+                    const float rho = clampf(fmaf((float)q, step, rlo), 0.0f, fnv);
+                    ri[q] = (int)rho;
+                    fr[q] = fractf(rho);
                 }
                 float2 e[KC + 1];
                 // rows increase with q, so the first and last lookups bound the column's rows
