@@ -373,6 +373,8 @@ def main():
     plan = proj.plan(angles, dev)
     stream = torch.cuda.current_stream(dev)
     sid = stream.cuda_stream
+    proj = proj.to(dev)
+    angles_dev = angles.to(dev)
 
     A, B, AT = be.OP_PROJECTION, be.OP_BACKPROJECTION, be.OP_PROJECTION_T
     sino = torch.empty((1, 1, wl["V"], wl["Nv"], wl["Nu"]), device=dev)
@@ -448,6 +450,9 @@ def main():
     if not args.no_cpu:
         cpu = cpu_baseline(list(proj._host_options), phantom, angles.numpy(), args.cpu_views)
 
+    # FDK through the public API on the same sinogram (weights, rFFT ramp filter, FDK-weighted
+    # backprojection, scaling)
+    fdk_ms = _time_gpu(lambda: proj.filtered_backprojection(sino, angles_dev), max(3, K // 2), stream)
     batched = None if args.no_batched else bench_batched(dev, max(2, K // 2), args.verbose)
     configs = None
     if not args.no_configs:
@@ -487,6 +492,8 @@ def main():
                        "prefix_passes": ms_step - t_cell - t_vox},
         "adjoint": {"ms_per_pair": ms_adj, "value": vox_views / (ms_adj * 1e-3), "unit": "voxel-views/s",
                     "pair": "A + A^T"},
+        "fdk": {"ms": fdk_ms, "value": vox_views / (fdk_ms * 1e-3), "unit": "voxel-views/s",
+                "path": "Projector3d.filtered_backprojection (API, includes the filter stage)"},
         "cpu_baseline": cpu,
         "batched": batched,
         "configs": configs,
