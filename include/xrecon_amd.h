@@ -119,6 +119,19 @@ int xr_plan3d_execute_phase(const xr_plan3d *plan, int op, int phase, const floa
                             int num_batch, void *workspace, size_t workspace_bytes, int tile_rows,
                             int voxel_chunk, void *stream);
 
+/* FDK / FBP in one pass over the sinogram: replaces, for inference, the Python-level
+ * chain of Projector3d.filtered_backprojection (xrecon/projector/projector3d.py:183-195;
+ * 2D: projector.py:195-206) -- cosine weighting, the zero-padded row correlation with the
+ * 2*Nu-1 tap filter, the FDK-weighted backprojection -- for detectors of at most 1024
+ * columns.  `filter` (2*Nu-1 taps, already multiplied by the mean angular step and by 1/2
+ * for a full scan) and `weight` (Nu column weights) are device pointers in physical
+ * column order.  The filter stage accumulates in FP64 and writes the row-prefix table of
+ * the filtered sinogram directly; the workspace is that of XR_OP_BACKPROJECTION_W.
+ * Not a reference entry point: the reference composes these steps with torch ops. */
+int xr_plan3d_execute_fdk(const xr_plan3d *plan, const float *in, float *out, int num_batch, const float *filter,
+                          const float *weight, void *workspace, size_t workspace_bytes, int voxel_chunk,
+                          void *stream);
+
 /* Which gather family runs `op` on this plan: XR_KERNEL_CELL (detector-cell gather,
  * tuned by tile_rows) or XR_KERNEL_VOXEL (voxel-column gather, tuned by voxel_chunk);
  * negative for an unknown operator. */

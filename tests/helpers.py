@@ -55,6 +55,19 @@ def shapes3d(opts, nb, V):
     return (nb, D, H, W), (nb, V, Nv, Nu)
 
 
+def correlate_fp64(sino, filt):
+    """out[..., u] = sum_s sino[..., s] * filt[s - u + Nu - 1] with zero padding, in FP64:
+    the reference's conv over the detector axis with padding Nu - 1 (projector3d.py:190,
+    projector.py:201), written directly."""
+    sino = np.asarray(sino, dtype=np.float64)
+    filt = np.asarray(filt, dtype=np.float64).ravel()
+    nu = sino.shape[-1]
+    out = np.zeros(sino.shape, dtype=np.float64)
+    for u in range(nu):
+        out[..., u] = sino @ filt[nu - 1 - u: 2 * nu - 1 - u]
+    return out
+
+
 def rel_l2(a, b) -> float:
     a = np.asarray(a, dtype=np.float64).ravel()
     b = np.asarray(b, dtype=np.float64).ravel()
@@ -104,6 +117,27 @@ def emu_op3d(op, x, angles, opts, R=0, KC=0, is2d=False, nthreads=None, cell_var
     out = np.zeros(sino if op in VOLUME_IN else vol, dtype=np.float32)
     rc = lib.emu_op3d(OP_ID[op], _p(vn), _p(vf), _p(gn), _p(gf), _p(angles), angles.shape[0], _p(x), _p(out), nb,
                       R, KC, nthreads or min(8, os.cpu_count() or 1))
+    if rc != 0:
+        raise RuntimeError(lib.emu_error().decode())
+    return out
+
+
+def emu_fdk3d(y, angles, opts, filt, weight, is2d=False, KC=0, nthreads=None):
+    """The fused FDK pass on the CPU: sinogram [nb, V, Nv, Nu] -> volume [nb, D, H, W]."""
+    lib = _load_emu()
+    vn, vf, gn, gf = oracle.split3d(opts)
+    if is2d:
+        gn = gn.copy()
+        gn[0] |= 0x100
+    y = np.ascontiguousarray(y, dtype=np.float32)
+    angles = np.ascontiguousarray(angles, dtype=np.float32)
+    filt = np.ascontiguousarray(filt, dtype=np.float32).ravel()
+    weight = np.ascontiguousarray(weight, dtype=np.float32).ravel()
+    nb = y.shape[0]
+    vol, _ = shapes3d(opts, nb, angles.shape[0])
+    out = np.zeros(vol, dtype=np.float32)
+    rc = lib.emu_fdk3d(_p(vn), _p(vf), _p(gn), _p(gf), _p(angles), angles.shape[0], _p(y), _p(filt), _p(weight),
+                       _p(out), nb, KC, nthreads or min(8, os.cpu_count() or 1))
     if rc != 0:
         raise RuntimeError(lib.emu_error().decode())
     return out

@@ -226,6 +226,59 @@ void dispatch(const Plan3d &p, const Route &rt, int R, int KC, const float *in, 
 
 std::string g_err;
 
+// The fused FDK pass on the CPU: the filter stage as fdk_filter_output (the device
+// kernel's order), the in-place prefix, then the voxel gather of the weighted route.
+template <int GEO>
+void fdk_dispatch(const Plan3d &p, int KC, const float *in, const float *filt, const float *wgt, float *out, int nb,
+                  int nt)
+{
+    const Route rt = route(p, XR_OP_BACKPROJECTION_W);
+    const SinoArgs sa = make_sino_args(p, nb);
+    const int nu = p.geo.nu, nv = p.geo.nv;
+    std::vector<float2> Q((size_t)nb * p.nview * (nv + 1) * nu + kSinoPad);
+    for (int b = 0; b < nb; ++b)
+        for (int v = 0; v < p.nview; ++v) {
+            const int64_t sv = (int64_t)b * p.nview + v;
+            for (int r = 0; r < nv; ++r) {
+                const float *row = in + (sv * nv + r) * nu;
+                for (int u = 0; u < nu; ++u) {
+                    const int c = p.d_hdr[v].keep ? u : nu - 1 - u;
+                    Q[(sv * (nv + 1) + r) * nu + c] = float2{0.0f, (float)fdk_filter_output(row, wgt, filt, nu, u)};
+                }
+            }
+            for (int c = 0; c < nu; ++c) {
+                if (rt.kop == VOX_AT) fdk_prefix_body<SINO_AT, GEO>(p.d_hdr, p.d_col, Q.data(), sa, b, v, c);
+                else fdk_prefix_body<SINO_PLAIN, GEO>(p.d_hdr, p.d_col, Q.data(), sa, b, v, c);
+            }
+        }
+    if (rt.kop == VOX_B) run_voxel<VOX_B, GEO>(p, KC, Q.data(), out, nb, nt);
+    else if (rt.kop == VOX_BW) run_voxel<VOX_BW, GEO>(p, KC, Q.data(), out, nb, nt);
+    else run_voxel<VOX_AT, GEO>(p, KC, Q.data(), out, nb, nt);
+}
+
+int make_plan(Plan3d &p, PlanTables &t, const int *vn, const float *vf, const int *gn, const float *gf,
+              const float *angles, int nview)
+{
+    p.vol = xr_volume3d{vn[0], vn[1], vn[2], vf[0], vf[1], vf[2], vf[3], vf[4], vf[5]};
+    p.geo = xr_geometry3d{gn[0] & 0xff, gn[1], gn[2], gf[0], gf[1], gf[2], gf[3], gf[4], gf[5]};
+    p.is2d = (gn[0] & 0x100) ? 1 : 0;
+    p.kind = p.geo.type;
+    p.nview = nview;
+    std::string err;
+    const int rc = build_tables(p, angles, t, err);
+    if (rc) {
+        g_err = err;
+        return rc;
+    }
+    p.d_hdr = t.hdr.data();
+    p.d_lineC = t.lineC.data();
+    p.d_lineV = t.lineV.data();
+    p.d_col = t.col.data();
+    p.d_vtile = t.vtile.data();
+    p.d_csig = t.csig.data();
+    return XR_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -235,6 +288,25 @@ const char *emu_error(void) { return g_err.c_str(); }
 void emu_set_cell_variant(int v) { g_cell_variant = v; }
 
 void emu_set_voxel_variant(int v) { g_voxel_variant = v; }
+
+// The fused FDK pass (xr_plan3d_execute_fdk) on the CPU: `in` [nb, V, Nv, Nu], filter taps
+// (scaled, 2 Nu - 1), column weights (Nu), `out` [nb, Nz, Ny, Nx].
+int emu_fdk3d(const int *vn, const float *vf, const int *gn, const float *gf, const float *angles, int nview,
+              const float *in, const float *filt, const float *wgt, float *out, int nbatch, int voxel_chunk,
+              int nthreads)
+{
+    Plan3d p;
+    PlanTables t;
+    const int rc = make_plan(p, t, vn, vf, gn, gf, angles, nview);
+    if (rc) return rc;
+    int KC = voxel_chunk;
+    if (KC == 0) KC = p.vol.nz >= 64 ? 16 : p.vol.nz >= 8 ? 8 : p.vol.nz >= 4 ? 4 : 1;
+    if (nthreads < 1) nthreads = 1;
+    if (p.kind == GEO_FLAT) fdk_dispatch<GEO_FLAT>(p, KC, in, filt, wgt, out, nbatch, nthreads);
+    else if (p.kind == GEO_ARC) fdk_dispatch<GEO_ARC>(p, KC, in, filt, wgt, out, nbatch, nthreads);
+    else fdk_dispatch<GEO_PAR>(p, KC, in, filt, wgt, out, nbatch, nthreads);
+    return XR_OK;
+}
 
 // Per-view span class of the cell gather (ViewHdr::span) for a geometry.
 int emu_view_spans(const int *vn, const float *vf, const int *gn, const float *gf, const float *angles, int nview,
@@ -264,24 +336,9 @@ int emu_op3d(int op, const int *vn, const float *vf, const int *gn, const float 
              const float *in, float *out, int nbatch, int tile_rows, int voxel_chunk, int nthreads)
 {
     Plan3d p;
-    p.vol = xr_volume3d{vn[0], vn[1], vn[2], vf[0], vf[1], vf[2], vf[3], vf[4], vf[5]};
-    p.geo = xr_geometry3d{gn[0] & 0xff, gn[1], gn[2], gf[0], gf[1], gf[2], gf[3], gf[4], gf[5]};
-    p.is2d = (gn[0] & 0x100) ? 1 : 0;
-    p.kind = p.geo.type;
-    p.nview = nview;
     PlanTables t;
-    std::string err;
-    int rc = build_tables(p, angles, t, err);
-    if (rc) {
-        g_err = err;
-        return rc;
-    }
-    p.d_hdr = t.hdr.data();
-    p.d_lineC = t.lineC.data();
-    p.d_lineV = t.lineV.data();
-    p.d_col = t.col.data();
-    p.d_vtile = t.vtile.data();
-    p.d_csig = t.csig.data();
+    const int rc = make_plan(p, t, vn, vf, gn, gf, angles, nview);
+    if (rc) return rc;
     const Route rt = route(p, op);
     if (rt.kop < 0) {
         g_err = "unknown op";

@@ -6,8 +6,8 @@ import numpy as np
 import pytest
 
 import oracle
-from helpers import CASES3D, OPS, VOLUME_IN, angles_with_ties, emu_available, emu_op3d, emu_view_spans, opts2d, \
-    opts2d_as_3d, opts3d, rel_l2, shapes3d
+from helpers import CASES3D, OPS, VOLUME_IN, angles_with_ties, correlate_fp64, emu_available, emu_fdk3d, emu_op3d, \
+    emu_view_spans, opts2d, opts2d_as_3d, opts3d, rel_l2, shapes3d
 
 pytestmark = pytest.mark.skipif(not emu_available(), reason="emulator not built (run __graft_entry__.build())")
 
@@ -148,3 +148,40 @@ def test_voxel_gather_staged_and_direct_agree(geo):
         direct = emu_op3d(op, y, ang, opts, voxel_variant=1)
         assert rel_l2(staged, want) <= 1e-5
         assert np.array_equal(staged, direct)
+
+
+def _fdk_inputs(nu, seed):
+    from xrecon_amd.filters import filter_generate
+
+    rng = np.random.default_rng(seed)
+    # an asymmetric kernel, so that a reversed tap index or column order shows
+    filt = (filter_generate(nu, 1.0, "shepplogan").numpy() * 0.37 + 0.05 * rng.random(2 * nu - 1)).astype(np.float32)
+    weight = (0.5 + rng.random(nu)).astype(np.float32)
+    return rng, filt, weight
+
+
+@pytest.mark.parametrize("geo", sorted(CASES3D))
+def test_fused_fdk_vs_fp64_pipeline(geo):
+    """The fused FDK pass (filter stage, in-place prefix, weighted voxel gather) against
+    weight -> FP64 correlation -> FP64 oracle B_W."""
+    opts = CASES3D[geo]
+    V = 24
+    ang = angles_with_ties(V)
+    _, sino = shapes3d(opts, 2, V)
+    rng, filt, weight = _fdk_inputs(sino[-1], 31)
+    y = rng.random(sino).astype(np.float32)
+    got = emu_fdk3d(y, ang, opts, filt, weight)
+    want = oracle.op3d("BW", correlate_fp64(y.astype(np.float64) * weight, filt), ang, opts)
+    assert rel_l2(got, want) <= 1e-5
+
+
+@pytest.mark.parametrize("geo", sorted(CASES2D))
+def test_fused_fbp_2d_vs_fp64_pipeline(geo):
+    o2 = CASES2D[geo]
+    V = 30
+    ang = angles_with_ties(V)
+    rng, filt, weight = _fdk_inputs(45, 32)
+    y = rng.random((2, V, 45)).astype(np.float32)
+    got = emu_fdk3d(y[:, :, None], ang, opts2d_as_3d(o2), filt, weight, is2d=True).reshape(2, 17, 33)
+    want = oracle.op2d("BW", correlate_fp64(y.astype(np.float64) * weight, filt), ang, o2)
+    assert rel_l2(got, want) <= 1e-5

@@ -4,6 +4,10 @@
   pipeline built from the projector's own weight and filter buffers, a direct FP64
   correlation (the reference's zero-padded conv, projector3d.py:190 / projector.py:201)
   and the FP64 oracle's weighted backprojection: relative L2 <= 1e-5.
+* The fused FDK pass (xr_plan3d_execute_fdk: filter stage, in-place prefix, weighted
+  voxel gather) on every geometry against the FP64 pipeline and the CPU emulator of the
+  same kernel bodies, and the projector's fused and differentiable composite paths
+  against each other.
 * Config 1 (2D fan-flat 256^2, 256 detectors, 720 views) and config 5 (cone-arc
   512^2 x 256 at 0.7 mm, 736 x 128 detector, 46.6 degree fan) at full size: A on sampled
   views and B, B_W on a subset of views against the FP64 oracle, relative L2 <= 1e-5.
@@ -15,18 +19,10 @@ import torch
 
 import oracle
 import xrecon_amd
-from helpers import rel_l2, uniform_angles
+from helpers import CASES3D, angles_with_ties, correlate_fp64, emu_available, emu_fdk3d, rel_l2, shapes3d, \
+    uniform_angles
 
 pytestmark = pytest.mark.gpu
-
-
-def _correlate_fp64(sino, filt):
-    """out[..., u] = sum_s sino[..., s] * filt[s - u + Nu - 1] (zero padding), FP64."""
-    nu = sino.shape[-1]
-    out = np.zeros(sino.shape, dtype=np.float64)
-    for u in range(nu):
-        out[..., u] = sino @ filt[nu - 1 - u: 2 * nu - 1 - u]
-    return out
 
 
 def _mean_step(ang):
@@ -43,9 +39,49 @@ def test_fdk_pipeline_vs_fp64(cuda):
     got = proj.filtered_backprojection(torch.from_numpy(y).to(cuda), torch.from_numpy(ang).to(cuda))
     w = proj.weight.detach().cpu().double().numpy().reshape(-1)
     h = proj.filter.detach().cpu().double().numpy().reshape(-1)
-    filtered = _correlate_fp64(y[0, 0].astype(np.float64) * w, h)
+    filtered = correlate_fp64(y[0, 0].astype(np.float64) * w, h)
     want = oracle.op3d("BW", filtered[None], ang, list(proj._host_options)) * _mean_step(ang) / 2
     assert rel_l2(got.cpu().numpy()[0, 0], want[0]) <= 1e-5
+
+
+@pytest.mark.parametrize("geo", sorted(CASES3D))
+def test_fused_fdk_vs_fp64_and_emulator(cuda, geo):
+    from xrecon_amd import _backend as be
+    from xrecon_amd.filters import filter_generate
+
+    opts = CASES3D[geo]
+    V = 40
+    ang = angles_with_ties(V)
+    _, sino = shapes3d(opts, 2, V)
+    nu = sino[-1]
+    rng = np.random.default_rng(41)
+    filt = (filter_generate(nu, 1.0, "shepplogan").numpy() * 0.37 + 0.05 * rng.random(2 * nu - 1)).astype(np.float32)
+    weight = (0.5 + rng.random(nu)).astype(np.float32)
+    y = rng.random(sino).astype(np.float32)
+    plan = be.native().Plan([float(v) for v in opts], torch.from_numpy(ang), 0, False)
+    print(f"[gpu] fused FDK {geo} shape={y.shape}", flush=True)
+    got = plan.fdk(torch.from_numpy(y).to(cuda).unsqueeze(1), torch.from_numpy(filt).to(cuda),
+                   torch.from_numpy(weight).to(cuda), torch.cuda.current_stream().cuda_stream)
+    got = got.cpu().numpy()[:, 0]
+    want = oracle.op3d("BW", correlate_fp64(y.astype(np.float64) * weight, filt), ang, opts)
+    assert rel_l2(got, want) <= 1e-5
+    if emu_available():
+        assert rel_l2(got, emu_fdk3d(y, ang, opts, filt, weight)) <= 1e-6
+
+
+@pytest.mark.parametrize("scan", ["flat", "arc"])
+def test_fused_fdk_matches_differentiable_path(cuda, scan):
+    """filtered_backprojection without gradients (fused pass) and with a gradient wanted
+    through the sinogram (weight, FFT correlation, weighted backprojection)."""
+    proj = xrecon_amd.Projector3d((40, 36, 10), (64, 12), 1.0, 1.0, 500.0, 1000.0, scan_type=scan).to(cuda)
+    ang = torch.from_numpy(uniform_angles(72)).to(cuda)
+    y = torch.rand((2, 1, 72, 12, 64), device=cuda, generator=torch.Generator(device=cuda).manual_seed(9))
+    with torch.no_grad():
+        fused = proj.filtered_backprojection(y, ang)
+    yg = y.clone().requires_grad_(True)
+    comp = proj.filtered_backprojection(yg, ang)
+    assert comp.requires_grad
+    assert rel_l2(fused.cpu().numpy(), comp.detach().cpu().numpy()) <= 1e-5
 
 
 def test_fbp2d_config1_vs_fp64(cuda):
@@ -64,7 +100,7 @@ def test_fbp2d_config1_vs_fp64(cuda):
     rec = proj.filtered_backprojection(sino, a)
     w = proj.weight.detach().cpu().double().numpy().reshape(-1)
     h = proj.filter.detach().cpu().double().numpy().reshape(-1)
-    filtered = _correlate_fp64(sino.cpu().double().numpy()[0, 0] * w, h)
+    filtered = correlate_fp64(sino.cpu().double().numpy()[0, 0] * w, h)
     want = oracle.op2d("BW", filtered[None], ang, opts) * _mean_step(ang) / 2
     assert rel_l2(rec.cpu().numpy()[0, 0], want[0]) <= 1e-5
 

@@ -168,6 +168,14 @@ int xr_plan3d_execute_phase(const xr_plan3d *plan, int op, int phase, const floa
                              voxel_chunk, static_cast<hipStream_t>(stream));
 }
 
+int xr_plan3d_execute_fdk(const xr_plan3d *plan, const float *in, float *out, int num_batch, const float *filter,
+                          const float *weight, void *workspace, size_t workspace_bytes, int voxel_chunk, void *stream)
+{
+    if (!plan) return fail(XR_EINVAL, "null plan");
+    return xr::execute_fdk(plan->p, in, out, num_batch, filter, weight, workspace, workspace_bytes, voxel_chunk,
+                           static_cast<hipStream_t>(stream));
+}
+
 int xr_plan3d_execute(const xr_plan3d *plan, int op, const float *in, float *out, int num_batch, void *workspace,
                       size_t workspace_bytes, void *stream)
 {

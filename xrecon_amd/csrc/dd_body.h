@@ -334,6 +334,64 @@ __host__ __device__ inline void sino_prefix_body(const ViewHdr *__restrict__ hdr
     dst[(int64_t)a.nv * a.nu] = make_float2((float)run, 0.0f);
 }
 
+// ---------------------------------------------------------------- FDK filter stage
+
+// Largest detector width the fused FDK filter stage handles (per-thread work grows with
+// the width; wider detectors take the composite path).
+constexpr int kFdkMaxNu = 1024;
+
+// The filter stage reads its taps from an FP64 table with kFdkTapPad zeros on each side
+// (the last row and column blocks index up to that far outside the 2 Nu - 1 taps).
+constexpr int kFdkTapPad = 64;
+
+inline size_t fdk_taps_bytes(const Plan3d &p) { return (size_t)(2 * p.geo.nu - 1 + 2 * kFdkTapPad) * sizeof(double); }
+
+// Filtered, weighted sample of detector column u (physical order) of one row p:
+//   f[u] = sum_s p[s] w[s] h[s - u + Nu - 1],  s ascending, products and sum in FP64,
+// which is the reference's zero-padded correlation of the cosine-weighted row with the
+// 2 Nu - 1 tap filter (projector3d.py:189-190, projector.py:200-201).  k_fdk_filter
+// computes the same sums in the same order.
+__host__ __device__ inline double fdk_filter_output(const float *__restrict__ p, const float *__restrict__ w,
+                                                    const float *__restrict__ h, int nu, int u)
+{
+    double acc = 0.0;
+    for (int s = 0; s < nu; ++s) acc = fma((double)p[s] * (double)w[s], (double)h[s - u + nu - 1], acc);
+    return acc;
+}
+
+// In-place row-prefix of filtered values already stored in Q[.].y: the same table as
+// sino_prefix_body builds from a sinogram, for internal column c of view v, batch b.
+template <int MODE, int GEO>
+__host__ __device__ inline void fdk_prefix_body(const ViewHdr *__restrict__ hdr, const float4 *__restrict__ colt,
+                                                float2 *__restrict__ Q, const SinoArgs &a, int b, int v, int c)
+{
+    const ViewHdr h = hdr[v];
+    const int64_t sv = (int64_t)b * a.nview + v;
+    float2 *dst = Q + sv * (int64_t)(a.nv + 1) * a.nu + c;
+    float scale0 = 1.0f, dbp = 0.0f, sg = 1.0f;
+    if (MODE == SINO_AT) {
+        const float4 cc = colt[(int64_t)v * (a.nu + 1) + c];
+        const float dn = colt[(int64_t)v * (a.nu + 1) + c + 1].x;
+        scale0 = h.delta / cc.w;
+        dbp = 0.5f * (cc.x + dn) - h.P;
+        sg = cc.y;
+    }
+    double total = 0.0;
+    for (int r = 0; r < a.nv; ++r) {
+        float q = dst[(int64_t)r * a.nu].y;
+        if (MODE == SINO_AT) q *= at_scale<GEO>(h, scale0, dbp, sg, a.v0, a.dvv, r);
+        total += (double)q;
+    }
+    double run = -0.5 * total;
+    for (int r = 0; r < a.nv; ++r) {
+        float q = dst[(int64_t)r * a.nu].y;
+        if (MODE == SINO_AT) q *= at_scale<GEO>(h, scale0, dbp, sg, a.v0, a.dvv, r);
+        dst[(int64_t)r * a.nu] = make_float2((float)run, q);
+        run += (double)q;
+    }
+    dst[(int64_t)a.nv * a.nu] = make_float2((float)run, 0.0f);
+}
+
 // ---------------------------------------------------------------- cell gather
 
 // Per-thread state of the cell gather: internal column c (boundaries d0, d1, table entry

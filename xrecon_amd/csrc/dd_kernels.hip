@@ -89,6 +89,86 @@ __global__ __launch_bounds__(kBlock) void k_sino_prefix(const ViewHdr *__restric
     sino_prefix_body<MODE, GEO>(hdr, colt, p, Q, a, blockIdx.z, blockIdx.y, c);
 }
 
+// FDK filter stage.  Every detector row of every view and batch item is filtered with the
+// same taps, so rows are the parallel axis: a workgroup takes kFdkRows consecutive rows
+// (one per lane) and kFdkCols output columns (kFdkOut consecutive columns per wave).
+// The weighted samples p[s] w[s] of a chunk of kFdkChunk columns are staged in LDS in
+// FP64, transposed so that each lane reads its own row conflict-free; the taps are
+// wave-uniform (they depend on s and the output column only) and come from the FP64
+// table T through the scalar cache.  Each output accumulates fma(p[s] w[s], h[.], acc)
+// over s ascending, as fdk_filter_output states it.
+constexpr int kFdkRows = kWave;
+constexpr int kFdkOut = 16;
+constexpr int kFdkCols = kFdkOut * kWavesPerBlock;
+constexpr int kFdkChunk = 64;
+static_assert(kFdkTapPad >= kFdkCols && kFdkTapPad >= kFdkChunk, "tap table padding must cover the last blocks");
+
+// T[i] = h[i - kFdkTapPad] inside the filter, 0 in the padding on either side.
+__global__ __launch_bounds__(kBlock) void k_fdk_taps(const float *__restrict__ h, double *__restrict__ T, int nu)
+{
+    const int n = 2 * nu - 1 + 2 * kFdkTapPad;
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+        const int k = i - kFdkTapPad;
+        T[i] = (k >= 0 && k < 2 * nu - 1) ? (double)h[k] : 0.0;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_fdk_filter(const ViewHdr *__restrict__ hdr, const float *__restrict__ p,
+                                                       const float *__restrict__ wgt, const double *__restrict__ T,
+                                                       float2 *__restrict__ Q, SinoArgs a, int64_t nrows)
+{
+    __shared__ double tile[kFdkChunk][kFdkRows + 1];  // [sample][row], +1 against bank conflicts
+    const int nu = a.nu;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const int64_t g0 = (int64_t)blockIdx.x * kFdkRows;
+    const int ub = blockIdx.y * kFdkCols + wave * kFdkOut;
+    // Tb[s - j] is the tap of sample s for output column ub + j: h[s - (ub + j) + nu - 1]
+    const double *Tb = T + kFdkTapPad + nu - 1 - ub;
+    double acc[kFdkOut];
+#pragma unroll
+    for (int j = 0; j < kFdkOut; ++j) acc[j] = 0.0;
+    for (int s0 = 0; s0 < nu; s0 += kFdkChunk) {
+        __syncthreads();  // the previous chunk has been read
+        const int s = s0 + lane;
+        const double ws = s < nu ? (double)wgt[s] : 0.0;
+        for (int rr = wave; rr < kFdkRows; rr += kWavesPerBlock) {
+            const int64_t g = g0 + rr;
+            tile[lane][rr] = (g < nrows && s < nu) ? (double)p[g * nu + s] * ws : 0.0;
+        }
+        __syncthreads();
+        for (int k0 = 0; k0 < kFdkChunk; k0 += 8) {
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const double x = tile[k0 + k][lane];
+                const int t = s0 + k0 + k;
+#pragma unroll
+                for (int j = 0; j < kFdkOut; ++j) acc[j] = fma(x, Tb[t - j], acc[j]);
+            }
+        }
+    }
+    const int64_t g = g0 + lane;
+    if (g >= nrows) return;
+    const int64_t sv = g / a.nv;
+    const int r = (int)(g - sv * a.nv);
+    const bool keep = hdr[sv % a.nview].keep;
+    float2 *dst = Q + (sv * (a.nv + 1) + r) * nu;
+#pragma unroll
+    for (int j = 0; j < kFdkOut; ++j) {
+        const int u = ub + j;
+        if (u < nu) dst[keep ? u : nu - 1 - u] = make_float2(0.0f, (float)acc[j]);
+    }
+}
+
+template <int MODE, int GEO>
+__global__ __launch_bounds__(kBlock) void k_fdk_prefix(const ViewHdr *__restrict__ hdr, const float4 *__restrict__ colt,
+                                                       float2 *__restrict__ Q, SinoArgs a)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= a.nu) return;
+    fdk_prefix_body<MODE, GEO>(hdr, colt, Q, a, blockIdx.z, blockIdx.y, c);
+}
+
 template <int OPK, int GEO, int R, bool BIG>
 __global__ __launch_bounds__(kBlock) void k_cell(const ViewHdr *__restrict__ hdr, const float4 *__restrict__ lineC,
                                                  const float4 *__restrict__ colt, const float2 *__restrict__ csig,
@@ -215,7 +295,74 @@ void launch_sino_prefix(const Plan3d &p, bool at, const float *in, float2 *Q, in
     else hipLaunchKernelGGL((k_sino_prefix<SINO_PLAIN, GEO>), grid, dim3(kBlock), 0, s, p.d_hdr, p.d_col, in, Q, a);
 }
 
+// The voxel gather over the row-prefix table at the start of ws, with the view-set sum
+// pass when the plan splits its views.
+void voxel_gather(const Plan3d &p, int kop, int KC, void *ws, float *out, int nbatch, hipStream_t s)
+{
+    const float2 *Q = static_cast<const float2 *>(ws);
+    float *part = p.vsplit > 1 ? reinterpret_cast<float *>(static_cast<char *>(ws) + align256(sino_prefix_bytes(p, nbatch)))
+                               : nullptr;
+    if (p.kind == GEO_FLAT) launch_voxel_geo<GEO_FLAT>(p, kop, KC, Q, out, part, nbatch, s);
+    else if (p.kind == GEO_ARC) launch_voxel_geo<GEO_ARC>(p, kop, KC, Q, out, part, nbatch, s);
+    else launch_voxel_geo<GEO_PAR>(p, kop, KC, Q, out, part, nbatch, s);
+    if (p.vsplit > 1) {
+        const int64_t n = (int64_t)nbatch * p.vol.nz * p.vol.ny * p.vol.nx;
+        hipLaunchKernelGGL(k_split_sum, dim3((unsigned)std::min<int64_t>(ceil_div(n, kBlock), 8192)), dim3(kBlock), 0,
+                           s, part, out, n, p.vsplit);
+    }
+}
+
+template <int MODE, int GEO>
+void launch_fdk_prefix(const Plan3d &p, float2 *Q, int nbatch, hipStream_t s)
+{
+    const SinoArgs a = make_sino_args(p, nbatch);
+    hipLaunchKernelGGL((k_fdk_prefix<MODE, GEO>), dim3(ceil_div(a.nu, kBlock), p.nview, nbatch), dim3(kBlock), 0, s,
+                       p.d_hdr, p.d_col, Q, a);
+}
+
 }  // namespace
+
+int execute_fdk(const Plan3d &p, const float *in, float *out, int nbatch, const float *filter, const float *weight,
+                void *ws, size_t ws_bytes, int voxel_chunk, hipStream_t s)
+{
+    const Route rt = route(p, XR_OP_BACKPROJECTION_W);
+    if (nbatch < 0) return fail(XR_EINVAL, "negative batch count");
+    if (nbatch == 0) return XR_OK;
+    if (!in || !out || !filter || !weight) return fail(XR_EINVAL, "null input, output, filter or weight pointer");
+    if (p.geo.nu > kFdkMaxNu)
+        return fail(XR_EINVAL, "the fused FDK filter stage takes at most " + std::to_string(kFdkMaxNu) +
+                                   " detector columns");
+    if (p.nview == 0) {
+        const int64_t n = (int64_t)nbatch * p.vol.nz * p.vol.ny * p.vol.nx;
+        hipLaunchKernelGGL(k_zero, dim3(std::min<int64_t>(ceil_div(n, kBlock), 4096)), dim3(kBlock), 0, s, out, n);
+        if (hipGetLastError() != hipSuccess) return fail(XR_EHIP, "zero-fill launch failed");
+        return XR_OK;
+    }
+    const size_t need = workspace_bytes(p, XR_OP_BACKPROJECTION_W, nbatch);
+    if (ws_bytes < need || !ws)
+        return fail(XR_EWORKSPACE, "workspace of " + std::to_string(ws_bytes) + " bytes, need " + std::to_string(need));
+    if (voxel_chunk == 0) voxel_chunk = default_chunk(p);
+    if (!valid_chunk(voxel_chunk)) return fail(XR_EINVAL, "voxel_chunk must be 1, 4, 8 or 16");
+    float2 *Q = static_cast<float2 *>(ws);
+    // the taps live where the view-split partial volumes go later (workspace_bytes
+    // reserves room for both); the gather runs after the filter stage on the same stream
+    double *T = reinterpret_cast<double *>(static_cast<char *>(ws) + align256(sino_prefix_bytes(p, nbatch)));
+    const SinoArgs a = make_sino_args(p, nbatch);
+    const int nu = p.geo.nu;
+    const int64_t nrows = (int64_t)nbatch * p.nview * p.geo.nv;
+    hipLaunchKernelGGL(k_fdk_taps, dim3((unsigned)ceil_div(2 * nu - 1 + 2 * kFdkTapPad, kBlock)), dim3(kBlock), 0, s,
+                       filter, T, nu);
+    hipLaunchKernelGGL(k_fdk_filter, dim3((unsigned)ceil_div(nrows, kFdkRows), (unsigned)ceil_div(nu, kFdkCols)),
+                       dim3(kBlock), 0, s, p.d_hdr, in, weight, T, Q, a, nrows);
+    const bool at = rt.kop == VOX_AT;
+    if (p.kind == GEO_FLAT) at ? launch_fdk_prefix<SINO_AT, GEO_FLAT>(p, Q, nbatch, s) : launch_fdk_prefix<SINO_PLAIN, GEO_FLAT>(p, Q, nbatch, s);
+    else if (p.kind == GEO_ARC) at ? launch_fdk_prefix<SINO_AT, GEO_ARC>(p, Q, nbatch, s) : launch_fdk_prefix<SINO_PLAIN, GEO_ARC>(p, Q, nbatch, s);
+    else at ? launch_fdk_prefix<SINO_AT, GEO_PAR>(p, Q, nbatch, s) : launch_fdk_prefix<SINO_PLAIN, GEO_PAR>(p, Q, nbatch, s);
+    voxel_gather(p, rt.kop, voxel_chunk, ws, out, nbatch, s);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(XR_EHIP, std::string("kernel launch failed: ") + hipGetErrorString(e));
+    return XR_OK;
+}
 
 int kernel_kind(const Plan3d &p, int op)
 {
@@ -230,7 +377,9 @@ size_t workspace_bytes(const Plan3d &p, int op, int nbatch)
     const Route rt = route(p, op);
     if (rt.kop < 0) return 0;
     if (rt.cell) return align256(vol_prefix_bytes(p, nbatch)) * (p.any_y ? 2 : 1);
-    return align256(sino_prefix_bytes(p, nbatch)) + align256(split_partial_bytes(p, nbatch));
+    size_t tail = split_partial_bytes(p, nbatch);
+    if (op == XR_OP_BACKPROJECTION_W) tail = std::max(tail, fdk_taps_bytes(p));  // FDK filter taps
+    return align256(sino_prefix_bytes(p, nbatch)) + align256(tail);
 }
 
 int execute(const Plan3d &p, int op, const float *in, float *out, int nbatch, void *ws, size_t ws_bytes,
@@ -292,19 +441,7 @@ int execute_phase(const Plan3d &p, int op, int phase, const float *in, float *ou
             else if (p.kind == GEO_ARC) launch_sino_prefix<GEO_ARC>(p, at, in, Q, nbatch, s);
             else launch_sino_prefix<GEO_PAR>(p, at, in, Q, nbatch, s);
         }
-        if (do_gather) {
-            float *part = p.vsplit > 1
-                              ? reinterpret_cast<float *>(static_cast<char *>(ws) + align256(sino_prefix_bytes(p, nbatch)))
-                              : nullptr;
-            if (p.kind == GEO_FLAT) launch_voxel_geo<GEO_FLAT>(p, rt.kop, voxel_chunk, Q, out, part, nbatch, s);
-            else if (p.kind == GEO_ARC) launch_voxel_geo<GEO_ARC>(p, rt.kop, voxel_chunk, Q, out, part, nbatch, s);
-            else launch_voxel_geo<GEO_PAR>(p, rt.kop, voxel_chunk, Q, out, part, nbatch, s);
-            if (p.vsplit > 1) {
-                const int64_t n = (int64_t)nbatch * p.vol.nz * p.vol.ny * p.vol.nx;
-                hipLaunchKernelGGL(k_split_sum, dim3((unsigned)std::min<int64_t>(ceil_div(n, kBlock), 8192)),
-                                   dim3(kBlock), 0, s, part, out, n, p.vsplit);
-            }
-        }
+        if (do_gather) voxel_gather(p, rt.kop, voxel_chunk, ws, out, nbatch, s);
     }
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(XR_EHIP, std::string("kernel launch failed: ") + hipGetErrorString(e));

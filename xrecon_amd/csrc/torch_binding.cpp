@@ -174,6 +174,33 @@ public:
         if (rc) raise_xr(rc, "xr_plan3d_execute");
     }
 
+    // FDK / FBP in one pass: cosine weight, row filter, weighted backprojection.  `filter`
+    // (2 Nu - 1 taps, scaling folded in) and `weight` (Nu) are float32 GPU tensors.
+    torch::Tensor fdk(const torch::Tensor &x, const torch::Tensor &filter, const torch::Tensor &weight, int64_t stream,
+                      int voxel_chunk) const
+    {
+        const int op = XR_OP_BACKPROJECTION_W;
+        check_input(op, x);
+        const int64_t nu = desc_.geo.nu;
+        for (const torch::Tensor *t : {&filter, &weight})
+            TORCH_CHECK(t->is_cuda() && t->scalar_type() == torch::kFloat32 && t->is_contiguous() &&
+                            t->get_device() == device_,
+                        "filter and weight must be contiguous float32 GPU tensors on the plan's device");
+        TORCH_CHECK(filter.numel() == 2 * nu - 1, "filter must have 2*Nu-1 = ", 2 * nu - 1, " taps, got ",
+                    filter.numel());
+        TORCH_CHECK(weight.numel() == nu, "weight must have Nu = ", nu, " entries, got ", weight.numel());
+        const int64_t B = x.size(0), C = x.size(1);
+        auto out = torch::empty(volume_shape(B, C), x.options());
+        const int64_t need = workspace_bytes(op, B * C);
+        auto ws = torch::empty({need > 0 ? need : 0}, x.options().dtype(torch::kUInt8));
+        const int rc = xr_plan3d_execute_fdk(plan_, x.data_ptr<float>(), out.data_ptr<float>(), (int)(B * C),
+                                             filter.data_ptr<float>(), weight.data_ptr<float>(),
+                                             need > 0 ? ws.data_ptr() : nullptr, (size_t)need, voxel_chunk,
+                                             reinterpret_cast<void *>(stream));
+        if (rc) raise_xr(rc, "xr_plan3d_execute_fdk");
+        return out;
+    }
+
     torch::Tensor execute(int op, const torch::Tensor &x, int64_t stream) const
     {
         check_input(op, x);
@@ -324,6 +351,8 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m)
         .def("view_flags", &Plan::view_flags)
         .def("kernel_kind", &Plan::kernel_kind)
         .def("execute", &Plan::execute, py::arg("op"), py::arg("input"), py::arg("stream") = 0)
+        .def("fdk", &Plan::fdk, py::arg("input"), py::arg("filter"), py::arg("weight"), py::arg("stream") = 0,
+             py::arg("voxel_chunk") = 0)
         .def("execute_into", &Plan::execute_into, py::arg("op"), py::arg("input"), py::arg("out"),
              py::arg("workspace"), py::arg("stream") = 0, py::arg("tile_rows") = 0, py::arg("voxel_chunk") = 0,
              py::arg("phase") = XR_PHASE_ALL);
@@ -337,6 +366,7 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m)
         .def("execute_into", &Batch::execute_into, py::arg("op"), py::arg("inputs"), py::arg("outputs"),
              py::arg("workspace"), py::arg("stream") = 0, py::arg("tile_rows") = 0, py::arg("voxel_chunk") = 0);
     m.attr("BATCH_MAX") = XR_BATCH_MAX;
+    m.attr("FDK_MAX_NU") = 1024;
 
     m.attr("OP_PROJECTION") = XR_OP_PROJECTION;
     m.attr("OP_PROJECTION_T") = XR_OP_PROJECTION_T;

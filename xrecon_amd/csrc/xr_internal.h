@@ -102,6 +102,8 @@ int execute(const Plan3d &p, int op, const float *in, float *out, int nbatch, vo
 int kernel_kind(const Plan3d &p, int op);
 // phase 0: prefix + gather; 1: prefix pass only (fills the workspace from `in`);
 // 2: gather pass only (reads the workspace, writes `out`).
+int execute_fdk(const Plan3d &p, const float *in, float *out, int nbatch, const float *filter, const float *weight,
+                void *ws, size_t ws_bytes, int voxel_chunk, hipStream_t s);
 int execute_phase(const Plan3d &p, int op, int phase, const float *in, float *out, int nbatch,
                   void *ws, size_t ws_bytes, int tile_rows, int voxel_chunk, hipStream_t stream);
 

@@ -103,3 +103,14 @@ def correlate_rows(sino: torch.Tensor, filt: torch.Tensor) -> torch.Tensor:
     g[m - (nu - 1):] = hr[:nu - 1]
     spec = torch.fft.rfft(sino, n=m) * torch.fft.rfft(g, n=m)
     return torch.fft.irfft(spec, n=m)[..., :nu]
+
+
+def fused_fdk_ok(sino: torch.Tensor, filt: torch.Tensor, weight: torch.Tensor) -> bool:
+    """Whether FBP/FDK may take the fused native pass: float32 GPU data, a detector of at
+    most 1024 columns, and no gradient wanted through the sinogram, filter or weights
+    (the composite path is the differentiable one)."""
+    if not (sino.is_cuda and sino.dtype == torch.float32 and sino.shape[-1] <= 1024):
+        return False
+    if torch.is_grad_enabled() and (sino.requires_grad or filt.requires_grad or weight.requires_grad):
+        return False
+    return True

@@ -15,6 +15,7 @@ from torch import Tensor
 
 from . import _backend as be
 from .filters import correlate_rows, filter_generate
+from .filters import fused_fdk_ok as _fused_fdk_ok
 from .functions import LinearOp
 from .geometry import FanBeam2d, Geometry2d, ParallelBeam2d, Volume2d, make_geometry2d
 
@@ -164,13 +165,17 @@ class Projector2d(nn.Module):
                 redundant = False
         elif redundant is None:
             redundant = True
+        scale = self._mean_angle_step(angles).to(device=sino.device, dtype=sino.dtype) * (0.5 if redundant else 1.0)
+        if _fused_fdk_ok(sino, filt, self.weight):
+            # one native pass: cosine weight, row filter (FP64 sums), weighted backprojection
+            plan = be.get_plan(self._host_options, torch.as_tensor(angles), sino, True)
+            taps = (filt.reshape(-1).to(device=sino.device, dtype=torch.float32) * scale).contiguous()
+            w = self.weight.reshape(-1).to(device=sino.device, dtype=torch.float32).contiguous()
+            return plan.fdk(sino.contiguous(), taps, w, be.current_stream(sino))
         sino = sino * self.weight.to(device=sino.device, dtype=sino.dtype)
         filtered = correlate_rows(sino, filt.to(device=sino.device, dtype=sino.dtype))
         recon = self.backprojection_weighted(filtered, angles)
-        recon = recon * self._mean_angle_step(angles).to(device=recon.device, dtype=recon.dtype)
-        if redundant:
-            recon = recon / 2
-        return recon
+        return recon * scale
 
     def apply_parker_weight(self, sino: Tensor, angles: Tensor) -> Tensor:
         return sino * self.parker_weight(angles).to(device=sino.device, dtype=sino.dtype)

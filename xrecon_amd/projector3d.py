@@ -16,6 +16,7 @@ from torch import Tensor
 
 from . import _backend as be
 from .filters import correlate_rows, filter_generate
+from .filters import fused_fdk_ok as _fused_fdk_ok
 from .functions import LinearOp
 from .geometry import ConeBeam3d, Geometry3d, ParallelBeam3d, Volume3d, make_geometry3d
 
@@ -180,13 +181,17 @@ class Projector3d(nn.Module):
                 redundant = False
         elif redundant is None:
             redundant = True
+        scale = self._mean_angle_step(angles).to(device=sino.device, dtype=sino.dtype) * (0.5 if redundant else 1.0)
+        if _fused_fdk_ok(sino, filt, self.weight):
+            # one native pass: cosine weight, row filter (FP64 sums), weighted backprojection
+            plan = be.get_plan(self._host_options, torch.as_tensor(angles), sino, False)
+            taps = (filt.reshape(-1).to(device=sino.device, dtype=torch.float32) * scale).contiguous()
+            w = self.weight.reshape(-1).to(device=sino.device, dtype=torch.float32).contiguous()
+            return plan.fdk(sino.contiguous(), taps, w, be.current_stream(sino))
         sino = sino * self.weight.to(device=sino.device, dtype=sino.dtype)
         filtered = correlate_rows(sino, filt.to(device=sino.device, dtype=sino.dtype))
         recon = self.backprojection_weighted(filtered, angles)
-        recon = recon * self._mean_angle_step(angles).to(device=recon.device, dtype=recon.dtype)
-        if redundant:
-            recon = recon / 2
-        return recon
+        return recon * scale
 
     def parker_weight(self, angles: Tensor) -> Tensor:
         """Parker short-scan weights (projector3d.py:197-227), [1, 1, V, 1, Nu]."""
