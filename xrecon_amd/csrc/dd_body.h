@@ -359,15 +359,13 @@ __host__ __device__ inline double fdk_filter_output(const float *__restrict__ p,
     return acc;
 }
 
-// In-place row-prefix of filtered values already stored in Q[.].y: the same table as
-// sino_prefix_body builds from a sinogram, for internal column c of view v, batch b.
-template <int MODE, int GEO>
-__host__ __device__ inline void fdk_prefix_body(const ViewHdr *__restrict__ hdr, const float4 *__restrict__ colt,
-                                                float2 *__restrict__ Q, const SinoArgs &a, int b, int v, int c)
+// Row-prefix table entries of internal column c of view v (header h) from the filtered
+// samples q(r), r = 0 .. Nv-1, written to dst[r * Nu]: the same table sino_prefix_body
+// builds from a sinogram (A^T mode scales each sample first).
+template <int MODE, int GEO, class QOf>
+__host__ __device__ inline void fdk_prefix_column(const ViewHdr &h, const float4 *__restrict__ colt, const SinoArgs &a,
+                                                  int v, int c, QOf qof, float2 *__restrict__ dst)
 {
-    const ViewHdr h = hdr[v];
-    const int64_t sv = (int64_t)b * a.nview + v;
-    float2 *dst = Q + sv * (int64_t)(a.nv + 1) * a.nu + c;
     float scale0 = 1.0f, dbp = 0.0f, sg = 1.0f;
     if (MODE == SINO_AT) {
         const float4 cc = colt[(int64_t)v * (a.nu + 1) + c];
@@ -378,18 +376,29 @@ __host__ __device__ inline void fdk_prefix_body(const ViewHdr *__restrict__ hdr,
     }
     double total = 0.0;
     for (int r = 0; r < a.nv; ++r) {
-        float q = dst[(int64_t)r * a.nu].y;
+        float q = qof(r);
         if (MODE == SINO_AT) q *= at_scale<GEO>(h, scale0, dbp, sg, a.v0, a.dvv, r);
         total += (double)q;
     }
     double run = -0.5 * total;
     for (int r = 0; r < a.nv; ++r) {
-        float q = dst[(int64_t)r * a.nu].y;
+        float q = qof(r);
         if (MODE == SINO_AT) q *= at_scale<GEO>(h, scale0, dbp, sg, a.v0, a.dvv, r);
         dst[(int64_t)r * a.nu] = make_float2((float)run, q);
         run += (double)q;
     }
     dst[(int64_t)a.nv * a.nu] = make_float2((float)run, 0.0f);
+}
+
+// In-place row-prefix of filtered values already stored in Q[.].y, for internal column
+// c of view v, batch item b.
+template <int MODE, int GEO>
+__host__ __device__ inline void fdk_prefix_body(const ViewHdr *__restrict__ hdr, const float4 *__restrict__ colt,
+                                                float2 *__restrict__ Q, const SinoArgs &a, int b, int v, int c)
+{
+    const int64_t sv = (int64_t)b * a.nview + v;
+    float2 *dst = Q + sv * (int64_t)(a.nv + 1) * a.nu + c;
+    fdk_prefix_column<MODE, GEO>(hdr[v], colt, a, v, c, [dst, &a](int r) { return dst[(int64_t)r * a.nu].y; }, dst);
 }
 
 // ---------------------------------------------------------------- cell gather

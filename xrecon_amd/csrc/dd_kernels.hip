@@ -93,15 +93,23 @@ __global__ __launch_bounds__(kBlock) void k_sino_prefix(const ViewHdr *__restric
 // same taps, so rows are the parallel axis: a workgroup takes kFdkRows consecutive rows
 // (one per lane) and kFdkCols output columns (kFdkOut consecutive columns per wave).
 // The weighted samples p[s] w[s] of a chunk of kFdkChunk columns are staged in LDS in
-// FP64, transposed so that each lane reads its own row conflict-free; the taps are
-// wave-uniform (they depend on s and the output column only) and come from the FP64
-// table T through the scalar cache.  Each output accumulates fma(p[s] w[s], h[.], acc)
-// over s ascending, as fdk_filter_output states it.
+// FP64, transposed so that each lane reads its own row conflict-free, while the next
+// chunk's samples load into registers; the taps are wave-uniform (they depend on s and
+// the output column only) and come from the FP64 table T through the scalar cache.  Each
+// output accumulates fma(p[s] w[s], h[.], acc) over s ascending, as fdk_filter_output
+// states it.
+//
+// FUSED (Nv divides kFdkRows, so a row group holds whole views): the workgroup then
+// builds the row-prefix table of its views' columns itself (fdk_prefix_column over the
+// filtered samples kept in LDS).  Otherwise it writes the filtered samples to Q[.].y and
+// k_fdk_prefix builds the table.
 constexpr int kFdkRows = kWave;
 constexpr int kFdkOut = 16;
 constexpr int kFdkCols = kFdkOut * kWavesPerBlock;
 constexpr int kFdkChunk = 64;
+constexpr int kFdkLoads = kFdkRows / kWavesPerBlock;  // samples per thread per chunk
 static_assert(kFdkTapPad >= kFdkCols && kFdkTapPad >= kFdkChunk, "tap table padding must cover the last blocks");
+static_assert(kFdkChunk == kWave, "one staged sample column per lane");
 
 // T[i] = h[i - kFdkTapPad] inside the filter, 0 in the padding on either side.
 __global__ __launch_bounds__(kBlock) void k_fdk_taps(const float *__restrict__ h, double *__restrict__ T, int nu)
@@ -113,9 +121,11 @@ __global__ __launch_bounds__(kBlock) void k_fdk_taps(const float *__restrict__ h
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_fdk_filter(const ViewHdr *__restrict__ hdr, const float *__restrict__ p,
-                                                       const float *__restrict__ wgt, const double *__restrict__ T,
-                                                       float2 *__restrict__ Q, SinoArgs a, int64_t nrows)
+template <int MODE, int GEO, bool FUSED>
+__global__ __launch_bounds__(kBlock) void k_fdk_filter(const ViewHdr *__restrict__ hdr, const float4 *__restrict__ colt,
+                                                       const float *__restrict__ p, const float *__restrict__ wgt,
+                                                       const double *__restrict__ T, float2 *__restrict__ Q,
+                                                       SinoArgs a, int64_t nrows)
 {
     __shared__ double tile[kFdkChunk][kFdkRows + 1];  // [sample][row], +1 against bank conflicts
     const int nu = a.nu;
@@ -128,15 +138,24 @@ __global__ __launch_bounds__(kBlock) void k_fdk_filter(const ViewHdr *__restrict
     double acc[kFdkOut];
 #pragma unroll
     for (int j = 0; j < kFdkOut; ++j) acc[j] = 0.0;
+    float next[kFdkLoads];  // thread's samples of the next chunk: rows wave + 4 i, column lane
+    auto load = [&](int s0) {
+        const int s = s0 + lane;
+#pragma unroll
+        for (int i = 0; i < kFdkLoads; ++i) {
+            const int64_t g = g0 + wave + kWavesPerBlock * i;
+            next[i] = (g < nrows && s < nu) ? p[g * nu + s] : 0.0f;
+        }
+    };
+    load(0);
     for (int s0 = 0; s0 < nu; s0 += kFdkChunk) {
-        __syncthreads();  // the previous chunk has been read
         const int s = s0 + lane;
         const double ws = s < nu ? (double)wgt[s] : 0.0;
-        for (int rr = wave; rr < kFdkRows; rr += kWavesPerBlock) {
-            const int64_t g = g0 + rr;
-            tile[lane][rr] = (g < nrows && s < nu) ? (double)p[g * nu + s] * ws : 0.0;
-        }
+        __syncthreads();  // the previous chunk has been read
+#pragma unroll
+        for (int i = 0; i < kFdkLoads; ++i) tile[lane][wave + kWavesPerBlock * i] = (double)next[i] * ws;
         __syncthreads();
+        if (s0 + kFdkChunk < nu) load(s0 + kFdkChunk);
         for (int k0 = 0; k0 < kFdkChunk; k0 += 8) {
 #pragma unroll
             for (int k = 0; k < 8; ++k) {
@@ -147,16 +166,41 @@ __global__ __launch_bounds__(kBlock) void k_fdk_filter(const ViewHdr *__restrict
             }
         }
     }
-    const int64_t g = g0 + lane;
-    if (g >= nrows) return;
-    const int64_t sv = g / a.nv;
-    const int r = (int)(g - sv * a.nv);
-    const bool keep = hdr[sv % a.nview].keep;
-    float2 *dst = Q + (sv * (a.nv + 1) + r) * nu;
+    if (!FUSED) {
+        const int64_t g = g0 + lane;
+        if (g >= nrows) return;
+        const int64_t sv = g / a.nv;
+        const int r = (int)(g - sv * a.nv);
+        const bool keep = hdr[sv % a.nview].keep;
+        float2 *dst = Q + (sv * (a.nv + 1) + r) * nu;
 #pragma unroll
-    for (int j = 0; j < kFdkOut; ++j) {
-        const int u = ub + j;
-        if (u < nu) dst[keep ? u : nu - 1 - u] = make_float2(0.0f, (float)acc[j]);
+        for (int j = 0; j < kFdkOut; ++j) {
+            const int u = ub + j;
+            if (u < nu) dst[keep ? u : nu - 1 - u] = make_float2(0.0f, (float)acc[j]);
+        }
+        return;
+    }
+    // filtered samples of the row group, [row][column], in the staging buffer
+    constexpr int kQPitch = kFdkCols + 1;
+    static_assert(kFdkRows * kQPitch * sizeof(float) <= sizeof(tile), "filtered block must fit the staging buffer");
+    float *qbuf = reinterpret_cast<float *>(&tile[0][0]);
+    __syncthreads();  // every wave has read its last chunk
+#pragma unroll
+    for (int j = 0; j < kFdkOut; ++j) qbuf[lane * kQPitch + wave * kFdkOut + j] = (float)acc[j];
+    __syncthreads();
+    const int nvg = kFdkRows / a.nv;  // whole views in the row group
+    const int64_t sv0 = g0 / a.nv;
+    for (int task = threadIdx.x; task < nvg * kFdkCols; task += kBlock) {
+        const int vg = task / kFdkCols, cl = task - vg * kFdkCols;
+        const int u = blockIdx.y * kFdkCols + cl;
+        const int64_t sv = sv0 + vg;
+        if (u >= nu || sv * a.nv >= nrows) continue;
+        const int v = (int)(sv % a.nview);
+        const ViewHdr h = hdr[v];
+        const int c = h.keep ? u : nu - 1 - u;
+        const float *qcol = qbuf + vg * a.nv * kQPitch + cl;
+        fdk_prefix_column<MODE, GEO>(h, colt, a, v, c, [qcol](int r) { return qcol[r * kQPitch]; },
+                                     Q + sv * (int64_t)(a.nv + 1) * nu + c);
     }
 }
 
@@ -312,10 +356,23 @@ void voxel_gather(const Plan3d &p, int kop, int KC, void *ws, float *out, int nb
     }
 }
 
+// Filter stage + row-prefix table of the filtered sinogram (fused when Nv divides the
+// row group, two kernels otherwise).
 template <int MODE, int GEO>
-void launch_fdk_prefix(const Plan3d &p, float2 *Q, int nbatch, hipStream_t s)
+void launch_fdk_table(const Plan3d &p, const float *in, const float *weight, const double *T, float2 *Q, int nbatch,
+                      hipStream_t s)
 {
     const SinoArgs a = make_sino_args(p, nbatch);
+    const int64_t nrows = (int64_t)nbatch * p.nview * p.geo.nv;
+    const dim3 grid((unsigned)ceil_div(nrows, kFdkRows), (unsigned)ceil_div(a.nu, kFdkCols));
+    if (kFdkRows % a.nv == 0) {
+        hipLaunchKernelGGL((k_fdk_filter<MODE, GEO, true>), grid, dim3(kBlock), 0, s, p.d_hdr, p.d_col, in, weight, T,
+                           Q, a, nrows);
+        return;
+    }
+    // the unfused filter does not depend on the mode or geometry: one instance
+    hipLaunchKernelGGL((k_fdk_filter<SINO_PLAIN, GEO_FLAT, false>), grid, dim3(kBlock), 0, s, p.d_hdr, p.d_col, in,
+                       weight, T, Q, a, nrows);
     hipLaunchKernelGGL((k_fdk_prefix<MODE, GEO>), dim3(ceil_div(a.nu, kBlock), p.nview, nbatch), dim3(kBlock), 0, s,
                        p.d_hdr, p.d_col, Q, a);
 }
@@ -347,17 +404,16 @@ int execute_fdk(const Plan3d &p, const float *in, float *out, int nbatch, const 
     // the taps live where the view-split partial volumes go later (workspace_bytes
     // reserves room for both); the gather runs after the filter stage on the same stream
     double *T = reinterpret_cast<double *>(static_cast<char *>(ws) + align256(sino_prefix_bytes(p, nbatch)));
-    const SinoArgs a = make_sino_args(p, nbatch);
     const int nu = p.geo.nu;
-    const int64_t nrows = (int64_t)nbatch * p.nview * p.geo.nv;
     hipLaunchKernelGGL(k_fdk_taps, dim3((unsigned)ceil_div(2 * nu - 1 + 2 * kFdkTapPad, kBlock)), dim3(kBlock), 0, s,
                        filter, T, nu);
-    hipLaunchKernelGGL(k_fdk_filter, dim3((unsigned)ceil_div(nrows, kFdkRows), (unsigned)ceil_div(nu, kFdkCols)),
-                       dim3(kBlock), 0, s, p.d_hdr, in, weight, T, Q, a, nrows);
     const bool at = rt.kop == VOX_AT;
-    if (p.kind == GEO_FLAT) at ? launch_fdk_prefix<SINO_AT, GEO_FLAT>(p, Q, nbatch, s) : launch_fdk_prefix<SINO_PLAIN, GEO_FLAT>(p, Q, nbatch, s);
-    else if (p.kind == GEO_ARC) at ? launch_fdk_prefix<SINO_AT, GEO_ARC>(p, Q, nbatch, s) : launch_fdk_prefix<SINO_PLAIN, GEO_ARC>(p, Q, nbatch, s);
-    else at ? launch_fdk_prefix<SINO_AT, GEO_PAR>(p, Q, nbatch, s) : launch_fdk_prefix<SINO_PLAIN, GEO_PAR>(p, Q, nbatch, s);
+    if (p.kind == GEO_FLAT) at ? launch_fdk_table<SINO_AT, GEO_FLAT>(p, in, weight, T, Q, nbatch, s)
+                               : launch_fdk_table<SINO_PLAIN, GEO_FLAT>(p, in, weight, T, Q, nbatch, s);
+    else if (p.kind == GEO_ARC) at ? launch_fdk_table<SINO_AT, GEO_ARC>(p, in, weight, T, Q, nbatch, s)
+                                   : launch_fdk_table<SINO_PLAIN, GEO_ARC>(p, in, weight, T, Q, nbatch, s);
+    else at ? launch_fdk_table<SINO_AT, GEO_PAR>(p, in, weight, T, Q, nbatch, s)
+            : launch_fdk_table<SINO_PLAIN, GEO_PAR>(p, in, weight, T, Q, nbatch, s);
     voxel_gather(p, rt.kop, voxel_chunk, ws, out, nbatch, s);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(XR_EHIP, std::string("kernel launch failed: ") + hipGetErrorString(e));
