@@ -143,16 +143,19 @@ def emu_fdk3d(y, angles, opts, filt, weight, is2d=False, KC=0, nthreads=None):
     return out
 
 
-def emu_view_spans(angles, opts):
-    """Span class per view (2 / 3: fixed-span cell path, 0: general loop)."""
+def emu_view_spans(angles, opts, groups=False):
+    """Span class per view (2 / 3: fixed-span cell path, 0: general loop); with
+    groups=True also the class per (view, 64-column group), which the kernels use."""
     lib = _load_emu()
     vn, vf, gn, gf = oracle.split3d(opts)
     angles = np.ascontiguousarray(angles, dtype=np.float32)
     spans = np.zeros(angles.shape[0], dtype=np.int32)
-    rc = lib.emu_view_spans(_p(vn), _p(vf), _p(gn), _p(gf), _p(angles), angles.shape[0], _p(spans))
+    ncg = -(-int(opts[3]) // 64)
+    gsp = np.zeros((angles.shape[0], ncg), dtype=np.int32)
+    rc = lib.emu_view_spans(_p(vn), _p(vf), _p(gn), _p(gf), _p(angles), angles.shape[0], _p(spans), _p(gsp))
     if rc != 0:
         raise RuntimeError(lib.emu_error().decode())
-    return spans
+    return (spans, gsp) if groups else spans
 
 
 def opts2d_as_3d(o2):

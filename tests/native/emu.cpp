@@ -40,11 +40,11 @@ void emu_cell_wave(const Plan3d &p, const CellArgs &a, const float2 *WX, const f
     const float2 *W = cell_table(WX, WY, h, a, b);
     const float4 *lt = p.d_lineC + (int64_t)v * a.maxline;
     const float dfirst = ct[cg * kWave].x, dlast = ct[imin(a.nu, cg * kWave + kWave)].x;
-    const float2 sg = p.d_csig[(int64_t)v * ceil_div(a.nu, kWave) + cg];
+    const CellGroup grp = p.d_cgrp[(int64_t)v * ceil_div(a.nu, kWave) + cg];
     std::vector<float2> lds(cell_lds_cap<R>());
     for (int L = 0; L < h.nline; ++L) {
         const float4 lc = lt[L];
-        const CellBox bx = cell_box<R, SPAN>(a, lc, dfirst, dlast, sg.x, sg.y, r0, h.nalong);
+        const CellBox bx = cell_box<R, SPAN>(a, lc, dfirst, dlast, grp.sig_lo, grp.sig_hi, r0, h.nalong);
         if (bx.pitch == 0) continue;
         for (int lane = 0; lane < kWave; ++lane) cell_stage(W + (int64_t)L * h.nalong, bx, lds.data(), a.plane8, lane);
         for (int lane = 0; lane < kWave; ++lane)
@@ -70,27 +70,34 @@ void run_cell_RV(const Plan3d &p, const float2 *WX, const float2 *WY, float *out
         th.emplace_back([&, t]() {
             for (int it = t; it < items; it += nthreads) {
                 const int v = it % p.nview, b = it / p.nview;
-                const int span = (MODE == 0 || MODE == 3) && !p.big ? p.d_hdr[v].span : 0;
-                for (int g = 0; g < ngroups; ++g) {
-                    if (MODE == 0 && span == 2) {
-                        for (int cg = 0; cg < ncg; ++cg) emu_cell_wave<OPK, GEO, R, 2>(p, a, WX, WY, out, cg, g * R, v, b);
-                        continue;
+                for (int g = 0; g < ngroups; ++g)
+                    for (int cg = 0; cg < ncg; ++cg) {
+                        // the kernels choose the variant per (view, column group)
+                        const int span =
+                            (MODE == 0 || MODE == 3) && !p.big ? p.d_cgrp[(int64_t)v * ncg + cg].span : 0;
+                        if (MODE == 0 && span == 2) {
+                            emu_cell_wave<OPK, GEO, R, 2>(p, a, WX, WY, out, cg, g * R, v, b);
+                            continue;
+                        }
+                        if (MODE == 0 && span == 3) {
+                            emu_cell_wave<OPK, GEO, R, 3>(p, a, WX, WY, out, cg, g * R, v, b);
+                            continue;
+                        }
+                        for (int c = cg * kWave; c < std::min(a.nu, (cg + 1) * kWave); ++c) {
+                            if (span == 2)
+                                cell_body<OPK, GEO, R, 2, false>(p.d_hdr, p.d_lineC, p.d_col, WX, WY, out, a, c, g * R,
+                                                                 v, b);
+                            else if (span == 3)
+                                cell_body<OPK, GEO, R, 3, false>(p.d_hdr, p.d_lineC, p.d_col, WX, WY, out, a, c, g * R,
+                                                                 v, b);
+                            else if (MODE == 2 || p.big)
+                                cell_body<OPK, GEO, R, 0, true>(p.d_hdr, p.d_lineC, p.d_col, WX, WY, out, a, c, g * R,
+                                                                v, b);
+                            else
+                                cell_body<OPK, GEO, R, 0, false>(p.d_hdr, p.d_lineC, p.d_col, WX, WY, out, a, c, g * R,
+                                                                 v, b);
+                        }
                     }
-                    if (MODE == 0 && span == 3) {
-                        for (int cg = 0; cg < ncg; ++cg) emu_cell_wave<OPK, GEO, R, 3>(p, a, WX, WY, out, cg, g * R, v, b);
-                        continue;
-                    }
-                    for (int c = 0; c < a.nu; ++c) {
-                        if (span == 2)
-                            cell_body<OPK, GEO, R, 2, false>(p.d_hdr, p.d_lineC, p.d_col, WX, WY, out, a, c, g * R, v, b);
-                        else if (span == 3)
-                            cell_body<OPK, GEO, R, 3, false>(p.d_hdr, p.d_lineC, p.d_col, WX, WY, out, a, c, g * R, v, b);
-                        else if (MODE == 2 || p.big)
-                            cell_body<OPK, GEO, R, 0, true>(p.d_hdr, p.d_lineC, p.d_col, WX, WY, out, a, c, g * R, v, b);
-                        else
-                            cell_body<OPK, GEO, R, 0, false>(p.d_hdr, p.d_lineC, p.d_col, WX, WY, out, a, c, g * R, v, b);
-                    }
-                }
             }
         });
     }
@@ -275,7 +282,7 @@ int make_plan(Plan3d &p, PlanTables &t, const int *vn, const float *vf, const in
     p.d_lineV = t.lineV.data();
     p.d_col = t.col.data();
     p.d_vtile = t.vtile.data();
-    p.d_csig = t.csig.data();
+    p.d_cgrp = t.cgrp.data();
     return XR_OK;
 }
 
@@ -308,24 +315,19 @@ int emu_fdk3d(const int *vn, const float *vf, const int *gn, const float *gf, co
     return XR_OK;
 }
 
-// Per-view span class of the cell gather (ViewHdr::span) for a geometry.
+// Span classes of the cell gather for a geometry: per view (ViewHdr::span) into
+// `spans` [nview], and per (view, 64-column group) (CellGroup::span) into `group_spans`
+// [nview][ceil(Nu/64)] when it is not null.
 int emu_view_spans(const int *vn, const float *vf, const int *gn, const float *gf, const float *angles, int nview,
-                   int *spans)
+                   int *spans, int *group_spans)
 {
     Plan3d p;
-    p.vol = xr_volume3d{vn[0], vn[1], vn[2], vf[0], vf[1], vf[2], vf[3], vf[4], vf[5]};
-    p.geo = xr_geometry3d{gn[0] & 0xff, gn[1], gn[2], gf[0], gf[1], gf[2], gf[3], gf[4], gf[5]};
-    p.is2d = (gn[0] & 0x100) ? 1 : 0;
-    p.kind = p.geo.type;
-    p.nview = nview;
     PlanTables t;
-    std::string err;
-    const int rc = build_tables(p, angles, t, err);
-    if (rc) {
-        g_err = err;
-        return rc;
-    }
+    const int rc = make_plan(p, t, vn, vf, gn, gf, angles, nview);
+    if (rc) return rc;
     for (int v = 0; v < nview; ++v) spans[v] = t.hdr[v].span;
+    if (group_spans)
+        for (size_t k = 0; k < t.cgrp.size(); ++k) group_spans[k] = t.cgrp[k].span;
     return XR_OK;
 }
 

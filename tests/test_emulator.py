@@ -133,6 +133,30 @@ def test_span_classes_cover_fixed_and_general_paths():
         assert rel_l2(emu_op3d(op, x, ang, opts), oracle.op3d(op, x.astype(np.float64), ang, opts)) <= 1e-5
 
 
+def test_span_classes_per_column_group():
+    """The kernels take the span class of their 64-column group: never wider than the
+    view's class, the view's class is the widest of its groups, and on diagonal views of a
+    wide flat panel the central groups take the two-pixel path while the edge groups take
+    three pixels.  Views with mixed groups agree with the oracle through every cell
+    operator, staged and unstaged."""
+    opts = opts3d(64, 60, 8, 160, 8, 1.0, 1.0, 1.0, 1.2, 1.0, 500.0, 1000.0, 1, 0.2, -0.1, 0.1, 0.2, 0.1)
+    ang = angles_with_ties(48)
+    spans, groups = emu_view_spans(ang, opts, groups=True)
+    rank = {2: 0, 3: 1, 0: 2}
+    vr = np.vectorize(rank.get)
+    assert np.all(vr(groups) <= vr(spans)[:, None])
+    assert np.all(vr(groups).max(axis=1) == vr(spans))
+    mixed = [v for v in range(len(ang)) if len(set(groups[v])) > 1]
+    assert mixed
+    sub = np.ascontiguousarray(ang[mixed])
+    vol, _ = shapes3d(opts, 1, len(mixed))
+    x = np.random.default_rng(12).random(vol).astype(np.float32)
+    for op in ("A", "BT", "BWT"):
+        want = oracle.op3d(op, x.astype(np.float64), sub, opts)
+        for variant in (0, 3):
+            assert rel_l2(emu_op3d(op, x, sub, opts, cell_variant=variant), want) <= 1e-5, (op, variant)
+
+
 @pytest.mark.parametrize("geo", sorted(CASES3D))
 def test_voxel_gather_staged_and_direct_agree(geo):
     """The voxel gather with the per-view block staged in LDS (lookups outside the block

@@ -27,7 +27,7 @@ struct BatchDesc {
     const ViewHdr *hdr;
     const float4 *lineC, *lineV, *col;
     const VoxTile *vtile;
-    const float2 *csig;
+    const CellGroup *cgrp;
     CellArgs ca;
     VoxArgs va;  // nchunk is filled per launch from the launch's voxel chunk
     SinoArgs sa;
@@ -146,21 +146,21 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
     const float2 *WX = reinterpret_cast<const float2 *>(ws + d.ws_volx);
     const float2 *WY = reinterpret_cast<const float2 *>(ws + d.ws_voly);
     const int v = bz % a.nview, b = bz / a.nview;
-    // the per-view variant choice is uniform over the workgroup; a batch holding a big
-    // plan runs every instance through the 64-bit general path
-    const int span = BIG ? 0 : d.hdr[v].span;
+    // the variant is chosen per (view, column group), uniform over the workgroup; a
+    // batch holding a big plan runs every instance through the 64-bit general path
+    const int span = BIG ? 0 : d.cgrp[(int64_t)v * gx + bx].span;
     if (BIG) {
         if (c < a.nu) cell_body<OPK, GEO, R, 0, true>(d.hdr, d.lineC, d.col, WX, WY, io.out[i], a, c, r0, v, b);
     } else if (span == 2) {
 #if XR_CELL_STAGED
-        cell_wave<OPK, GEO, R, 2>(d.hdr, d.lineC, d.col, d.csig, WX, WY, io.out[i], a, c, c < a.nu, bx, r0, v, b,
+        cell_wave<OPK, GEO, R, 2>(d.hdr, d.lineC, d.col, d.cgrp, WX, WY, io.out[i], a, c, c < a.nu, bx, r0, v, b,
                                   lds[wave]);
 #else
         if (c < a.nu) cell_body<OPK, GEO, R, 2, false>(d.hdr, d.lineC, d.col, WX, WY, io.out[i], a, c, r0, v, b);
 #endif
     } else if (span == 3) {
 #if XR_CELL_STAGED
-        cell_wave<OPK, GEO, R, 3>(d.hdr, d.lineC, d.col, d.csig, WX, WY, io.out[i], a, c, c < a.nu, bx, r0, v, b,
+        cell_wave<OPK, GEO, R, 3>(d.hdr, d.lineC, d.col, d.cgrp, WX, WY, io.out[i], a, c, c < a.nu, bx, r0, v, b,
                                   lds[wave]);
 #else
         if (c < a.nu) cell_body<OPK, GEO, R, 3, false>(d.hdr, d.lineC, d.col, WX, WY, io.out[i], a, c, r0, v, b);
@@ -318,7 +318,7 @@ int xr_batch3d_create(xr_plan3d *const *plans, int n, const int *num_batch, xr_b
         d.lineV = p.d_lineV;
         d.col = p.d_col;
         d.vtile = p.d_vtile;
-        d.csig = p.d_csig;
+        d.cgrp = p.d_cgrp;
         d.ca = xr::make_cell_args(p, num_batch[i]);
         d.va = xr::make_vox_args(p, num_batch[i], 1);
         d.sa = xr::make_sino_args(p, num_batch[i]);

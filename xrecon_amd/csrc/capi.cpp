@@ -88,11 +88,11 @@ void free_tables(Plan3d &p)
     if (p.d_lineV) (void)hipFree(p.d_lineV);
     if (p.d_col) (void)hipFree(p.d_col);
     if (p.d_vtile) (void)hipFree(p.d_vtile);
-    if (p.d_csig) (void)hipFree(p.d_csig);
+    if (p.d_cgrp) (void)hipFree(p.d_cgrp);
     p.d_hdr = nullptr;
     p.d_lineC = p.d_lineV = p.d_col = nullptr;
     p.d_vtile = nullptr;
-    p.d_csig = nullptr;
+    p.d_cgrp = nullptr;
 }
 
 }  // namespace
@@ -128,7 +128,7 @@ int xr_plan3d_create(const xr_volume3d *vol, const xr_geometry3d *geo, const flo
     DeviceGuard g(device);
     if ((rc = upload(&p.d_hdr, t.hdr)) || (rc = upload(&p.d_lineC, t.lineC)) || (rc = upload(&p.d_lineV, t.lineV)) ||
         (rc = upload(&p.d_col, t.col)) || (rc = upload(&p.d_vtile, t.vtile)) ||
-        (rc = upload(&p.d_csig, t.csig))) {
+        (rc = upload(&p.d_cgrp, t.cgrp))) {
         free_tables(p);
         return rc;
     }

@@ -702,7 +702,7 @@ __host__ __device__ inline void cell_body(const ViewHdr *__restrict__ hdr, const
 // barrier: the wave waits for its own loads), then every lane gathers from it.
 template <int OPK, int GEO, int R, int SPAN>
 __device__ inline void cell_wave(const ViewHdr *__restrict__ hdr, const float4 *__restrict__ lineC,
-                                 const float4 *__restrict__ colt, const float2 *__restrict__ csig,
+                                 const float4 *__restrict__ colt, const CellGroup *__restrict__ cgrp,
                                  const float2 *__restrict__ WX, const float2 *__restrict__ WY,
                                  float *__restrict__ out, const CellArgs &a, int c, bool lane_ok, int cg, int r0,
                                  int v, int b, float2 *__restrict__ lds)
@@ -714,11 +714,11 @@ __device__ inline void cell_wave(const ViewHdr *__restrict__ hdr, const float4 *
     const float2 *W = cell_table(WX, WY, h, a, b);
     const float4 *lt = lineC + (int64_t)v * a.maxline;
     const float dfirst = ct[cg * kWave].x, dlast = ct[imin(a.nu, cg * kWave + kWave)].x;
-    const float2 sg = csig[(int64_t)v * ceil_div(a.nu, kWave) + cg];
+    const CellGroup grp = cgrp[(int64_t)v * ceil_div(a.nu, kWave) + cg];
     const int lane = threadIdx.x & (kWave - 1);
     for (int L = 0; L < h.nline; ++L) {
         const float4 lc = lt[L];
-        const CellBox bx = cell_box<R, SPAN>(a, lc, dfirst, dlast, sg.x, sg.y, r0, h.nalong);
+        const CellBox bx = cell_box<R, SPAN>(a, lc, dfirst, dlast, grp.sig_lo, grp.sig_hi, r0, h.nalong);
         if (bx.pitch == 0) continue;  // no column of the wave meets this line
         cell_stage(W + (int64_t)L * h.nalong, bx, lds, a.plane8, lane);
         __builtin_amdgcn_s_waitcnt(0);  // this wave's direct-to-LDS loads have landed

@@ -215,7 +215,7 @@ __global__ __launch_bounds__(kBlock) void k_fdk_prefix(const ViewHdr *__restrict
 
 template <int OPK, int GEO, int R, bool BIG>
 __global__ __launch_bounds__(kBlock) void k_cell(const ViewHdr *__restrict__ hdr, const float4 *__restrict__ lineC,
-                                                 const float4 *__restrict__ colt, const float2 *__restrict__ csig,
+                                                 const float4 *__restrict__ colt, const CellGroup *__restrict__ cgrp,
                                                  const float2 *__restrict__ WX, const float2 *__restrict__ WY,
                                                  float *__restrict__ out, CellArgs a)
 {
@@ -229,17 +229,16 @@ __global__ __launch_bounds__(kBlock) void k_cell(const ViewHdr *__restrict__ hdr
     const int v = blockIdx.z % a.nview;
     const int b = blockIdx.z / a.nview;
     if (r0 >= a.nv) return;  // the whole wave
-    // the per-view variant choice is uniform over the workgroup (one view per workgroup)
-    const int span = BIG ? 0 : hdr[v].span;
+    // the variant is chosen per (view, column group): uniform over the workgroup
+    const int span = BIG ? 0 : cgrp[(int64_t)v * gridDim.x + blockIdx.x].span;
 #if XR_CELL_STAGED
     if (span == 2)
-        cell_wave<OPK, GEO, R, 2>(hdr, lineC, colt, csig, WX, WY, out, a, c, c < a.nu, blockIdx.x, r0, v, b, lds[wave]);
+        cell_wave<OPK, GEO, R, 2>(hdr, lineC, colt, cgrp, WX, WY, out, a, c, c < a.nu, blockIdx.x, r0, v, b, lds[wave]);
     else if (span == 3)
-        cell_wave<OPK, GEO, R, 3>(hdr, lineC, colt, csig, WX, WY, out, a, c, c < a.nu, blockIdx.x, r0, v, b, lds[wave]);
+        cell_wave<OPK, GEO, R, 3>(hdr, lineC, colt, cgrp, WX, WY, out, a, c, c < a.nu, blockIdx.x, r0, v, b, lds[wave]);
     else if (c < a.nu)
         cell_body<OPK, GEO, R, 0, BIG>(hdr, lineC, colt, WX, WY, out, a, c, r0, v, b);
 #else
-    (void)csig;
     if (c >= a.nu) return;
     if (span == 2) cell_body<OPK, GEO, R, 2, false>(hdr, lineC, colt, WX, WY, out, a, c, r0, v, b);
     else if (span == 3) cell_body<OPK, GEO, R, 3, false>(hdr, lineC, colt, WX, WY, out, a, c, r0, v, b);
@@ -284,7 +283,7 @@ void launch_cell(const Plan3d &p, int R, const float2 *WX, const float2 *WY, flo
     if (p.big) R = 8;  // tables of 4 GiB and more: one 64-bit general variant
     dim3 grid(ceil_div(a.nu, kWave), ceil_div(a.nv, R * kWavesPerBlock), nbatch * p.nview);
 #define XR_CELL(RR, BB) \
-    hipLaunchKernelGGL((k_cell<OPK, GEO, RR, BB>), grid, dim3(kBlock), 0, s, p.d_hdr, p.d_lineC, p.d_col, p.d_csig, WX, \
+    hipLaunchKernelGGL((k_cell<OPK, GEO, RR, BB>), grid, dim3(kBlock), 0, s, p.d_hdr, p.d_lineC, p.d_col, p.d_cgrp, WX, \
                        WY, out, a)
     if (p.big) XR_CELL(8, true);
     else {

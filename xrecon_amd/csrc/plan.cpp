@@ -93,6 +93,8 @@ int build_tables(Plan3d &p, const float *angles, PlanTables &t, std::string &err
     p.any_x = p.any_y = 0;
     p.big = ((int64_t)(nz + 1) * nx * ny * 8 >= ((int64_t)1 << 32) || (int64_t)nx * ny * 8 >= ((int64_t)1 << 24)) ? 1 : 0;
 
+    const int ncg = ceil_div(nu, kWave);
+    std::vector<int> gspan((size_t)V * ncg, 0);
     std::vector<double> d(nu + 1), sig(nu);
     for (int v = 0; v < V; ++v) {
         const float af = angles[v];
@@ -216,15 +218,24 @@ int build_tables(Plan3d &p, const float *angles, PlanTables &t, std::string &err
         t.col[cb0 + nu] = float4{(float)d[nu], 0.0f, 0.0f, 0.0f};
         h.dlo = (float)d[0];
         h.dhi = (float)d[nu];
-        // Narrow view: the widest column, measured in pixel widths of the line with the
-        // smallest pixel footprint, stays below one, so a column overlaps at most two
-        // consecutive pixels of any line (the cell gather then takes a fixed two-pixel
-        // path).  A small margin keeps FP32 rounding of the boundaries on the safe side.
+        // Span class: the widest column, measured in pixel widths of the line with the
+        // smallest pixel footprint, below one (below two): a column overlaps at most two
+        // (three) consecutive pixels of any line, and the cell gather takes a fixed two-
+        // (three-) pixel path.  A small margin keeps FP32 rounding of the boundaries on the
+        // safe side.  Classified per 64-column group of the cell gather (the view's own
+        // class is the widest).
         {
-            double wmax = 0.0, dmin = 1e300;
-            for (int c = 0; c < nu; ++c) wmax = std::max(wmax, d[c + 1] - d[c]);
+            double dmin = 1e300;
             for (int L = 0; L < h.nline; ++L) dmin = std::min(dmin, 1.0 / (double)t.lineC[(size_t)v * ML + L].y);
-            h.span = (wmax < 0.99 * dmin) ? 2 : (wmax < 1.99 * dmin) ? 3 : 0;
+            auto cls = [dmin](double w) { return (w < 0.99 * dmin) ? 2 : (w < 1.99 * dmin) ? 3 : 0; };
+            double wmax = 0.0;
+            for (int g = 0; g < ncg; ++g) {
+                double wg = 0.0;
+                for (int c = g * kWave; c < std::min(nu, (g + 1) * kWave); ++c) wg = std::max(wg, d[c + 1] - d[c]);
+                wmax = std::max(wmax, wg);
+                gspan[(size_t)v * ncg + g] = cls(wg);
+            }
+            h.span = cls(wmax);
         }
 
         // Inverse map axis point -> fractional internal column, used by the voxel kernels
@@ -262,9 +273,9 @@ int build_tables(Plan3d &p, const float *angles, PlanTables &t, std::string &err
         p.vsplit = std::max(1, std::min(sp, V));
     }
 
-    // Cell-gather staging: depth-scale range of each wave's 64 columns per view.
-    const int ncg = ceil_div(nu, kWave);
-    t.csig.assign((size_t)V * ncg, float2{0.0f, 0.0f});
+    // Cell-gather column groups: depth-scale range of each group's columns per view, and
+    // the span class from above.
+    t.cgrp.assign((size_t)V * ncg, CellGroup{0.0f, 0.0f, 0, 0});
     for (int v = 0; v < V; ++v)
         for (int g = 0; g < ncg; ++g) {
             float lo = 3.0e38f, hi = -3.0e38f;
@@ -273,7 +284,7 @@ int build_tables(Plan3d &p, const float *angles, PlanTables &t, std::string &err
                 lo = std::min(lo, sg);
                 hi = std::max(hi, sg);
             }
-            t.csig[(size_t)v * ncg + g] = float2{lo, hi};
+            t.cgrp[(size_t)v * ncg + g] = CellGroup{lo, hi, gspan[(size_t)v * ncg + g], 0};
         }
 
     // Voxel-gather staging estimates per (view, 16 x 16 tile of voxel columns).

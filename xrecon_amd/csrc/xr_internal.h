@@ -35,6 +35,7 @@ struct alignas(16) ViewHdr {
     float sx, sy;    // source position (cone) for the FDK weights
     float dlo, dhi;  // d_0 and d_Nu
     int span;        // 2 / 3: every column overlaps at most 2 / 3 pixels of any line; 0: no bound
+                     // (the widest class over the view's column groups; see CellGroup)
     float pad1;
 };
 static_assert(sizeof(ViewHdr) == 80, "ViewHdr layout");
@@ -49,6 +50,17 @@ struct VoxTile {
     float zk_lo, zk_hi;
 };
 static_assert(sizeof(VoxTile) == 16, "VoxTile layout");
+
+// Per (view, 64-column group of the cell gather): the range of the depth scale sigma over
+// the group's columns (staged variant), and the group's span class -- 2 / 3: each of its
+// columns overlaps at most 2 / 3 consecutive pixels of any line, 0: no bound.  On a
+// diagonal view the edge columns of a flat panel are wider on the axis than the central
+// ones, so most groups of a three-pixel view still take the two-pixel path.
+struct CellGroup {
+    float sig_lo, sig_hi;
+    int span, pad;
+};
+static_assert(sizeof(CellGroup) == 16, "CellGroup layout");
 
 // Geometry family of a plan; selects the column inverse g() and the weights.
 enum GeoKind : int { GEO_PAR = 0, GEO_FLAT = 1, GEO_ARC = 2 };
@@ -78,7 +90,7 @@ struct Plan3d {
     float4 *d_lineV = nullptr;  // [nview][maxline] {P0, Delta, zeta, 1/(Delta zeta)}: voxel kernels
     float4 *d_col = nullptr;    // [nview][Nu+1]    {d, sigma, kappa, W}
     VoxTile *d_vtile = nullptr; // [nview][ceil(Ny/16)][ceil(Nx/16)]: voxel-gather staging estimates
-    float2 *d_csig = nullptr;   // [nview][ceil(Nu/64)] {min, max} of sigma over each 64-column group
+    CellGroup *d_cgrp = nullptr; // [nview][ceil(Nu/64)]: cell-gather column groups
 };
 
 // Host-side tables before upload (kept separate so tests can inspect the builder).
@@ -86,7 +98,7 @@ struct PlanTables {
     std::vector<ViewHdr> hdr;
     std::vector<float4> lineC, lineV, col;
     std::vector<VoxTile> vtile;
-    std::vector<float2> csig;
+    std::vector<CellGroup> cgrp;
 };
 
 // plan.cpp
