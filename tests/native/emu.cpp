@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <cstring>
 #include <string>
+#include <limits>
 #include <thread>
 #include <vector>
 
@@ -18,7 +19,7 @@ using namespace xr;
 namespace {
 
 // Cell-gather variant (see run_cell_RV): 0 = as launched, 1 = general 32-bit, 2 = general
-// 64-bit, 3 = fixed spans without staging.
+// 64-bit, 3 = the staged wave variant for fixed-span groups.
 int g_cell_variant = 0;
 
 // One wave of the staged fixed-span cell gather, lanes in lockstep per line: the block
@@ -54,10 +55,10 @@ void emu_cell_wave(const Plan3d &p, const CellArgs &a, const float2 *WX, const f
         if (ok[lane]) cell_store<OPK, GEO, R>(st[lane], h, a, out, cg * kWave + lane, r0, v, b);
 }
 
-// MODE 0: what the library launches (fixed-span views staged per wave, the rest through
-// the general loop; big plans 64-bit general); 1: general loop, 32-bit offsets, for every
-// view; 2: general loop, 64-bit offsets; 3: fixed-span paths read from the table (no
-// staging).
+// MODE 0: what the library launches (per column group: the group's class, each line of
+// a wider group on the narrowest path it allows; big plans 64-bit general); 1: general
+// loop, 32-bit offsets, on every line; 2: general loop, 64-bit offsets; 3: the staged
+// wave variant (XR_CELL_STAGED) for fixed-span groups.
 template <int OPK, int GEO, int R, int MODE>
 void run_cell_RV(const Plan3d &p, const float2 *WX, const float2 *WY, float *out, int nb, int nthreads)
 {
@@ -65,6 +66,7 @@ void run_cell_RV(const Plan3d &p, const float2 *WX, const float2 *WY, float *out
     const int ngroups = (a.nv + R - 1) / R;
     const int ncg = ceil_div(a.nu, kWave);
     const int items = nb * p.nview;
+    const float inf = std::numeric_limits<float>::infinity();
     std::vector<std::thread> th;
     for (int t = 0; t < nthreads; ++t) {
         th.emplace_back([&, t]() {
@@ -72,30 +74,30 @@ void run_cell_RV(const Plan3d &p, const float2 *WX, const float2 *WY, float *out
                 const int v = it % p.nview, b = it / p.nview;
                 for (int g = 0; g < ngroups; ++g)
                     for (int cg = 0; cg < ncg; ++cg) {
-                        // the kernels choose the variant per (view, column group)
-                        const int span =
-                            (MODE == 0 || MODE == 3) && !p.big ? p.d_cgrp[(int64_t)v * ncg + cg].span : 0;
-                        if (MODE == 0 && span == 2) {
+                        const CellGroup grp = p.d_cgrp[(int64_t)v * ncg + cg];
+                        const int span = (MODE == 0 || MODE == 3) && !p.big ? grp.span : 0;
+                        if (MODE == 3 && span == 2) {
                             emu_cell_wave<OPK, GEO, R, 2>(p, a, WX, WY, out, cg, g * R, v, b);
                             continue;
                         }
-                        if (MODE == 0 && span == 3) {
+                        if (MODE == 3 && span == 3) {
                             emu_cell_wave<OPK, GEO, R, 3>(p, a, WX, WY, out, cg, g * R, v, b);
                             continue;
                         }
+                        const float wg = MODE == 0 && !p.big ? grp.wmax : inf;
                         for (int c = cg * kWave; c < std::min(a.nu, (cg + 1) * kWave); ++c) {
                             if (span == 2)
                                 cell_body<OPK, GEO, R, 2, false>(p.d_hdr, p.d_lineC, p.d_col, WX, WY, out, a, c, g * R,
-                                                                 v, b);
+                                                                 v, b, wg);
                             else if (span == 3)
                                 cell_body<OPK, GEO, R, 3, false>(p.d_hdr, p.d_lineC, p.d_col, WX, WY, out, a, c, g * R,
-                                                                 v, b);
+                                                                 v, b, wg);
                             else if (MODE == 2 || p.big)
                                 cell_body<OPK, GEO, R, 0, true>(p.d_hdr, p.d_lineC, p.d_col, WX, WY, out, a, c, g * R,
-                                                                v, b);
+                                                                v, b, inf);
                             else
                                 cell_body<OPK, GEO, R, 0, false>(p.d_hdr, p.d_lineC, p.d_col, WX, WY, out, a, c, g * R,
-                                                                 v, b);
+                                                                 v, b, wg);
                         }
                     }
             }

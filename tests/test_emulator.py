@@ -97,8 +97,9 @@ def test_volume_outside_field_of_view_gives_zero():
 def test_cell_gather_variants_agree(geo, variant):
     """The general cell-gather path with 32-bit byte offsets (variant 1), with 64-bit
     entry offsets (variant 2, launched for z-prefix tables of 4 GiB and more) and the
-    fixed-span paths reading the table directly (variant 3) give the oracle's results on
-    the same inputs as the wave-staged variant the library launches."""
+    wave-staged fixed-span variant (variant 3, built with XR_CELL_STAGED) give the
+    oracle's results on the same inputs as the per-group, per-line path selection the
+    library launches."""
     opts = CASES3D[geo]
     V = 16
     ang = angles_with_ties(V)

@@ -148,25 +148,26 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
     const int v = bz % a.nview, b = bz / a.nview;
     // the variant is chosen per (view, column group), uniform over the workgroup; a
     // batch holding a big plan runs every instance through the 64-bit general path
-    const int span = BIG ? 0 : d.cgrp[(int64_t)v * gx + bx].span;
+    const CellGroup grp = d.cgrp[(int64_t)v * gx + bx];
+    const int span = BIG ? 0 : grp.span;
     if (BIG) {
-        if (c < a.nu) cell_body<OPK, GEO, R, 0, true>(d.hdr, d.lineC, d.col, WX, WY, io.out[i], a, c, r0, v, b);
+        if (c < a.nu) cell_body<OPK, GEO, R, 0, true>(d.hdr, d.lineC, d.col, WX, WY, io.out[i], a, c, r0, v, b, INFINITY);
     } else if (span == 2) {
 #if XR_CELL_STAGED
         cell_wave<OPK, GEO, R, 2>(d.hdr, d.lineC, d.col, d.cgrp, WX, WY, io.out[i], a, c, c < a.nu, bx, r0, v, b,
                                   lds[wave]);
 #else
-        if (c < a.nu) cell_body<OPK, GEO, R, 2, false>(d.hdr, d.lineC, d.col, WX, WY, io.out[i], a, c, r0, v, b);
+        if (c < a.nu) cell_body<OPK, GEO, R, 2, false>(d.hdr, d.lineC, d.col, WX, WY, io.out[i], a, c, r0, v, b, grp.wmax);
 #endif
     } else if (span == 3) {
 #if XR_CELL_STAGED
         cell_wave<OPK, GEO, R, 3>(d.hdr, d.lineC, d.col, d.cgrp, WX, WY, io.out[i], a, c, c < a.nu, bx, r0, v, b,
                                   lds[wave]);
 #else
-        if (c < a.nu) cell_body<OPK, GEO, R, 3, false>(d.hdr, d.lineC, d.col, WX, WY, io.out[i], a, c, r0, v, b);
+        if (c < a.nu) cell_body<OPK, GEO, R, 3, false>(d.hdr, d.lineC, d.col, WX, WY, io.out[i], a, c, r0, v, b, grp.wmax);
 #endif
     } else if (c < a.nu) {
-        cell_body<OPK, GEO, R, 0, false>(d.hdr, d.lineC, d.col, WX, WY, io.out[i], a, c, r0, v, b);
+        cell_body<OPK, GEO, R, 0, false>(d.hdr, d.lineC, d.col, WX, WY, io.out[i], a, c, r0, v, b, grp.wmax);
     }
 }
 

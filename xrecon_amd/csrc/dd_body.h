@@ -670,12 +670,15 @@ __host__ __device__ inline const float2 *cell_table(const float2 *WX, const floa
 }
 
 // One thread, reading the tables in HBM only: internal column c, rows r0..r0+R-1 of view
-// v, batch b.  SPAN / BIG as for cell_line.
+// v, batch b.  SPAN: the class of the thread's column group (CellGroup::span), whose
+// widest column spans wgrp on the axis; in a group of class 3 or 0 each line takes the
+// narrowest fixed path its pixel footprint allows (wgrp = +inf: the general loop on every
+// line).  BIG as for cell_line.
 template <int OPK, int GEO, int R, int SPAN, bool BIG>
 __host__ __device__ inline void cell_body(const ViewHdr *__restrict__ hdr, const float4 *__restrict__ lineC,
                                           const float4 *__restrict__ colt, const float2 *__restrict__ WX,
                                           const float2 *__restrict__ WY, float *__restrict__ out, const CellArgs &a,
-                                          int c, int r0, int v, int b)
+                                          int c, int r0, int v, int b, float wgrp)
 {
     const ViewHdr h = hdr[v];
     CellLane<R> st;
@@ -683,7 +686,18 @@ __host__ __device__ inline void cell_body(const ViewHdr *__restrict__ hdr, const
     const float2 *W = cell_table(WX, WY, h, a, b);
     const float4 *lt = lineC + (int64_t)v * a.maxline;
     const CellBox none{0, 0, 0, 0};
-    for (int L = 0; L < h.nline; ++L) cell_line<OPK, GEO, R, SPAN, BIG>(st, h, a, lt[L], L, W, none, nullptr);
+    for (int L = 0; L < h.nline; ++L) {
+        const float4 lc = lt[L];
+        if (SPAN == 2) {
+            cell_line<OPK, GEO, R, 2, BIG>(st, h, a, lc, L, W, none, nullptr);
+            continue;
+        }
+        // wgrp / Delta_L: the group's widest column in pixels of this line (wave-uniform)
+        const float s = wgrp * lc.y;
+        if (s < 0.99f) cell_line<OPK, GEO, R, 2, BIG>(st, h, a, lc, L, W, none, nullptr);
+        else if (SPAN == 3 || s < 1.99f) cell_line<OPK, GEO, R, 3, BIG>(st, h, a, lc, L, W, none, nullptr);
+        else cell_line<OPK, GEO, R, 0, BIG>(st, h, a, lc, L, W, none, nullptr);
+    }
     cell_store<OPK, GEO, R>(st, h, a, out, c, r0, v, b);
 }
 

@@ -230,19 +230,21 @@ __global__ __launch_bounds__(kBlock) void k_cell(const ViewHdr *__restrict__ hdr
     const int b = blockIdx.z / a.nview;
     if (r0 >= a.nv) return;  // the whole wave
     // the variant is chosen per (view, column group): uniform over the workgroup
-    const int span = BIG ? 0 : cgrp[(int64_t)v * gridDim.x + blockIdx.x].span;
+    const CellGroup grp = cgrp[(int64_t)v * gridDim.x + blockIdx.x];
+    const int span = BIG ? 0 : grp.span;
+    const float wgrp = BIG ? INFINITY : grp.wmax;
 #if XR_CELL_STAGED
     if (span == 2)
         cell_wave<OPK, GEO, R, 2>(hdr, lineC, colt, cgrp, WX, WY, out, a, c, c < a.nu, blockIdx.x, r0, v, b, lds[wave]);
     else if (span == 3)
         cell_wave<OPK, GEO, R, 3>(hdr, lineC, colt, cgrp, WX, WY, out, a, c, c < a.nu, blockIdx.x, r0, v, b, lds[wave]);
     else if (c < a.nu)
-        cell_body<OPK, GEO, R, 0, BIG>(hdr, lineC, colt, WX, WY, out, a, c, r0, v, b);
+        cell_body<OPK, GEO, R, 0, BIG>(hdr, lineC, colt, WX, WY, out, a, c, r0, v, b, wgrp);
 #else
     if (c >= a.nu) return;
-    if (span == 2) cell_body<OPK, GEO, R, 2, false>(hdr, lineC, colt, WX, WY, out, a, c, r0, v, b);
-    else if (span == 3) cell_body<OPK, GEO, R, 3, false>(hdr, lineC, colt, WX, WY, out, a, c, r0, v, b);
-    else cell_body<OPK, GEO, R, 0, BIG>(hdr, lineC, colt, WX, WY, out, a, c, r0, v, b);
+    if (span == 2) cell_body<OPK, GEO, R, 2, false>(hdr, lineC, colt, WX, WY, out, a, c, r0, v, b, wgrp);
+    else if (span == 3) cell_body<OPK, GEO, R, 3, false>(hdr, lineC, colt, WX, WY, out, a, c, r0, v, b, wgrp);
+    else cell_body<OPK, GEO, R, 0, BIG>(hdr, lineC, colt, WX, WY, out, a, c, r0, v, b, wgrp);
 #endif
 }
 

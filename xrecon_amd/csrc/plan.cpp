@@ -95,6 +95,7 @@ int build_tables(Plan3d &p, const float *angles, PlanTables &t, std::string &err
 
     const int ncg = ceil_div(nu, kWave);
     std::vector<int> gspan((size_t)V * ncg, 0);
+    std::vector<double> gwmax((size_t)V * ncg, 0.0);
     std::vector<double> d(nu + 1), sig(nu);
     for (int v = 0; v < V; ++v) {
         const float af = angles[v];
@@ -234,6 +235,7 @@ int build_tables(Plan3d &p, const float *angles, PlanTables &t, std::string &err
                 for (int c = g * kWave; c < std::min(nu, (g + 1) * kWave); ++c) wg = std::max(wg, d[c + 1] - d[c]);
                 wmax = std::max(wmax, wg);
                 gspan[(size_t)v * ncg + g] = cls(wg);
+                gwmax[(size_t)v * ncg + g] = wg;
             }
             h.span = cls(wmax);
         }
@@ -275,7 +277,7 @@ int build_tables(Plan3d &p, const float *angles, PlanTables &t, std::string &err
 
     // Cell-gather column groups: depth-scale range of each group's columns per view, and
     // the span class from above.
-    t.cgrp.assign((size_t)V * ncg, CellGroup{0.0f, 0.0f, 0, 0});
+    t.cgrp.assign((size_t)V * ncg, CellGroup{0.0f, 0.0f, 0, 0.0f});
     for (int v = 0; v < V; ++v)
         for (int g = 0; g < ncg; ++g) {
             float lo = 3.0e38f, hi = -3.0e38f;
@@ -284,7 +286,7 @@ int build_tables(Plan3d &p, const float *angles, PlanTables &t, std::string &err
                 lo = std::min(lo, sg);
                 hi = std::max(hi, sg);
             }
-            t.cgrp[(size_t)v * ncg + g] = CellGroup{lo, hi, gspan[(size_t)v * ncg + g], 0};
+            t.cgrp[(size_t)v * ncg + g] = CellGroup{lo, hi, gspan[(size_t)v * ncg + g], (float)gwmax[(size_t)v * ncg + g]};
         }
 
     // Voxel-gather staging estimates per (view, 16 x 16 tile of voxel columns).

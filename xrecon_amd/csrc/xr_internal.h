@@ -52,13 +52,16 @@ struct VoxTile {
 static_assert(sizeof(VoxTile) == 16, "VoxTile layout");
 
 // Per (view, 64-column group of the cell gather): the range of the depth scale sigma over
-// the group's columns (staged variant), and the group's span class -- 2 / 3: each of its
-// columns overlaps at most 2 / 3 consecutive pixels of any line, 0: no bound.  On a
-// diagonal view the edge columns of a flat panel are wider on the axis than the central
-// ones, so most groups of a three-pixel view still take the two-pixel path.
+// the group's columns (staged variant), the group's span class -- 2 / 3: each of its
+// columns overlaps at most 2 / 3 consecutive pixels of any line, 0: no bound -- and its
+// widest column on the axis, wmax.  On a diagonal view the edge columns of a flat panel
+// are wider on the axis than the central ones, so most groups of a three-pixel view
+// still take the two-pixel path; within a group, a line whose pixel footprint Delta_L
+// exceeds wmax / 0.99 (wmax / 1.99) takes the two- (three-) pixel path.
 struct CellGroup {
     float sig_lo, sig_hi;
-    int span, pad;
+    int span;
+    float wmax;
 };
 static_assert(sizeof(CellGroup) == 16, "CellGroup layout");
 
