@@ -274,11 +274,22 @@ def bench_cfg5(dev, reps):
             "tiles": "defaults (8 rows, 16 slices)"}
 
 
-def traffic_from_profiles(kernel: str):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 summaries: a line
-    "traffic <kernel> fetch_bytes=F write_bytes=W total_bytes=T source=<pmc runs>" in
-    profiles/*.txt, written from FETCH_SIZE / WRITE_SIZE passes scaled by the copy-kernel
-    calibration.  None when no summary covers this kernel."""
+def traffic_from_profiles(kernel):
+    """HBM bytes per launch of `kernel` (a symbol, or a list of symbols whose launches
+    together make one pass: their bytes add) from the committed rocprofv3 summaries: a
+    line "traffic <kernel> fetch_bytes=F write_bytes=W total_bytes=T source=<pmc runs>"
+    in profiles/*.txt, written from FETCH_SIZE / WRITE_SIZE passes scaled by the
+    copy-kernel calibration.  None when a summary is missing for any of them."""
+    if isinstance(kernel, (list, tuple)):
+        total, srcs = 0.0, []
+        for k in kernel:
+            t, src = traffic_from_profiles(k)
+            if t is None:
+                return None, None
+            total += t
+            if src not in srcs:
+                srcs.append(src)
+        return total, ", ".join(srcs)
     pdir = os.path.join(ROOT, "profiles")
     if not os.path.isdir(pdir):
         return None, None
@@ -440,7 +451,15 @@ def main():
     nnz = nnz_of(WORKLOAD)
     flops = 4.0 * nnz
     if t_cell >= t_vox:
-        dominant, t_dom, symbol = "cell_gather(A)", t_cell, f"k_cell<0, 1, {tune_a[0]}, false>"
+        # one launch per span class of the column groups (two-pixel at the tuned rows,
+        # three-pixel and general at most 8 rows)
+        ncls = plan.cell_classes()
+        rows = (tune_a[0], min(tune_a[0], 8), min(tune_a[0], 8))
+        if sum(ncls) * -(-wl["Nv"] // (4 * tune_a[0])) < 4096:  # small grids: one launch for all classes
+            symbol = [f"k_cell<0, 1, {tune_a[0]}, -1, false>"]
+        else:
+            symbol = [f"k_cell<0, 1, {r}, {span}, false>" for r, span, n in zip(rows, (2, 3, 0), ncls) if n > 0]
+        dominant, t_dom = "cell_gather(A)", t_cell
     else:
         dominant, t_dom, symbol = "voxel_gather(B)", t_vox, f"k_voxel<0, 1, {tune_b[1]}>"
     achieved = flops / (t_dom * 1e-3) / 1e12

@@ -144,6 +144,11 @@ int xr_plan3d_kernel_kind(const xr_plan3d *plan, int op);
 int xr_plan3d_num_view(const xr_plan3d *plan);
 int xr_plan3d_view_flags(const xr_plan3d *plan, int *xdriven, int *keep_order);
 
+/* Number of (view, 64-column group) pairs of the cell gathers (A, B^T, B_W^T) in each span
+ * class: counts[0] two-pixel, counts[1] three-pixel, counts[2] general.  The cell gather
+ * launches one kernel per non-empty class.  Introspection only; no reference counterpart. */
+int xr_plan3d_cell_classes(const xr_plan3d *plan, int counts[3]);
+
 /* ------------------------------------------ batched uneven instances (one launch) */
 
 /* A batch groups up to XR_BATCH_MAX independent plans of one geometry family (uneven

@@ -95,6 +95,45 @@ def test_tilings_agree(cuda, tile_rows, voxel_chunk):
     assert rel_l2(outs[2], want_b) <= 1e-5
 
 
+def test_cell_classes_mixed_groups(cuda):
+    """A wide flat panel whose diagonal views mix two- and three-pixel column groups: the
+    cell gathers launch one kernel per class and every operator matches the oracle."""
+    from helpers import opts3d
+
+    C = _native()
+    opts = opts3d(64, 60, 8, 160, 8, 1.0, 1.0, 1.0, 1.2, 1.0, 500.0, 1000.0, 1, 0.2, -0.1, 0.1, 0.2, 0.1)
+    V = 48
+    ang = angles_with_ties(V)
+    plan = C.Plan([float(v) for v in opts], torch.from_numpy(ang), 0, False)
+    n2, n3, n0 = plan.cell_classes()
+    assert n2 + n3 + n0 == V * 3 and n2 > 0 and n3 > 0
+    rng = np.random.default_rng(17)
+    vol, _ = shapes3d(opts, 2, V)
+    x = rng.random(vol).astype(np.float32)
+    for op in ("A", "BT", "BWT"):
+        print(f"[gpu] mixed cell classes {op}", flush=True)
+        got = gpu_op(op, x, ang, opts, cuda)
+        assert rel_l2(got, oracle.op3d(op, x.astype(np.float64), ang, opts)) <= 1e-5, op
+
+
+def test_cell_general_class_split_launch(cuda):
+    """A coarse detector (every column group in the general class) with enough views that
+    the cell gathers launch per class rather than in one mixed grid."""
+    from helpers import opts3d
+
+    C = _native()
+    opts = opts3d(20, 18, 6, 12, 5, 0.5, 0.5, 0.5, 4.0, 2.0, 500.0, 1000.0, 1, 0.1, 0.0, 0.0, 0.3, 0.1)
+    V = 4200
+    ang = uniform_angles(V)
+    plan = C.Plan([float(v) for v in opts], torch.from_numpy(ang), 0, False)
+    assert plan.cell_classes() == [0, 0, V]
+    x = np.random.default_rng(19).random(shapes3d(opts, 1, V)[0]).astype(np.float32)
+    for op in ("A", "BWT"):
+        print(f"[gpu] general-class split launch {op}", flush=True)
+        got = gpu_op(op, x, ang, opts, cuda)
+        assert rel_l2(got, oracle.op3d(op, x.astype(np.float64), ang, opts)) <= 1e-5, op
+
+
 # ------------------------------------------------------------------ full size (cfg 2)
 
 CFG2 = [256, 256, 64, 512, 64, 1.0, 1.0, 1.0, 1.0, 1.0, 500.0, 1000.0, 0, 0, 0, 0, 0, 1]

@@ -29,6 +29,8 @@ int validate(const xr_volume3d *vol, const xr_geometry3d *geo, int num_view)
     if (vol->nx <= 0 || vol->ny <= 0 || vol->nz <= 0) return fail(XR_EINVAL, "volume sizes must be positive");
     if (geo->nu <= 0 || geo->nv <= 0) return fail(XR_EINVAL, "detector sizes must be positive");
     if (num_view < 0) return fail(XR_EINVAL, "negative number of views");
+    if ((int64_t)num_view * ((geo->nu + 63) / 64) >= ((int64_t)1 << 31))
+        return fail(XR_EINVAL, "too many views for this detector width (views x ceil(Nu/64) must stay below 2^31)");
     if (!finite_pos(vol->dx) || !finite_pos(vol->dy) || !finite_pos(vol->dz))
         return fail(XR_EINVAL, "voxel spacings must be positive and finite");
     if (!finite_pos(geo->du) || !finite_pos(geo->dv))
@@ -89,10 +91,12 @@ void free_tables(Plan3d &p)
     if (p.d_col) (void)hipFree(p.d_col);
     if (p.d_vtile) (void)hipFree(p.d_vtile);
     if (p.d_cgrp) (void)hipFree(p.d_cgrp);
+    if (p.d_citems) (void)hipFree(p.d_citems);
     p.d_hdr = nullptr;
     p.d_lineC = p.d_lineV = p.d_col = nullptr;
     p.d_vtile = nullptr;
     p.d_cgrp = nullptr;
+    p.d_citems = nullptr;
 }
 
 }  // namespace
@@ -128,7 +132,7 @@ int xr_plan3d_create(const xr_volume3d *vol, const xr_geometry3d *geo, const flo
     DeviceGuard g(device);
     if ((rc = upload(&p.d_hdr, t.hdr)) || (rc = upload(&p.d_lineC, t.lineC)) || (rc = upload(&p.d_lineV, t.lineV)) ||
         (rc = upload(&p.d_col, t.col)) || (rc = upload(&p.d_vtile, t.vtile)) ||
-        (rc = upload(&p.d_cgrp, t.cgrp))) {
+        (rc = upload(&p.d_cgrp, t.cgrp)) || (rc = upload(&p.d_citems, t.citems))) {
         free_tables(p);
         return rc;
     }
@@ -188,6 +192,13 @@ int xr_plan3d_kernel_kind(const xr_plan3d *plan, int op)
 {
     if (!plan) return fail(XR_EINVAL, "null plan");
     return xr::kernel_kind(plan->p, op);
+}
+
+int xr_plan3d_cell_classes(const xr_plan3d *plan, int counts[3])
+{
+    if (!plan || !counts) return fail(XR_EINVAL, "null plan or counts");
+    for (int k = 0; k < 3; ++k) counts[k] = plan->p.ncls[k];
+    return XR_OK;
 }
 
 int xr_plan3d_view_flags(const xr_plan3d *plan, int *xdriven, int *keep_order)

@@ -213,9 +213,16 @@ __global__ __launch_bounds__(kBlock) void k_fdk_prefix(const ViewHdr *__restrict
     fdk_prefix_body<MODE, GEO>(hdr, colt, Q, a, blockIdx.z, blockIdx.y, c);
 }
 
-template <int OPK, int GEO, int R, bool BIG>
+// Cell gather over the column groups of one span class: workgroup blockIdx.x takes group
+// items[blockIdx.x % nitems] (view v, 64 columns) of batch item blockIdx.x / nitems, and
+// blockIdx.y * 4 + wave the rows r0 .. r0+R-1.  SPAN = the class (2, 3 or 0), so each
+// class runs with the registers of its own paths; SPAN = -1: every class in one launch,
+// each workgroup reading its group's class (small problems, where launches cost more
+// than registers).
+template <int OPK, int GEO, int R, int SPAN, bool BIG>
 __global__ __launch_bounds__(kBlock) void k_cell(const ViewHdr *__restrict__ hdr, const float4 *__restrict__ lineC,
                                                  const float4 *__restrict__ colt, const CellGroup *__restrict__ cgrp,
+                                                 const int *__restrict__ items, int nitems,
                                                  const float2 *__restrict__ WX, const float2 *__restrict__ WY,
                                                  float *__restrict__ out, CellArgs a)
 {
@@ -224,28 +231,35 @@ __global__ __launch_bounds__(kBlock) void k_cell(const ViewHdr *__restrict__ hdr
 #endif
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = threadIdx.x / kWave;
-    const int c = blockIdx.x * kWave + lane;
+    const int ncg = ceil_div(a.nu, kWave);
+    const int item = items[blockIdx.x % nitems];
+    const int b = blockIdx.x / nitems;
+    const int v = item / ncg, cg = item - v * ncg;
+    const int c = cg * kWave + lane;
     const int r0 = (blockIdx.y * kWavesPerBlock + wave) * R;
-    const int v = blockIdx.z % a.nview;
-    const int b = blockIdx.z / a.nview;
     if (r0 >= a.nv) return;  // the whole wave
-    // the variant is chosen per (view, column group): uniform over the workgroup
-    const CellGroup grp = cgrp[(int64_t)v * gridDim.x + blockIdx.x];
-    const int span = BIG ? 0 : grp.span;
-    const float wgrp = BIG ? INFINITY : grp.wmax;
+    const float wgrp = BIG ? INFINITY : cgrp[item].wmax;
 #if XR_CELL_STAGED
-    if (span == 2)
-        cell_wave<OPK, GEO, R, 2>(hdr, lineC, colt, cgrp, WX, WY, out, a, c, c < a.nu, blockIdx.x, r0, v, b, lds[wave]);
-    else if (span == 3)
-        cell_wave<OPK, GEO, R, 3>(hdr, lineC, colt, cgrp, WX, WY, out, a, c, c < a.nu, blockIdx.x, r0, v, b, lds[wave]);
-    else if (c < a.nu)
-        cell_body<OPK, GEO, R, 0, BIG>(hdr, lineC, colt, WX, WY, out, a, c, r0, v, b, wgrp);
-#else
+    {
+        const int span = SPAN >= 0 ? SPAN : cgrp[item].span;
+        if (span == 2 || span == 3) {
+            if (span == 2)
+                cell_wave<OPK, GEO, R, 2>(hdr, lineC, colt, cgrp, WX, WY, out, a, c, c < a.nu, cg, r0, v, b, lds[wave]);
+            else
+                cell_wave<OPK, GEO, R, 3>(hdr, lineC, colt, cgrp, WX, WY, out, a, c, c < a.nu, cg, r0, v, b, lds[wave]);
+            return;
+        }
+    }
+#endif
     if (c >= a.nu) return;
+    if (SPAN >= 0) {
+        cell_body<OPK, GEO, R, (SPAN >= 0 ? SPAN : 0), BIG>(hdr, lineC, colt, WX, WY, out, a, c, r0, v, b, wgrp);
+        return;
+    }
+    const int span = cgrp[item].span;
     if (span == 2) cell_body<OPK, GEO, R, 2, false>(hdr, lineC, colt, WX, WY, out, a, c, r0, v, b, wgrp);
     else if (span == 3) cell_body<OPK, GEO, R, 3, false>(hdr, lineC, colt, WX, WY, out, a, c, r0, v, b, wgrp);
-    else cell_body<OPK, GEO, R, 0, BIG>(hdr, lineC, colt, WX, WY, out, a, c, r0, v, b, wgrp);
-#endif
+    else cell_body<OPK, GEO, R, 0, false>(hdr, lineC, colt, WX, WY, out, a, c, r0, v, b, wgrp);
 }
 
 template <int OPK, int GEO, int KC>
@@ -278,26 +292,53 @@ __global__ __launch_bounds__(kBlock) void k_split_sum(const float *__restrict__ 
     }
 }
 
+template <int OPK, int GEO, int SPAN, bool BIG>
+void launch_cell_class(const Plan3d &p, const CellArgs &a, int R, const int *items, int nitems, const float2 *WX,
+                       const float2 *WY, float *out, int nbatch, hipStream_t s)
+{
+    if (nitems == 0) return;
+    const dim3 grid((unsigned)((int64_t)nitems * nbatch), ceil_div(a.nv, R * kWavesPerBlock));
+#define XR_CELL(RR)                                                                                                  \
+    hipLaunchKernelGGL((k_cell<OPK, GEO, RR, SPAN, BIG>), grid, dim3(kBlock), 0, s, p.d_hdr, p.d_lineC, p.d_col,    \
+                       p.d_cgrp, items, nitems, WX, WY, out, a)
+    switch (R) {
+    case 1: XR_CELL(1); break;
+    case 2: XR_CELL(2); break;
+    case 4: XR_CELL(4); break;
+    case 16:
+        if constexpr (SPAN == 2 || SPAN == -1) XR_CELL(16);  // the wider classes run at most 8 rows
+        else XR_CELL(8);
+        break;
+    default: XR_CELL(8); break;
+    }
+#undef XR_CELL
+}
+
+// One launch per span class.  tile_rows R applies to the two-pixel class; the wider
+// classes carry more registers per row and run at most 8 rows per thread.  Grids of
+// fewer than kCellSplitMin workgroups (2D images, small volumes) take one launch for
+// every class instead.
+constexpr int64_t kCellSplitMin = 4096;
+
 template <int OPK, int GEO>
 void launch_cell(const Plan3d &p, int R, const float2 *WX, const float2 *WY, float *out, int nbatch, hipStream_t s)
 {
     const CellArgs a = make_cell_args(p, nbatch);
-    if (p.big) R = 8;  // tables of 4 GiB and more: one 64-bit general variant
-    dim3 grid(ceil_div(a.nu, kWave), ceil_div(a.nv, R * kWavesPerBlock), nbatch * p.nview);
-#define XR_CELL(RR, BB) \
-    hipLaunchKernelGGL((k_cell<OPK, GEO, RR, BB>), grid, dim3(kBlock), 0, s, p.d_hdr, p.d_lineC, p.d_col, p.d_cgrp, WX, \
-                       WY, out, a)
-    if (p.big) XR_CELL(8, true);
-    else {
-        switch (R) {
-        case 1: XR_CELL(1, false); break;
-        case 2: XR_CELL(2, false); break;
-        case 4: XR_CELL(4, false); break;
-        case 16: XR_CELL(16, false); break;
-        default: XR_CELL(8, false); break;
-        }
+    const int ntot = p.ncls[0] + p.ncls[1] + p.ncls[2];
+    if (p.big) {
+        // tables of 4 GiB and more: one 64-bit general variant over every group
+        launch_cell_class<OPK, GEO, 0, true>(p, a, 8, p.d_citems, ntot, WX, WY, out, nbatch, s);
+        return;
     }
-#undef XR_CELL
+    if ((int64_t)ntot * nbatch * ceil_div(a.nv, R * kWavesPerBlock) < kCellSplitMin) {
+        launch_cell_class<OPK, GEO, -1, false>(p, a, R, p.d_citems, ntot, WX, WY, out, nbatch, s);
+        return;
+    }
+    const int Rw = R > 8 ? 8 : R;
+    launch_cell_class<OPK, GEO, 2, false>(p, a, R, p.d_citems, p.ncls[0], WX, WY, out, nbatch, s);
+    launch_cell_class<OPK, GEO, 3, false>(p, a, Rw, p.d_citems + p.ncls[0], p.ncls[1], WX, WY, out, nbatch, s);
+    launch_cell_class<OPK, GEO, 0, false>(p, a, Rw, p.d_citems + p.ncls[0] + p.ncls[1], p.ncls[2], WX, WY, out,
+                                          nbatch, s);
 }
 
 template <int OPK, int GEO>
@@ -472,6 +513,9 @@ int execute_phase(const Plan3d &p, int op, int phase, const float *in, float *ou
     if (!valid_rows(tile_rows)) return fail(XR_EINVAL, "tile_rows must be 1, 2, 4, 8 or 16");
     if (!valid_chunk(voxel_chunk)) return fail(XR_EINVAL, "voxel_chunk must be 1, 4, 8 or 16");
 
+    if (rt.cell && (int64_t)(p.ncls[0] + p.ncls[1] + p.ncls[2]) * nbatch >= ((int64_t)1 << 31))
+        return fail(XR_EINVAL, "batch too large for one launch of the cell gather (views x column groups x batch "
+                               "must stay below 2^31)");
     if (rt.cell) {
         float2 *WX = static_cast<float2 *>(ws);
         float2 *WY = p.any_y ? reinterpret_cast<float2 *>(static_cast<char *>(ws) + align256(vol_prefix_bytes(p, nbatch)))

@@ -289,6 +289,21 @@ int build_tables(Plan3d &p, const float *angles, PlanTables &t, std::string &err
             t.cgrp[(size_t)v * ncg + g] = CellGroup{lo, hi, gspan[(size_t)v * ncg + g], (float)gwmax[(size_t)v * ncg + g]};
         }
 
+    // The cell gather launches one grid per span class over that class's groups, so that
+    // the narrow classes run with the registers of their own path only.
+    t.citems.clear();
+    const int classes[3] = {2, 3, 0};
+    for (int k = 0; k < 3; ++k) {
+        t.ncls[k] = 0;
+        for (int v = 0; v < V; ++v)
+            for (int g = 0; g < ncg; ++g)
+                if (t.cgrp[(size_t)v * ncg + g].span == classes[k]) {
+                    t.citems.push_back(v * ncg + g);
+                    ++t.ncls[k];
+                }
+        p.ncls[k] = t.ncls[k];
+    }
+
     // Voxel-gather staging estimates per (view, 16 x 16 tile of voxel columns).
     t.vtile.assign((size_t)V * gx * gy, VoxTile{0, 0, 0.0f, 0.0f});
     for (int v = 0; v < V; ++v) {

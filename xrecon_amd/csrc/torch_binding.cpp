@@ -123,6 +123,14 @@ public:
         return k;
     }
 
+    std::vector<int> cell_classes() const
+    {
+        std::vector<int> n(3, 0);
+        const int rc = xr_plan3d_cell_classes(plan_, n.data());
+        if (rc) raise_xr(rc, "xr_plan3d_cell_classes");
+        return n;
+    }
+
     std::vector<int> view_flags() const
     {
         std::vector<int> x(nview_), k(nview_);
@@ -349,6 +357,7 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m)
         .def_property_readonly("handle", &Plan::handle)
         .def("workspace_bytes", &Plan::workspace_bytes)
         .def("view_flags", &Plan::view_flags)
+        .def("cell_classes", &Plan::cell_classes)
         .def("kernel_kind", &Plan::kernel_kind)
         .def("execute", &Plan::execute, py::arg("op"), py::arg("input"), py::arg("stream") = 0)
         .def("fdk", &Plan::fdk, py::arg("input"), py::arg("filter"), py::arg("weight"), py::arg("stream") = 0,
