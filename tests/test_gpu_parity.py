@@ -116,6 +116,30 @@ def test_cell_classes_mixed_groups(cuda):
         assert rel_l2(got, oracle.op3d(op, x.astype(np.float64), ang, opts)) <= 1e-5, op
 
 
+def test_cell_tilings_bitwise_equal(cuda):
+    """Rows per thread change only the work split of the cell gather: every row is
+    computed from its absolute index with the same explicit operations, so all tilings
+    (and the classes' different row counts) give bitwise-equal sinograms."""
+    from helpers import opts3d
+
+    C = _native()
+    opts = opts3d(64, 60, 8, 160, 8, 1.0, 1.0, 1.0, 1.2, 1.0, 500.0, 1000.0, 1, 0.2, -0.1, 0.1, 0.2, 0.1)
+    V = 48
+    ang = angles_with_ties(V)
+    plan = C.Plan([float(v) for v in opts], torch.from_numpy(ang), 0, False)
+    x = torch.from_numpy(np.random.default_rng(23).random(shapes3d(opts, 1, V)[0]).astype(np.float32)).to(cuda)
+    x = x.unsqueeze(1)
+    s = torch.cuda.current_stream().cuda_stream
+    ws = torch.empty(plan.workspace_bytes(0, 1), dtype=torch.uint8, device=cuda)
+    outs = []
+    for R in (1, 2, 4, 8, 16):
+        out = torch.empty((1, 1, V, 8, 160), device=cuda)
+        plan.execute_into(0, x, out, ws, s, R, 0, 0)
+        outs.append(out)
+    for R, o in zip((2, 4, 8, 16), outs[1:]):
+        assert torch.equal(o, outs[0]), f"tile_rows {R} differs from tile_rows 1"
+
+
 def test_cell_general_class_split_launch(cuda):
     """A coarse detector (every column group in the general class) with enough views that
     the cell gathers launch per class rather than in one mixed grid."""

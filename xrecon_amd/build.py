@@ -24,8 +24,12 @@ ARCH = os.environ.get("XRECON_AMD_ARCH", "gfx950")
 
 HIP_SOURCES = ["dd_kernels.hip", "batch.hip", "plan.cpp", "capi.cpp", "sysmat.cpp"]
 # Packed-FP32 instructions are disabled for all device code (they have produced silent
-# wrong results on this hardware generation in some forms).
-HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-shared", "-Wall",
+# wrong results on this hardware generation in some forms).  No implicit contraction of
+# a * b + c: every fused multiply-add in the kernels is an explicit fmaf, so results do not
+# depend on how a template instance or launch shape happened to be scheduled (the tile
+# shapes and the batched path are bitwise equal to the plain one), and the host emulator
+# evaluates the same expressions.
+HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-shared", "-Wall", "-ffp-contract=off",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
 

@@ -28,6 +28,8 @@ struct BatchDesc {
     const float4 *lineC, *lineV, *col;
     const VoxTile *vtile;
     const CellGroup *cgrp;
+    const int *citems;  // the plan's column groups by span class (Plan3d::d_citems)
+    int ncls[3];
     CellArgs ca;
     VoxArgs va;  // nchunk is filled per launch from the launch's voxel chunk
     SinoArgs sa;
@@ -51,10 +53,21 @@ __host__ __device__ inline int blocks_transpose(const BatchDesc &d)
 {
     return d.any_y ? ceil_div(d.ca.nx, 32) * ceil_div(d.ca.ny, 32) * d.nb * (d.ca.nz + 1) : 0;
 }
-__host__ __device__ inline int blocks_cell(const BatchDesc &d, int R)
+// Column groups of span-class index K (0: two-pixel, 1: three-pixel, 2: general, 3: all).
+__host__ __device__ inline int cell_class_items(const BatchDesc &d, int K)
 {
-    return ceil_div(d.ca.nu, kWave) * ceil_div(d.ca.nv, R * kWavesPerBlock) * d.nb * d.ca.nview;
+    return K == 3 ? d.ncls[0] + d.ncls[1] + d.ncls[2] : d.ncls[K];
 }
+__host__ __device__ inline int cell_class_offset(const BatchDesc &d, int K)
+{
+    return K == 1 ? d.ncls[0] : K == 2 ? d.ncls[0] + d.ncls[1] : 0;
+}
+__host__ __device__ inline int blocks_cell(const BatchDesc &d, int R, int K)
+{
+    return cell_class_items(d, K) * ceil_div(d.ca.nv, R * kWavesPerBlock) * d.nb;
+}
+// Rows per thread of span-class index K: R for the two-pixel class, at most 8 otherwise.
+__host__ __device__ inline int cell_class_rows(int R, int K) { return K == 0 || R <= 8 ? R : 8; }
 __host__ __device__ inline int blocks_sino_prefix(const BatchDesc &d)
 {
     return ceil_div(d.sa.nu, kBlock) * d.sa.nview * d.nb;
@@ -125,50 +138,43 @@ __global__ __launch_bounds__(kBlock) void k_transpose_batch(const BatchDesc *__r
     }
 }
 
-template <int OPK, int GEO, int R, bool BIG>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_cell_batch(const BatchDesc *__restrict__ D, int n, IOPtrs io,
-                                                       char *__restrict__ ws)
+// Cell gather over the column groups of span-class index K (as cell_class_items) of
+// every instance: the instance's groups in class order, then row groups, then batch
+// items.  One launch per class, as for a single plan.
+template <int OPK, int GEO, int R, int K, bool BIG>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_cell_batch(
+    const BatchDesc *__restrict__ D, int n, IOPtrs io, char *__restrict__ ws)
 {
 #if XR_CELL_STAGED
     __shared__ __attribute__((aligned(16))) float2 lds[kWavesPerBlock][cell_lds_cap<R>()];
 #endif
+    constexpr int SPAN = K == 0 ? 2 : K == 1 ? 3 : 0;
     int bid = blockIdx.x;
-    const int i = find_instance(D, n, bid, [](const BatchDesc &d) { return blocks_cell(d, R); });
+    const int i = find_instance(D, n, bid, [](const BatchDesc &d) { return blocks_cell(d, R, K); });
     if (i < 0) return;
     const BatchDesc &d = D[i];
     const CellArgs a = d.ca;
-    const int gx = ceil_div(a.nu, kWave), gy = ceil_div(a.nv, R * kWavesPerBlock);
-    const int bx = bid % gx, by = (bid / gx) % gy, bz = bid / (gx * gy);
+    const int ncg = ceil_div(a.nu, kWave), gy = ceil_div(a.nv, R * kWavesPerBlock);
+    const int nk = cell_class_items(d, K);
+    const int item = d.citems[cell_class_offset(d, K) + bid % nk];
+    const int rest = bid / nk;
+    const int by = rest % gy, b = rest / gy;
+    const int v = item / ncg, cg = item - v * ncg;
     const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
-    const int c = bx * kWave + lane;
+    const int c = cg * kWave + lane;
     const int r0 = (by * kWavesPerBlock + wave) * R;
     if (r0 >= a.nv) return;  // the whole wave
     const float2 *WX = reinterpret_cast<const float2 *>(ws + d.ws_volx);
     const float2 *WY = reinterpret_cast<const float2 *>(ws + d.ws_voly);
-    const int v = bz % a.nview, b = bz / a.nview;
-    // the variant is chosen per (view, column group), uniform over the workgroup; a
-    // batch holding a big plan runs every instance through the 64-bit general path
-    const CellGroup grp = d.cgrp[(int64_t)v * gx + bx];
-    const int span = BIG ? 0 : grp.span;
-    if (BIG) {
-        if (c < a.nu) cell_body<OPK, GEO, R, 0, true>(d.hdr, d.lineC, d.col, WX, WY, io.out[i], a, c, r0, v, b, INFINITY);
-    } else if (span == 2) {
+    const float wgrp = BIG ? INFINITY : d.cgrp[item].wmax;
 #if XR_CELL_STAGED
-        cell_wave<OPK, GEO, R, 2>(d.hdr, d.lineC, d.col, d.cgrp, WX, WY, io.out[i], a, c, c < a.nu, bx, r0, v, b,
-                                  lds[wave]);
-#else
-        if (c < a.nu) cell_body<OPK, GEO, R, 2, false>(d.hdr, d.lineC, d.col, WX, WY, io.out[i], a, c, r0, v, b, grp.wmax);
-#endif
-    } else if (span == 3) {
-#if XR_CELL_STAGED
-        cell_wave<OPK, GEO, R, 3>(d.hdr, d.lineC, d.col, d.cgrp, WX, WY, io.out[i], a, c, c < a.nu, bx, r0, v, b,
-                                  lds[wave]);
-#else
-        if (c < a.nu) cell_body<OPK, GEO, R, 3, false>(d.hdr, d.lineC, d.col, WX, WY, io.out[i], a, c, r0, v, b, grp.wmax);
-#endif
-    } else if (c < a.nu) {
-        cell_body<OPK, GEO, R, 0, false>(d.hdr, d.lineC, d.col, WX, WY, io.out[i], a, c, r0, v, b, grp.wmax);
+    if (!BIG && (SPAN == 2 || SPAN == 3)) {
+        cell_wave<OPK, GEO, R, (SPAN == 2 ? 2 : 3)>(d.hdr, d.lineC, d.col, d.cgrp, WX, WY, io.out[i], a, c, c < a.nu,
+                                                     cg, r0, v, b, lds[wave]);
+        return;
     }
+#endif
+    if (c < a.nu) cell_body<OPK, GEO, R, SPAN, BIG>(d.hdr, d.lineC, d.col, WX, WY, io.out[i], a, c, r0, v, b, wgrp);
 }
 
 template <int MODE, int GEO>
@@ -228,20 +234,36 @@ __global__ __launch_bounds__(kBlock) void k_split_sum_batch(const BatchDesc *__r
     io.out[i][e] = acc;
 }
 
+template <int OPK, int GEO, int K>
+void launch_cell_batch_class(int R, int grid, const BatchDesc *D, int n, const IOPtrs &io, char *ws, hipStream_t s)
+{
+    if (grid <= 0) return;
+#define XR_CELLB(RR) hipLaunchKernelGGL((k_cell_batch<OPK, GEO, RR, K, false>), dim3(grid), dim3(kBlock), 0, s, D, n, io, ws)
+    switch (R) {
+    case 1: XR_CELLB(1); break;
+    case 2: XR_CELLB(2); break;
+    case 4: XR_CELLB(4); break;
+    case 16:
+        if constexpr (K == 0) XR_CELLB(16);  // the wider classes run at most 8 rows
+        else XR_CELLB(8);
+        break;
+    default: XR_CELLB(8); break;
+    }
+#undef XR_CELLB
+}
+
+// grids[k]: blocks of span-class index k (k = 3: every group, big batches).
 template <int OPK, int GEO>
-void launch_cell_batch(int R, bool big, int grid, const BatchDesc *D, int n, const IOPtrs &io, char *ws, hipStream_t s)
+void launch_cell_batch(int R, bool big, const int64_t *grids, const BatchDesc *D, int n, const IOPtrs &io, char *ws,
+                       hipStream_t s)
 {
     if (big) {
-        hipLaunchKernelGGL((k_cell_batch<OPK, GEO, 8, true>), dim3(grid), dim3(kBlock), 0, s, D, n, io, ws);
+        hipLaunchKernelGGL((k_cell_batch<OPK, GEO, 8, 3, true>), dim3((int)grids[3]), dim3(kBlock), 0, s, D, n, io, ws);
         return;
     }
-    switch (R) {
-    case 1: hipLaunchKernelGGL((k_cell_batch<OPK, GEO, 1, false>), dim3(grid), dim3(kBlock), 0, s, D, n, io, ws); break;
-    case 2: hipLaunchKernelGGL((k_cell_batch<OPK, GEO, 2, false>), dim3(grid), dim3(kBlock), 0, s, D, n, io, ws); break;
-    case 4: hipLaunchKernelGGL((k_cell_batch<OPK, GEO, 4, false>), dim3(grid), dim3(kBlock), 0, s, D, n, io, ws); break;
-    case 16: hipLaunchKernelGGL((k_cell_batch<OPK, GEO, 16, false>), dim3(grid), dim3(kBlock), 0, s, D, n, io, ws); break;
-    default: hipLaunchKernelGGL((k_cell_batch<OPK, GEO, 8, false>), dim3(grid), dim3(kBlock), 0, s, D, n, io, ws); break;
-    }
+    launch_cell_batch_class<OPK, GEO, 0>(R, (int)grids[0], D, n, io, ws, s);
+    launch_cell_batch_class<OPK, GEO, 1>(cell_class_rows(R, 1), (int)grids[1], D, n, io, ws, s);
+    launch_cell_batch_class<OPK, GEO, 2>(cell_class_rows(R, 2), (int)grids[2], D, n, io, ws, s);
 }
 
 template <int OPK, int GEO>
@@ -256,13 +278,14 @@ void launch_voxel_batch(int KC, int grid, const BatchDesc *D, int n, const IOPtr
 }
 
 template <int GEO>
-void dispatch_gather(const Route &rt, int R, bool big, int KC, int grid, const BatchDesc *D, int n, const IOPtrs &io,
-                     char *ws, hipStream_t s)
+void dispatch_gather(const Route &rt, int R, bool big, int KC, const int64_t *grids, const BatchDesc *D, int n,
+                     const IOPtrs &io, char *ws, hipStream_t s)
 {
+    const int grid = (int)grids[0];
     if (rt.cell) {
-        if (rt.kop == CELL_A) launch_cell_batch<CELL_A, GEO>(R, big, grid, D, n, io, ws, s);
-        else if (rt.kop == CELL_BT) launch_cell_batch<CELL_BT, GEO>(R, big, grid, D, n, io, ws, s);
-        else launch_cell_batch<CELL_BWT, GEO>(R, big, grid, D, n, io, ws, s);
+        if (rt.kop == CELL_A) launch_cell_batch<CELL_A, GEO>(R, big, grids, D, n, io, ws, s);
+        else if (rt.kop == CELL_BT) launch_cell_batch<CELL_BT, GEO>(R, big, grids, D, n, io, ws, s);
+        else launch_cell_batch<CELL_BWT, GEO>(R, big, grids, D, n, io, ws, s);
     } else {
         if (rt.kop == VOX_B) launch_voxel_batch<VOX_B, GEO>(KC, grid, D, n, io, ws, s);
         else if (rt.kop == VOX_BW) launch_voxel_batch<VOX_BW, GEO>(KC, grid, D, n, io, ws, s);
@@ -320,6 +343,8 @@ int xr_batch3d_create(xr_plan3d *const *plans, int n, const int *num_batch, xr_b
         d.col = p.d_col;
         d.vtile = p.d_vtile;
         d.cgrp = p.d_cgrp;
+        d.citems = p.d_citems;
+        for (int k = 0; k < 3; ++k) d.ncls[k] = p.ncls[k];
         d.ca = xr::make_cell_args(p, num_batch[i]);
         d.va = xr::make_vox_args(p, num_batch[i], 1);
         d.sa = xr::make_sino_args(p, num_batch[i]);
@@ -406,19 +431,22 @@ int xr_batch3d_execute(const xr_batch3d *b, int op, const float *const *in, floa
     hipStream_t s = static_cast<hipStream_t>(stream);
     char *ws = static_cast<char *>(workspace);
     const xr::BatchDesc *D = b->d_desc;
-    int64_t g_pre = 0, g_tr = 0, g_gather = 0, g_sum = 0;
+    int64_t g_pre = 0, g_tr = 0, g_sum = 0;
+    int64_t g_gather[4] = {0, 0, 0, 0};  // voxel gathers: [0]; cell gathers: per span class, [3] = all (big)
     for (const xr::BatchDesc &d : b->desc) {
         if (rt.cell) {
             g_pre += xr::blocks_vol_prefix(d);
             g_tr += xr::blocks_transpose(d);
-            g_gather += xr::blocks_cell(d, R);
+            for (int k = 0; k < 3; ++k) g_gather[k] += xr::blocks_cell(d, xr::cell_class_rows(R, k), k);
+            g_gather[3] += xr::blocks_cell(d, R, 3);
         } else {
             g_pre += xr::blocks_sino_prefix(d);
-            g_gather += xr::blocks_voxel(d, KC);
+            g_gather[0] += xr::blocks_voxel(d, KC);
             g_sum += xr::blocks_split_sum(d);
         }
     }
-    if (g_pre >= INT32_MAX || g_tr >= INT32_MAX || g_gather >= INT32_MAX || g_sum >= INT32_MAX)
+    if (g_pre >= INT32_MAX || g_tr >= INT32_MAX || g_sum >= INT32_MAX || g_gather[0] >= INT32_MAX ||
+        g_gather[1] >= INT32_MAX || g_gather[2] >= INT32_MAX || g_gather[3] >= INT32_MAX)
         return fail(XR_EINVAL, "batch too large for one launch");
     const int kind = p0.kind;
     if (rt.cell) {
@@ -430,9 +458,9 @@ int xr_batch3d_execute(const xr_batch3d *b, int op, const float *const *in, floa
         else if (kind == xr::GEO_ARC) xr::dispatch_sino_prefix<xr::GEO_ARC>(at, (int)g_pre, D, n, io, ws, s);
         else xr::dispatch_sino_prefix<xr::GEO_PAR>(at, (int)g_pre, D, n, io, ws, s);
     }
-    if (kind == xr::GEO_FLAT) xr::dispatch_gather<xr::GEO_FLAT>(rt, R, big, KC, (int)g_gather, D, n, io, ws, s);
-    else if (kind == xr::GEO_ARC) xr::dispatch_gather<xr::GEO_ARC>(rt, R, big, KC, (int)g_gather, D, n, io, ws, s);
-    else xr::dispatch_gather<xr::GEO_PAR>(rt, R, big, KC, (int)g_gather, D, n, io, ws, s);
+    if (kind == xr::GEO_FLAT) xr::dispatch_gather<xr::GEO_FLAT>(rt, R, big, KC, g_gather, D, n, io, ws, s);
+    else if (kind == xr::GEO_ARC) xr::dispatch_gather<xr::GEO_ARC>(rt, R, big, KC, g_gather, D, n, io, ws, s);
+    else xr::dispatch_gather<xr::GEO_PAR>(rt, R, big, KC, g_gather, D, n, io, ws, s);
     if (g_sum > 0) hipLaunchKernelGGL(xr::k_split_sum_batch, dim3((int)g_sum), dim3(xr::kBlock), 0, s, D, n, io, ws);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(XR_EHIP, std::string("batched launch failed: ") + hipGetErrorString(e));
