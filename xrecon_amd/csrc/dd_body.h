@@ -297,19 +297,17 @@ __host__ __device__ inline float at_scale(const ViewHdr &h, float scale0, float 
     return scale0 / cosw;
 }
 
-// Row-prefix of one sinogram column (b, v, internal column c), in internal column
-// order: Q[b][v][r][c] = (Q_r, q_r), q_r = p[b][v][r][col(c)] * s(c, r),
-// Q_r = sum_{r'<r} q - total/2, r = 0..Nv.
-template <int MODE, int GEO>
-__host__ __device__ inline void sino_prefix_body(const ViewHdr *__restrict__ hdr, const float4 *__restrict__ colt,
-                                                 const float *__restrict__ p, float2 *__restrict__ Q,
-                                                 const SinoArgs &a, int b, int v, int c)
+// Row-prefix table entries of internal column c of view v (header h) from the samples
+// q(r), r = 0 .. Nv-1, written to dst[r * Nu], r = 0 .. Nv: entry r = (B_r, q_r) with
+//   Q_r = sum_{r' < r} q_r' - total / 2   (FP64, centred),   B_r = Q_r - r q_r,
+// so that the row integral up to a fractional row rho in [r, r+1] is
+//   I(rho) = Q_r + (rho - r) q_r = B_r + rho q_r    (one fmaf, no fract),
+// and entry Nv = (Q_Nv, 0).  A^T mode scales each sample by the forward normalisation
+// first.
+template <int MODE, int GEO, class QOf>
+__host__ __device__ inline void prefix_column(const ViewHdr &h, const float4 *__restrict__ colt, const SinoArgs &a,
+                                              int v, int c, QOf qof, float2 *__restrict__ dst)
 {
-    const ViewHdr h = hdr[v];
-    const int col = h.keep ? c : (a.nu - 1 - c);
-    const int64_t sv = (int64_t)b * a.nview + v;
-    const float *src = p + sv * a.nv * a.nu + col;
-    float2 *dst = Q + sv * (int64_t)(a.nv + 1) * a.nu + c;
     float scale0 = 1.0f, dbp = 0.0f, sg = 1.0f;
     if (MODE == SINO_AT) {
         const float4 cc = colt[(int64_t)v * (a.nu + 1) + c];
@@ -320,18 +318,33 @@ __host__ __device__ inline void sino_prefix_body(const ViewHdr *__restrict__ hdr
     }
     double total = 0.0;
     for (int r = 0; r < a.nv; ++r) {
-        float q = src[(int64_t)r * a.nu];
+        float q = qof(r);
         if (MODE == SINO_AT) q *= at_scale<GEO>(h, scale0, dbp, sg, a.v0, a.dvv, r);
         total += (double)q;
     }
     double run = -0.5 * total;
     for (int r = 0; r < a.nv; ++r) {
-        float q = src[(int64_t)r * a.nu];
+        float q = qof(r);
         if (MODE == SINO_AT) q *= at_scale<GEO>(h, scale0, dbp, sg, a.v0, a.dvv, r);
-        dst[(int64_t)r * a.nu] = make_float2((float)run, q);
+        dst[(int64_t)r * a.nu] = make_float2((float)(run - (double)r * (double)q), q);
         run += (double)q;
     }
     dst[(int64_t)a.nv * a.nu] = make_float2((float)run, 0.0f);
+}
+
+// Row-prefix table of one sinogram column (b, v, internal column c), in internal column
+// order, from the sinogram p: q_r = p[b][v][r][col(c)] (times the A^T scale).
+template <int MODE, int GEO>
+__host__ __device__ inline void sino_prefix_body(const ViewHdr *__restrict__ hdr, const float4 *__restrict__ colt,
+                                                 const float *__restrict__ p, float2 *__restrict__ Q,
+                                                 const SinoArgs &a, int b, int v, int c)
+{
+    const ViewHdr h = hdr[v];
+    const int col = h.keep ? c : (a.nu - 1 - c);
+    const int64_t sv = (int64_t)b * a.nview + v;
+    const float *src = p + sv * a.nv * a.nu + col;
+    prefix_column<MODE, GEO>(h, colt, a, v, c, [src, &a](int r) { return src[(int64_t)r * a.nu]; },
+                             Q + sv * (int64_t)(a.nv + 1) * a.nu + c);
 }
 
 // ---------------------------------------------------------------- FDK filter stage
@@ -359,37 +372,6 @@ __host__ __device__ inline double fdk_filter_output(const float *__restrict__ p,
     return acc;
 }
 
-// Row-prefix table entries of internal column c of view v (header h) from the filtered
-// samples q(r), r = 0 .. Nv-1, written to dst[r * Nu]: the same table sino_prefix_body
-// builds from a sinogram (A^T mode scales each sample first).
-template <int MODE, int GEO, class QOf>
-__host__ __device__ inline void fdk_prefix_column(const ViewHdr &h, const float4 *__restrict__ colt, const SinoArgs &a,
-                                                  int v, int c, QOf qof, float2 *__restrict__ dst)
-{
-    float scale0 = 1.0f, dbp = 0.0f, sg = 1.0f;
-    if (MODE == SINO_AT) {
-        const float4 cc = colt[(int64_t)v * (a.nu + 1) + c];
-        const float dn = colt[(int64_t)v * (a.nu + 1) + c + 1].x;
-        scale0 = h.delta / cc.w;
-        dbp = 0.5f * (cc.x + dn) - h.P;
-        sg = cc.y;
-    }
-    double total = 0.0;
-    for (int r = 0; r < a.nv; ++r) {
-        float q = qof(r);
-        if (MODE == SINO_AT) q *= at_scale<GEO>(h, scale0, dbp, sg, a.v0, a.dvv, r);
-        total += (double)q;
-    }
-    double run = -0.5 * total;
-    for (int r = 0; r < a.nv; ++r) {
-        float q = qof(r);
-        if (MODE == SINO_AT) q *= at_scale<GEO>(h, scale0, dbp, sg, a.v0, a.dvv, r);
-        dst[(int64_t)r * a.nu] = make_float2((float)run, q);
-        run += (double)q;
-    }
-    dst[(int64_t)a.nv * a.nu] = make_float2((float)run, 0.0f);
-}
-
 // In-place row-prefix of filtered values already stored in Q[.].y, for internal column
 // c of view v, batch item b.
 template <int MODE, int GEO>
@@ -398,7 +380,7 @@ __host__ __device__ inline void fdk_prefix_body(const ViewHdr *__restrict__ hdr,
 {
     const int64_t sv = (int64_t)b * a.nview + v;
     float2 *dst = Q + sv * (int64_t)(a.nv + 1) * a.nu + c;
-    fdk_prefix_column<MODE, GEO>(hdr[v], colt, a, v, c, [dst, &a](int r) { return dst[(int64_t)r * a.nu].y; }, dst);
+    prefix_column<MODE, GEO>(hdr[v], colt, a, v, c, [dst, &a](int r) { return dst[(int64_t)r * a.nu].y; }, dst);
 }
 
 // ---------------------------------------------------------------- cell gather
@@ -919,12 +901,11 @@ __host__ __device__ inline void voxel_view(const ViewHdr &h, const float4 *__res
                 // All KC+1 boundary lookups are independent: compute the rows, issue the
                 // loads back to back, then consume.
                 int ri[KC + 1];
-                float fr[KC + 1];
+                float rh[KC + 1];
 #pragma unroll
                 for (int q = 0; q <= KC; ++q) {
-                    const float rho = clampf(fmaf((float)q, step, rlo), 0.0f, fnv);
-                    ri[q] = (int)rho;
-                    fr[q] = fractf(rho);
+                    rh[q] = clampf(fmaf((float)q, step, rlo), 0.0f, fnv);
+                    ri[q] = (int)rh[q];
                 }
                 float2 e[KC + 1];
                 // rows increase with q, so the first and last lookups bound the column's rows
@@ -944,10 +925,11 @@ __host__ __device__ inline void voxel_view(const ViewHdr &h, const float4 *__res
                     for (int q = 0; q <= KC; ++q)
                         e[q] = *reinterpret_cast<const float2 *>(Qb + (mul24((unsigned)ri[q], nu8) + c8));
                 }
-                float Ip = fmaf(fr[0], e[0].y, e[0].x);
+                // I(rho) = B_r + rho q_r (see prefix_column)
+                float Ip = fmaf(rh[0], e[0].y, e[0].x);
 #pragma unroll
                 for (int q = 0; q < KC; ++q) {
-                    const float In = fmaf(fr[q + 1], e[q + 1].y, e[q + 1].x);
+                    const float In = fmaf(rh[q + 1], e[q + 1].y, e[q + 1].x);
                     acc[q] = fmaf(w, In - Ip, acc[q]);
                     Ip = In;
                 }

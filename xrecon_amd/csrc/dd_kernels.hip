@@ -17,11 +17,12 @@
 //    the line's column overlaps and their common row boundaries.
 //
 //  * Voxel gathers (B, B_W, A^T -> volume).  A thread owns one voxel column (i, j) and a
-//    chunk of KC slices; a wave is 64 adjacent i.  For each view it finds the detector
-//    columns its footprint overlaps (closed-form inverse of the column map, corrected
-//    against the FP32 table), and for each such column reads the row overlaps of all KC
-//    voxels from a row-prefix table of the sinogram, I(rho) = Q[r] + (rho - r) p[r];
-//    consecutive voxels share their boundary lookups.
+//    chunk of KC slices; a workgroup is a 16 x 16 tile of voxel columns.  For each view
+//    it finds the detector columns its footprint overlaps (closed-form inverse of the
+//    column map, corrected against the FP32 table), and for each such column reads the
+//    row overlaps of all KC voxels from a row-prefix table of the sinogram,
+//    I(rho) = B[r] + rho p[r] with B[r] = Q[r] - r p[r]; consecutive voxels share their
+//    boundary lookups.
 //
 // The per-thread bodies live in dd_body.h.  Geometry comes from per-view FP32 tables
 // computed in FP64 on the host (plan.cpp); all overlap arithmetic is FP32, the prefix
@@ -100,7 +101,7 @@ __global__ __launch_bounds__(kBlock) void k_sino_prefix(const ViewHdr *__restric
 // states it.
 //
 // FUSED (Nv divides kFdkRows, so a row group holds whole views): the workgroup then
-// builds the row-prefix table of its views' columns itself (fdk_prefix_column over the
+// builds the row-prefix table of its views' columns itself (prefix_column over the
 // filtered samples kept in LDS).  Otherwise it writes the filtered samples to Q[.].y and
 // k_fdk_prefix builds the table.
 constexpr int kFdkRows = kWave;
@@ -199,7 +200,7 @@ __global__ __launch_bounds__(kBlock) void k_fdk_filter(const ViewHdr *__restrict
         const ViewHdr h = hdr[v];
         const int c = h.keep ? u : nu - 1 - u;
         const float *qcol = qbuf + vg * a.nv * kQPitch + cl;
-        fdk_prefix_column<MODE, GEO>(h, colt, a, v, c, [qcol](int r) { return qcol[r * kQPitch]; },
+        prefix_column<MODE, GEO>(h, colt, a, v, c, [qcol](int r) { return qcol[r * kQPitch]; },
                                      Q + sv * (int64_t)(a.nv + 1) * nu + c);
     }
 }
