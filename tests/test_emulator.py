@@ -158,6 +158,22 @@ def test_span_classes_per_column_group():
             assert rel_l2(emu_op3d(op, x, sub, opts, cell_variant=variant), want) <= 1e-5, (op, variant)
 
 
+# detector columns about two pixels wide: every group in the general class, its lines on the
+# fixed three- and four-pixel paths
+WIDE_COLUMNS = opts3d(40, 36, 6, 40, 6, 0.5, 0.5, 0.5, 2.2, 1.0, 500.0, 1000.0, 1, 0.1, -0.05, 0.0, 0.3, 0.1)
+
+
+def test_wide_column_lines_take_four_pixel_path():
+    ang = angles_with_ties(24)
+    _, groups = emu_view_spans(ang, WIDE_COLUMNS, groups=True)
+    assert np.all(groups == 0)
+    x = np.random.default_rng(3).random(shapes3d(WIDE_COLUMNS, 1, 24)[0]).astype(np.float32)
+    for op in ("A", "BT", "BWT"):
+        want = oracle.op3d(op, x.astype(np.float64), ang, WIDE_COLUMNS)
+        assert rel_l2(emu_op3d(op, x, ang, WIDE_COLUMNS), want) <= 1e-5, op
+        assert rel_l2(emu_op3d(op, x, ang, WIDE_COLUMNS, cell_variant=1), want) <= 1e-5, op
+
+
 @pytest.mark.parametrize("geo", sorted(CASES3D))
 def test_voxel_gather_staged_and_direct_agree(geo):
     """The voxel gather with the per-view block staged in LDS (lookups outside the block

@@ -140,6 +140,21 @@ def test_cell_tilings_bitwise_equal(cuda):
         assert torch.equal(o, outs[0]), f"tile_rows {R} differs from tile_rows 1"
 
 
+def test_cell_four_pixel_lines(cuda):
+    """Detector columns about two pixels wide: general-class groups whose lines take the
+    fixed three- and four-pixel paths, against the oracle."""
+    from helpers import opts3d
+
+    opts = opts3d(40, 36, 6, 40, 6, 0.5, 0.5, 0.5, 2.2, 1.0, 500.0, 1000.0, 1, 0.1, -0.05, 0.0, 0.3, 0.1)
+    V = 24
+    ang = angles_with_ties(V)
+    x = np.random.default_rng(3).random(shapes3d(opts, 2, V)[0]).astype(np.float32)
+    for op in ("A", "BT", "BWT"):
+        print(f"[gpu] four-pixel lines {op}", flush=True)
+        got = gpu_op(op, x, ang, opts, cuda)
+        assert rel_l2(got, oracle.op3d(op, x.astype(np.float64), ang, opts)) <= 1e-5, op
+
+
 def test_cell_general_class_split_launch(cuda):
     """A coarse detector (every column group in the general class) with enough views that
     the cell gathers launch per class rather than in one mixed grid."""

@@ -91,8 +91,8 @@ inline size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
 
 // Workspace of the cell gathers: z-prefix of the volume in one orientation, plus
 // kVolPad zeroed pad entries after the table (the fixed-span loads of the cell gather may
-// read up to two entries past the last pixel, with weight zero).
-constexpr int kVolPad = 2;
+// read up to three entries past the last pixel, with weight zero).
+constexpr int kVolPad = 4;
 
 inline size_t vol_prefix_entries(const Plan3d &p, int nbatch)
 {
@@ -490,10 +490,11 @@ __host__ __device__ inline void cell_stage(const float2 *__restrict__ Wl, const 
 }
 
 // One line L (table coefficients lc) of the cell gather for this thread.  W: the batch
-// item's z-prefix table in the view's orientation.  SPAN = 2 or 3: every column of the
-// view overlaps at most SPAN consecutive pixels of any line (ViewHdr::span) and the line
-// takes a fixed SPAN-pixel path, reading the wave's staged block `lds` when given (and
-// when the lane's lookups fall in it); SPAN = 0: the general pixel loop.  BIG: 64-bit
+// item's z-prefix table in the view's orientation.  SPAN = 2, 3 or 4: the thread's column
+// overlaps at most SPAN consecutive pixels of this line (the caller's span test) and the
+// line takes a fixed SPAN-pixel path, reading the wave's staged block `lds` when given
+// (and when the lane's lookups fall in it; two- and three-pixel paths only); SPAN = 0:
+// the general pixel loop.  BIG: 64-bit
 // table offsets (entries), otherwise 32-bit byte offsets over the wave-uniform line base.
 template <int OPK, int GEO, int R, int SPAN, bool BIG>
 __host__ __device__ inline void cell_line(CellLane<R> &st, const ViewHdr &h, const CellArgs &a, const float4 &lc,
@@ -538,26 +539,34 @@ __host__ __device__ inline void cell_line(CellLane<R> &st, const ViewHdr &h, con
     const float2 *Wl = W + (int64_t)L * nalong;
     const char *Wb = reinterpret_cast<const char *>(Wl);
 
-    if (SPAN == 2 || SPAN == 3) {
+    if (SPAN == 2 || SPAN == 3 || SPAN == 4) {
         const int m0 = t0 > 0.0f ? (int)t0 : 0;
-        // Pixel weights: m0 and m0+1 (two-pixel path: the column spans less than one
-        // pixel), and m0+2 (three-pixel path: less than two).  Entries past the line's end
-        // are weighted by zero.
-        float w0, w1, w2 = 0.0f;
+        // Pixel weights of m0 .. m0+SPAN-1 (the column spans less than SPAN-1 pixels, so
+        // it ends before m0+SPAN); entries past the line's end are weighted by zero.
+        float w0, w1, w2 = 0.0f, w3 = 0.0f;
         w0 = pixel_weight(fminf(t1, (float)(m0 + 1)) - fmaxf(t0, (float)m0), m0);
         if (SPAN == 2) {
             w1 = m0 + 1 < nalong ? pixel_weight(fmaxf(t1 - (float)(m0 + 1), 0.0f), m0 + 1) : 0.0f;
         } else {
             w1 = m0 + 1 < nalong ? pixel_weight(fmaxf(fminf(t1, (float)(m0 + 2)) - (float)(m0 + 1), 0.0f), m0 + 1)
                                  : 0.0f;
-            w2 = m0 + 2 < nalong ? pixel_weight(fmaxf(t1 - (float)(m0 + 2), 0.0f), m0 + 2) : 0.0f;
+            if (SPAN == 3) {
+                w2 = m0 + 2 < nalong ? pixel_weight(fmaxf(t1 - (float)(m0 + 2), 0.0f), m0 + 2) : 0.0f;
+            } else {
+                w2 = m0 + 2 < nalong
+                         ? pixel_weight(fmaxf(fminf(t1, (float)(m0 + 3)) - (float)(m0 + 2), 0.0f), m0 + 2)
+                         : 0.0f;
+                w3 = m0 + 3 < nalong ? pixel_weight(fmaxf(t1 - (float)(m0 + 3), 0.0f), m0 + 3) : 0.0f;
+            }
         }
-        // Lookups: entries m0, m0+1 (one 16-byte pair) and m0+2 per row boundary, from the
-        // wave's staged block when they all fall in it, else from the table.
+        // Lookups per row boundary: entries m0, m0+1 (one 16-byte pair), and m0+2 (three
+        // pixels) or m0+2, m0+3 (a second pair, four pixels), from the wave's staged block
+        // when they all fall in it, else from the table.
         Pair2 e[R + 1];
         float2 g[R + 1];
-        const bool staged = !BIG && lds != nullptr && bx.pitch > 0 && ki[0] >= bx.Z0 && ki[R] < bx.Z0 + bx.NZ &&
-                            m0 + SPAN - 1 < bx.P0 + bx.pitch;
+        Pair2 e2[R + 1];
+        const bool staged = SPAN != 4 && !BIG && lds != nullptr && bx.pitch > 0 && ki[0] >= bx.Z0 &&
+                            ki[R] < bx.Z0 + bx.NZ && m0 + SPAN - 1 < bx.P0 + bx.pitch;
         if (staged) {
             const unsigned p8 = (unsigned)bx.pitch * 8u;
             const unsigned base = (unsigned)(m0 - bx.P0) * 8u - (unsigned)bx.Z0 * p8;
@@ -576,11 +585,13 @@ __host__ __device__ inline void cell_line(CellLane<R> &st, const ViewHdr &h, con
                 else pq = reinterpret_cast<const float2 *>(Wb + mad24((unsigned)ki[q], a.plane8, (unsigned)m0 * 8u));
                 e[q] = *reinterpret_cast<const Pair2 *>(pq);
                 if (SPAN == 3) g[q] = pq[2];
+                if (SPAN == 4) e2[q] = *reinterpret_cast<const Pair2 *>(pq + 2);
             }
         }
         float J0 = fmaf(fr[0], e[0].y0, e[0].x0);
         float J1 = fmaf(fr[0], e[0].y1, e[0].x1);
-        float J2 = SPAN == 3 ? fmaf(fr[0], g[0].y, g[0].x) : 0.0f;
+        float J2 = SPAN == 3 ? fmaf(fr[0], g[0].y, g[0].x) : SPAN == 4 ? fmaf(fr[0], e2[0].y0, e2[0].x0) : 0.0f;
+        float J3 = SPAN == 4 ? fmaf(fr[0], e2[0].y1, e2[0].x1) : 0.0f;
 #pragma unroll
         for (int q = 0; q < R; ++q) {
             const float K0 = fmaf(fr[q + 1], e[q + 1].y0, e[q + 1].x0);
@@ -590,6 +601,14 @@ __host__ __device__ inline void cell_line(CellLane<R> &st, const ViewHdr &h, con
                 const float K2 = fmaf(fr[q + 1], g[q + 1].y, g[q + 1].x);
                 t = fmaf(w2, K2 - J2, t);
                 J2 = K2;
+            }
+            if (SPAN == 4) {
+                const float K2 = fmaf(fr[q + 1], e2[q + 1].y0, e2[q + 1].x0);
+                const float K3 = fmaf(fr[q + 1], e2[q + 1].y1, e2[q + 1].x1);
+                t = fmaf(w3, K3 - J3, t);
+                t = fmaf(w2, K2 - J2, t);
+                J2 = K2;
+                J3 = K3;
             }
             st.acc[q] = fmaf(w0, K0 - J0, t);
             J0 = K0;
@@ -654,8 +673,9 @@ __host__ __device__ inline const float2 *cell_table(const float2 *WX, const floa
 // One thread, reading the tables in HBM only: internal column c, rows r0..r0+R-1 of view
 // v, batch b.  SPAN: the class of the thread's column group (CellGroup::span), whose
 // widest column spans wgrp on the axis; in a group of class 3 or 0 each line takes the
-// narrowest fixed path its pixel footprint allows (wgrp = +inf: the general loop on every
-// line).  BIG as for cell_line.
+// narrowest fixed path (two, three or four pixels) its pixel footprint allows, the
+// general loop beyond (wgrp = +inf: the general loop on every line).  BIG as for
+// cell_line.
 template <int OPK, int GEO, int R, int SPAN, bool BIG>
 __host__ __device__ inline void cell_body(const ViewHdr *__restrict__ hdr, const float4 *__restrict__ lineC,
                                           const float4 *__restrict__ colt, const float2 *__restrict__ WX,
@@ -678,6 +698,7 @@ __host__ __device__ inline void cell_body(const ViewHdr *__restrict__ hdr, const
         const float s = wgrp * lc.y;
         if (s < 0.99f) cell_line<OPK, GEO, R, 2, BIG>(st, h, a, lc, L, W, none, nullptr);
         else if (SPAN == 3 || s < 1.99f) cell_line<OPK, GEO, R, 3, BIG>(st, h, a, lc, L, W, none, nullptr);
+        else if (s < 2.99f) cell_line<OPK, GEO, R, 4, BIG>(st, h, a, lc, L, W, none, nullptr);
         else cell_line<OPK, GEO, R, 0, BIG>(st, h, a, lc, L, W, none, nullptr);
     }
     cell_store<OPK, GEO, R>(st, h, a, out, c, r0, v, b);
