@@ -136,13 +136,14 @@ def bench_batched(dev, steps, verbose=False):
     t_batch = timeit(batched, steps)
     bp.capture([A, B], vols)
     t_graph = timeit(bp.replay, steps)
-    # the batched path must reproduce the plain loop with the same tiles
-    same = True
+    # the batched path must reproduce the plain loop with the same tiles, bitwise
     for i, p in enumerate(plans):
         p.execute_into(A, vols[i], sinos[i], wsa[i], sid, bta[0], bta[1], 0)
+        p.execute_into(B, sinos[i], recs[i], wsb[i], sid, btb[0], btb[1], 0)
     bp.run(A, vols, bsino)
+    bp.run(B, bsino, brec)
     torch.cuda.synchronize()
-    same = all(torch.equal(bsino[i], sinos[i]) for i in range(16))
+    same = all(torch.equal(bsino[i], sinos[i]) and torch.equal(brec[i], recs[i]) for i in range(16))
     return {"workload": "16 uneven cone-flat instances (volumes 128^2x32..384^2x96, detectors 2W x D, "
                         "V in 180..984, random start angles), A+B",
             "voxel_views_per_operator": vv,
