@@ -240,6 +240,21 @@ def test_cfg2_adjointness(cfg2, cuda, pair):
     assert abs(lhs - rhs) / abs(rhs) <= 1e-5, (lhs, rhs)
 
 
+def test_cfg2_batch_items_in_split_launches(cfg2, cuda):
+    """cfg2 with two batch items: the per-class cell launches and the voxel gather decode
+    the batch item from the launch grid; each item equals its own single-item run."""
+    plan, ang, ph, x = cfg2
+    s = torch.cuda.current_stream().cuda_stream
+    x2 = torch.cat([x, torch.rand_like(x)], dim=0)
+    print("[gpu] cfg2 A and B with two batch items", flush=True)
+    y2 = plan.execute(0, x2, s)
+    for i in range(2):
+        assert torch.equal(y2[i:i + 1], plan.execute(0, x2[i:i + 1].contiguous(), s)), f"A item {i}"
+    b2 = plan.execute(2, y2, s)
+    for i in range(2):
+        assert torch.equal(b2[i:i + 1], plan.execute(2, y2[i:i + 1].contiguous(), s)), f"B item {i}"
+
+
 def test_cfg2_deterministic_and_linear(cfg2, cuda):
     plan, ang, ph, x = cfg2
     s = torch.cuda.current_stream().cuda_stream
