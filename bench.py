@@ -456,7 +456,7 @@ def main():
         # three-pixel and general at most 8 rows)
         ncls = plan.cell_classes()
         rows = (tune_a[0], min(tune_a[0], 8), min(tune_a[0], 8))
-        if sum(ncls) * -(-wl["Nv"] // (4 * tune_a[0])) < 4096:  # small grids: one launch for all classes
+        if -(-sum(ncls) // 4) * -(-wl["Nv"] // tune_a[0]) < 4096:  # small grids: one launch for all classes
             symbol = [f"k_cell<0, 1, {tune_a[0]}, -1, false>"]
         else:
             symbol = [f"k_cell<0, 1, {r}, {span}, false>" for r, span, n in zip(rows, (2, 3, 0), ncls) if n > 0]

@@ -62,9 +62,10 @@ __host__ __device__ inline int cell_class_offset(const BatchDesc &d, int K)
 {
     return K == 1 ? d.ncls[0] : K == 2 ? d.ncls[0] + d.ncls[1] : 0;
 }
+// Workgroups of class index K at R rows: quads of groups x row tiles x batch items.
 __host__ __device__ inline int blocks_cell(const BatchDesc &d, int R, int K)
 {
-    return cell_class_items(d, K) * ceil_div(d.ca.nv, R * kWavesPerBlock) * d.nb;
+    return ceil_div(cell_class_items(d, K), kWavesPerBlock) * ceil_div(d.ca.nv, R) * d.nb;
 }
 // Rows per thread of span-class index K: R for the two-pixel class, at most 8 otherwise.
 __host__ __device__ inline int cell_class_rows(int R, int K) { return K == 0 || R <= 8 ? R : 8; }
@@ -142,7 +143,7 @@ __global__ __launch_bounds__(kBlock) void k_transpose_batch(const BatchDesc *__r
 // every instance: the instance's groups in class order, then row groups, then batch
 // items.  One launch per class, as for a single plan.
 template <int OPK, int GEO, int R, int K, bool BIG>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_cell_batch(
+__global__ __launch_bounds__(kBlock) void k_cell_batch(
     const BatchDesc *__restrict__ D, int n, IOPtrs io, char *__restrict__ ws)
 {
 #if XR_CELL_STAGED
@@ -154,16 +155,18 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
     if (i < 0) return;
     const BatchDesc &d = D[i];
     const CellArgs a = d.ca;
-    const int ncg = ceil_div(a.nu, kWave), gy = ceil_div(a.nv, R * kWavesPerBlock);
-    const int nk = cell_class_items(d, K);
-    const int item = d.citems[cell_class_offset(d, K) + bid % nk];
-    const int rest = bid / nk;
-    const int by = rest % gy, b = rest / gy;
+    // as k_cell: 4 waves on 4 consecutive groups of the class, one row tile, one batch item
+    const int ncg = ceil_div(a.nu, kWave), nrt = ceil_div(a.nv, R);
+    const int nk = cell_class_items(d, K), nquad = ceil_div(nk, kWavesPerBlock);
+    const int lane = threadIdx.x & (kWave - 1), wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const int rt = bid % nrt, rest = bid / nrt;
+    const int quad = rest % nquad, b = rest / nquad;
+    const int idx = quad * kWavesPerBlock + wave;
+    if (idx >= nk) return;  // the whole wave
+    const int item = d.citems[cell_class_offset(d, K) + idx];
     const int v = item / ncg, cg = item - v * ncg;
-    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
     const int c = cg * kWave + lane;
-    const int r0 = (by * kWavesPerBlock + wave) * R;
-    if (r0 >= a.nv) return;  // the whole wave
+    const int r0 = rt * R;
     const float2 *WX = reinterpret_cast<const float2 *>(ws + d.ws_volx);
     const float2 *WY = reinterpret_cast<const float2 *>(ws + d.ws_voly);
     const float wgrp = BIG ? INFINITY : d.cgrp[item].wmax;

@@ -214,9 +214,10 @@ __global__ __launch_bounds__(kBlock) void k_fdk_prefix(const ViewHdr *__restrict
     fdk_prefix_body<MODE, GEO>(hdr, colt, Q, a, blockIdx.z, blockIdx.y, c);
 }
 
-// Cell gather over the column groups of one span class: workgroup blockIdx.x takes group
-// items[blockIdx.x % nitems] (view v, 64 columns) of batch item blockIdx.x / nitems, and
-// blockIdx.y * 4 + wave the rows r0 .. r0+R-1.  SPAN = the class (2, 3 or 0), so each
+// Cell gather over the column groups of one span class: a workgroup is 4 waves on 4
+// consecutive listed groups (view v, 64 columns each), rows r0 .. r0+R-1 of one row tile,
+// one batch item; blockIdx.x = (batch item, group quad, row tile), row tile fastest.
+// Every wave of a launch but the last quad's has work whatever Nv is.  SPAN = the class (2, 3 or 0), so each
 // class runs with the registers of its own paths; SPAN = -1: every class in one launch,
 // each workgroup reading its group's class (small problems, where launches cost more
 // than registers).
@@ -231,14 +232,17 @@ __global__ __launch_bounds__(kBlock) void k_cell(const ViewHdr *__restrict__ hdr
     __shared__ __attribute__((aligned(16))) float2 lds[kWavesPerBlock][cell_lds_cap<R>()];
 #endif
     const int lane = threadIdx.x & (kWave - 1);
-    const int wave = threadIdx.x / kWave;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     const int ncg = ceil_div(a.nu, kWave);
-    const int item = items[blockIdx.x % nitems];
-    const int b = blockIdx.x / nitems;
+    const int nrt = ceil_div(a.nv, R), nquad = ceil_div(nitems, kWavesPerBlock);
+    const int rt = blockIdx.x % nrt, rest = blockIdx.x / nrt;
+    const int quad = rest % nquad, b = rest / nquad;
+    const int idx = quad * kWavesPerBlock + wave;
+    if (idx >= nitems) return;  // the whole wave
+    const int item = items[idx];
     const int v = item / ncg, cg = item - v * ncg;
     const int c = cg * kWave + lane;
-    const int r0 = (blockIdx.y * kWavesPerBlock + wave) * R;
-    if (r0 >= a.nv) return;  // the whole wave
+    const int r0 = rt * R;
     const float wgrp = BIG ? INFINITY : cgrp[item].wmax;
 #if XR_CELL_STAGED
     {
@@ -293,12 +297,18 @@ __global__ __launch_bounds__(kBlock) void k_split_sum(const float *__restrict__ 
     }
 }
 
+// Workgroups of a cell-gather launch over `nitems` groups (see k_cell).
+inline int64_t cell_blocks(const CellArgs &a, int R, int nitems, int nbatch)
+{
+    return (int64_t)ceil_div(nitems, kWavesPerBlock) * ceil_div(a.nv, R) * nbatch;
+}
+
 template <int OPK, int GEO, int SPAN, bool BIG>
 void launch_cell_class(const Plan3d &p, const CellArgs &a, int R, const int *items, int nitems, const float2 *WX,
                        const float2 *WY, float *out, int nbatch, hipStream_t s)
 {
     if (nitems == 0) return;
-    const dim3 grid((unsigned)((int64_t)nitems * nbatch), ceil_div(a.nv, R * kWavesPerBlock));
+    const dim3 grid((unsigned)cell_blocks(a, R, nitems, nbatch));
 #define XR_CELL(RR)                                                                                                  \
     hipLaunchKernelGGL((k_cell<OPK, GEO, RR, SPAN, BIG>), grid, dim3(kBlock), 0, s, p.d_hdr, p.d_lineC, p.d_col,    \
                        p.d_cgrp, items, nitems, WX, WY, out, a)
@@ -331,7 +341,7 @@ void launch_cell(const Plan3d &p, int R, const float2 *WX, const float2 *WY, flo
         launch_cell_class<OPK, GEO, 0, true>(p, a, 8, p.d_citems, ntot, WX, WY, out, nbatch, s);
         return;
     }
-    if ((int64_t)ntot * nbatch * ceil_div(a.nv, R * kWavesPerBlock) < kCellSplitMin) {
+    if (cell_blocks(a, R, ntot, nbatch) < kCellSplitMin) {
         launch_cell_class<OPK, GEO, -1, false>(p, a, R, p.d_citems, ntot, WX, WY, out, nbatch, s);
         return;
     }
@@ -514,8 +524,9 @@ int execute_phase(const Plan3d &p, int op, int phase, const float *in, float *ou
     if (!valid_rows(tile_rows)) return fail(XR_EINVAL, "tile_rows must be 1, 2, 4, 8 or 16");
     if (!valid_chunk(voxel_chunk)) return fail(XR_EINVAL, "voxel_chunk must be 1, 4, 8 or 16");
 
-    if (rt.cell && (int64_t)(p.ncls[0] + p.ncls[1] + p.ncls[2]) * nbatch >= ((int64_t)1 << 31))
-        return fail(XR_EINVAL, "batch too large for one launch of the cell gather (views x column groups x batch "
+    if (rt.cell && cell_blocks(make_cell_args(p, nbatch), 1, p.ncls[0] + p.ncls[1] + p.ncls[2], nbatch) >=
+                       ((int64_t)1 << 31))
+        return fail(XR_EINVAL, "batch too large for one launch of the cell gather (column groups / 4 x rows x batch "
                                "must stay below 2^31)");
     if (rt.cell) {
         float2 *WX = static_cast<float2 *>(ws);
