@@ -317,6 +317,43 @@ int emu_fdk3d(const int *vn, const float *vf, const int *gn, const float *gf, co
     return XR_OK;
 }
 
+// The register-held prefix bodies (columns of at most kPrefixRegRows entries) against the
+// generic ones, on `in` (a volume and a sinogram of the plan's shapes, nb items): 1 when
+// both tables agree bitwise, 0 otherwise.
+int emu_prefix_bodies_agree(const int *vn, const float *vf, const int *gn, const float *gf, const float *angles,
+                            int nview, const float *vol, const float *sino, int nb)
+{
+    Plan3d p;
+    PlanTables t;
+    const int rc = make_plan(p, t, vn, vf, gn, gf, angles, nview);
+    if (rc) return rc;
+    const int64_t plane = (int64_t)p.vol.nx * p.vol.ny;
+    const size_t nvol = (size_t)nb * (p.vol.nz + 1) * plane;
+    std::vector<float2> W1(nvol), W2(nvol);
+    if (p.vol.nz <= kPrefixRegRows)
+        for (int b = 0; b < nb; ++b)
+            for (int64_t e = 0; e < plane; ++e) {
+                vol_prefix_body(vol, W1.data(), plane, p.vol.nz, b, e);
+                vol_prefix_reg(vol, W2.data(), plane, p.vol.nz, b, e);
+            }
+    const SinoArgs sa = make_sino_args(p, nb);
+    const size_t nq = (size_t)nb * p.nview * (p.geo.nv + 1) * p.geo.nu;
+    std::vector<float2> Q1(nq), Q2(nq), Q3(nq), Q4(nq);
+    if (p.geo.nv <= kPrefixRegRows)
+        for (int b = 0; b < nb; ++b)
+            for (int v = 0; v < p.nview; ++v)
+                for (int c = 0; c < p.geo.nu; ++c) {
+                    sino_prefix_body<SINO_PLAIN, GEO_FLAT>(p.d_hdr, p.d_col, sino, Q1.data(), sa, b, v, c);
+                    sino_prefix_reg<SINO_PLAIN, GEO_FLAT>(p.d_hdr, p.d_col, sino, Q2.data(), sa, b, v, c);
+                    sino_prefix_body<SINO_AT, GEO_FLAT>(p.d_hdr, p.d_col, sino, Q3.data(), sa, b, v, c);
+                    sino_prefix_reg<SINO_AT, GEO_FLAT>(p.d_hdr, p.d_col, sino, Q4.data(), sa, b, v, c);
+                }
+    const bool same = std::memcmp(W1.data(), W2.data(), nvol * sizeof(float2)) == 0 &&
+                      std::memcmp(Q1.data(), Q2.data(), nq * sizeof(float2)) == 0 &&
+                      std::memcmp(Q3.data(), Q4.data(), nq * sizeof(float2)) == 0;
+    return same ? 1 : 0;
+}
+
 // Span classes of the cell gather for a geometry: per view (ViewHdr::span) into
 // `spans` [nview], and per (view, 64-column group) (CellGroup::span) into `group_spans`
 // [nview][ceil(Nu/64)] when it is not null.

@@ -226,3 +226,21 @@ def test_fused_fbp_2d_vs_fp64_pipeline(geo):
     got = emu_fdk3d(y[:, :, None], ang, opts2d_as_3d(o2), filt, weight, is2d=True).reshape(2, 17, 33)
     want = oracle.op2d("BW", correlate_fp64(y.astype(np.float64) * weight, filt), ang, o2)
     assert rel_l2(got, want) <= 1e-5
+
+
+@pytest.mark.parametrize("geo", sorted(CASES3D))
+def test_register_prefix_bodies_match_generic(geo):
+    """The prefix passes keep columns of at most 64 entries in registers on the device
+    (vol_prefix_reg, sino_prefix_reg); their tables equal the generic bodies' bitwise."""
+    from helpers import _load_emu, _p
+
+    opts = CASES3D[geo]
+    V = 12
+    ang = angles_with_ties(V)
+    vol, sino = shapes3d(opts, 2, V)
+    rng = np.random.default_rng(5)
+    x = rng.random(vol).astype(np.float32)
+    y = rng.random(sino).astype(np.float32)
+    vn, vf, gn, gf = oracle.split3d(opts)
+    lib = _load_emu()
+    assert lib.emu_prefix_bodies_agree(_p(vn), _p(vf), _p(gn), _p(gf), _p(ang), V, _p(x), _p(y), 2) == 1

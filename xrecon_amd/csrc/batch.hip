@@ -107,7 +107,10 @@ __global__ __launch_bounds__(kBlock) void k_vol_prefix_batch(const BatchDesc *__
         reinterpret_cast<float2 *>(ws + d.ws_volx)[plane * (d.ca.nz + 1) * d.nb + idx] = make_float2(0.0f, 0.0f);
     if (idx >= plane * d.nb) return;
     const int64_t b = idx / plane;
-    vol_prefix_body(io.in[i], reinterpret_cast<float2 *>(ws + d.ws_volx), plane, d.ca.nz, b, idx - b * plane);
+    if (d.ca.nz <= kPrefixRegRows)
+        vol_prefix_reg(io.in[i], reinterpret_cast<float2 *>(ws + d.ws_volx), plane, d.ca.nz, b, idx - b * plane);
+    else
+        vol_prefix_body(io.in[i], reinterpret_cast<float2 *>(ws + d.ws_volx), plane, d.ca.nz, b, idx - b * plane);
 }
 
 __global__ __launch_bounds__(kBlock) void k_transpose_batch(const BatchDesc *__restrict__ D, int n,
@@ -192,7 +195,10 @@ __global__ __launch_bounds__(kBlock) void k_sino_prefix_batch(const BatchDesc *_
     const int c = (bid % gx) * kBlock + threadIdx.x;
     const int v = (bid / gx) % d.sa.nview, b = bid / (gx * d.sa.nview);
     if (c >= d.sa.nu) return;
-    sino_prefix_body<MODE, GEO>(d.hdr, d.col, io.in[i], reinterpret_cast<float2 *>(ws + d.ws_sino), d.sa, b, v, c);
+    if (d.sa.nv <= kPrefixRegRows)
+        sino_prefix_reg<MODE, GEO>(d.hdr, d.col, io.in[i], reinterpret_cast<float2 *>(ws + d.ws_sino), d.sa, b, v, c);
+    else
+        sino_prefix_body<MODE, GEO>(d.hdr, d.col, io.in[i], reinterpret_cast<float2 *>(ws + d.ws_sino), d.sa, b, v, c);
 }
 
 template <int OPK, int GEO, int KC>

@@ -46,11 +46,6 @@ __global__ __launch_bounds__(kBlock) void k_zero(float *__restrict__ p, int64_t 
         p[i] = 0.0f;
 }
 
-// Prefix columns of at most kPrefixRegRows entries keep their samples in registers: one
-// read of the input instead of two, all loads in flight at once.  The operations and
-// their order are those of vol_prefix_body / prefix_column.
-constexpr int kPrefixRegRows = 64;
-
 __global__ __launch_bounds__(kBlock) void k_vol_prefix(const float *__restrict__ f, float2 *__restrict__ W, int nx,
                                                        int ny, int nz, int nbatch)
 {
@@ -59,27 +54,8 @@ __global__ __launch_bounds__(kBlock) void k_vol_prefix(const float *__restrict__
     if (idx < kVolPad) W[plane * (nz + 1) * nbatch + idx] = make_float2(0.0f, 0.0f);  // pad entries
     if (idx >= plane * nbatch) return;
     const int64_t b = idx / plane, e = idx - b * plane;
-    if (nz > kPrefixRegRows) {
-        vol_prefix_body(f, W, plane, nz, b, e);
-        return;
-    }
-    const float *src = f + b * plane * nz + e;
-    float2 *dst = W + b * plane * (nz + 1) + e;
-    float v[kPrefixRegRows];
-#pragma unroll
-    for (int k = 0; k < kPrefixRegRows; ++k) v[k] = k < nz ? src[(int64_t)k * plane] : 0.0f;
-    double total = 0.0;
-#pragma unroll
-    for (int k = 0; k < kPrefixRegRows; ++k)
-        if (k < nz) total += (double)v[k];
-    double run = -0.5 * total;
-#pragma unroll
-    for (int k = 0; k < kPrefixRegRows; ++k)
-        if (k < nz) {
-            dst[(int64_t)k * plane] = make_float2((float)run, v[k]);
-            run += (double)v[k];
-        }
-    dst[(int64_t)nz * plane] = make_float2((float)run, 0.0f);
+    if (nz <= kPrefixRegRows) vol_prefix_reg(f, W, plane, nz, b, e);
+    else vol_prefix_body(f, W, plane, nz, b, e);
 }
 
 // y-driven line orientation: WY[p][i][ny-1-j] = WX[p][j][i] for every plane p of every
@@ -112,42 +88,8 @@ __global__ __launch_bounds__(kBlock) void k_sino_prefix(const ViewHdr *__restric
 {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= a.nu) return;
-    const int b = blockIdx.z, v = blockIdx.y;
-    if (a.nv > kPrefixRegRows) {
-        sino_prefix_body<MODE, GEO>(hdr, colt, p, Q, a, b, v, c);
-        return;
-    }
-    const ViewHdr h = hdr[v];
-    const int col = h.keep ? c : (a.nu - 1 - c);
-    const int64_t sv = (int64_t)b * a.nview + v;
-    const float *src = p + sv * a.nv * a.nu + col;
-    float2 *dst = Q + sv * (int64_t)(a.nv + 1) * a.nu + c;
-    float scale0 = 1.0f, dbp = 0.0f, sg = 1.0f;
-    if (MODE == SINO_AT) {
-        const float4 cc = colt[(int64_t)v * (a.nu + 1) + c];
-        const float dn = colt[(int64_t)v * (a.nu + 1) + c + 1].x;
-        scale0 = h.delta / cc.w;
-        dbp = 0.5f * (cc.x + dn) - h.P;
-        sg = cc.y;
-    }
-    float q[kPrefixRegRows];
-#pragma unroll
-    for (int r = 0; r < kPrefixRegRows; ++r) {
-        q[r] = r < a.nv ? src[(int64_t)r * a.nu] : 0.0f;
-        if (MODE == SINO_AT && r < a.nv) q[r] *= at_scale<GEO>(h, scale0, dbp, sg, a.v0, a.dvv, r);
-    }
-    double total = 0.0;
-#pragma unroll
-    for (int r = 0; r < kPrefixRegRows; ++r)
-        if (r < a.nv) total += (double)q[r];
-    double run = -0.5 * total;
-#pragma unroll
-    for (int r = 0; r < kPrefixRegRows; ++r)
-        if (r < a.nv) {
-            dst[(int64_t)r * a.nu] = make_float2((float)(run - (double)r * (double)q[r]), q[r]);
-            run += (double)q[r];
-        }
-    dst[(int64_t)a.nv * a.nu] = make_float2((float)run, 0.0f);
+    if (a.nv <= kPrefixRegRows) sino_prefix_reg<MODE, GEO>(hdr, colt, p, Q, a, blockIdx.z, blockIdx.y, c);
+    else sino_prefix_body<MODE, GEO>(hdr, colt, p, Q, a, blockIdx.z, blockIdx.y, c);
 }
 
 // FDK filter stage.  Every detector row of every view and batch item is filtered with the
