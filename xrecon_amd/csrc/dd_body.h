@@ -207,6 +207,18 @@ __host__ __device__ inline unsigned mul24(unsigned a, unsigned b)
 // a * b + c for a, b below 2^24: one full-rate v_mad_u32_u24 on the device.  Written as
 // an instruction because the compiler otherwise turns some of these (runtime operands it
 // cannot bound) into a mask and a quarter-rate v_mul_lo_u32.
+// A value that is the same on every lane of the wave, marked so for the compiler (kept in
+// a scalar register even when it was loaded through a pointer the compiler cannot prove
+// uniform, as in the batched kernels' descriptor tables).
+__host__ __device__ inline int wave_uniform(int x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_readfirstlane(x);
+#else
+    return x;
+#endif
+}
+
 __host__ __device__ inline unsigned mad24(unsigned a, unsigned b, unsigned c)
 {
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -566,7 +578,7 @@ __host__ __device__ inline void cell_line(CellLane<R> &st, const ViewHdr &h, con
                                           int L, const float2 *__restrict__ W, const CellBox &bx,
                                           const float2 *__restrict__ lds)
 {
-    const int nalong = h.nalong;
+    const int nalong = wave_uniform(h.nalong);
     const float fal = (float)nalong;
     // Column boundaries in pixel units along the line.
     const float t0 = fmaf(st.d0, lc.y, lc.x);
@@ -728,11 +740,14 @@ __host__ __device__ inline void cell_store(const CellLane<R> &st, const ViewHdr 
     }
 }
 
+// The z-prefix table of batch item b in the view's orientation (the same for every lane
+// of a wave: the header fields are marked wave-uniform so that the table base stays in
+// scalar registers also where the header was read through a descriptor table).
 __host__ __device__ inline const float2 *cell_table(const float2 *WX, const float2 *WY, const ViewHdr &h,
                                                     const CellArgs &a, int b)
 {
-    const int64_t plane = (int64_t)h.nline * h.nalong;
-    return (h.xdrv ? WX : WY) + (int64_t)b * (int64_t)(a.nz + 1) * plane;
+    const int64_t plane = (int64_t)wave_uniform(h.nline) * wave_uniform(h.nalong);
+    return (wave_uniform(h.xdrv) ? WX : WY) + (int64_t)b * (int64_t)(a.nz + 1) * plane;
 }
 
 // One thread, reading the tables in HBM only: internal column c, rows r0..r0+R-1 of view
