@@ -14,7 +14,10 @@
 //    covers a fraction of each pixel's z column.  That z overlap is read from a z-prefix
 //    table of the volume, J(q) = J[k] + (q - k) f[k], so one row costs one table lookup
 //    per boundary instead of a loop over the voxels it overlaps, and the R rows share
-//    the line's column overlaps and their common row boundaries.
+//    the line's column overlaps and their common row boundaries.  The 64-column groups
+//    of every view are classed by how many pixels their widest column can cover (two,
+//    three, more); each class is one launch over its listed groups, and inside a wider
+//    group each line takes the narrowest fixed path its pixel footprint allows.
 //
 //  * Voxel gathers (B, B_W, A^T -> volume).  A thread owns one voxel column (i, j) and a
 //    chunk of KC slices; a workgroup is a 16 x 16 tile of voxel columns.  For each view
@@ -26,8 +29,8 @@
 //
 // The per-thread bodies live in dd_body.h.  Geometry comes from per-view FP32 tables
 // computed in FP64 on the host (plan.cpp); all overlap arithmetic is FP32, the prefix
-// sums are accumulated in FP64 and stored FP32.  The path is VALU-bound (no dense
-// contraction, so no MFMA).
+// sums are accumulated in FP64 and stored FP32.  The path is bound by the vector ALU
+// and the vector-memory address path (no dense contraction, so no MFMA).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
