@@ -195,7 +195,7 @@ __global__ __launch_bounds__(kBlock) void k_sino_prefix_batch(const BatchDesc *_
     const int c = (bid % gx) * kBlock + threadIdx.x;
     const int v = (bid / gx) % d.sa.nview, b = bid / (gx * d.sa.nview);
     if (c >= d.sa.nu) return;
-    if (d.sa.nv <= kPrefixRegRows)
+    if (MODE == SINO_PLAIN && d.sa.nv <= kPrefixRegRows)
         sino_prefix_reg<MODE, GEO>(d.hdr, d.col, io.in[i], reinterpret_cast<float2 *>(ws + d.ws_sino), d.sa, b, v, c);
     else
         sino_prefix_body<MODE, GEO>(d.hdr, d.col, io.in[i], reinterpret_cast<float2 *>(ws + d.ws_sino), d.sa, b, v, c);

@@ -91,7 +91,10 @@ __global__ __launch_bounds__(kBlock) void k_sino_prefix(const ViewHdr *__restric
 {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= a.nu) return;
-    if (a.nv <= kPrefixRegRows) sino_prefix_reg<MODE, GEO>(hdr, colt, p, Q, a, blockIdx.z, blockIdx.y, c);
+    // the A^T scaling (a square root and divisions per row) makes the register path
+    // slower than the generic one: plain mode only
+    if (MODE == SINO_PLAIN && a.nv <= kPrefixRegRows)
+        sino_prefix_reg<MODE, GEO>(hdr, colt, p, Q, a, blockIdx.z, blockIdx.y, c);
     else sino_prefix_body<MODE, GEO>(hdr, colt, p, Q, a, blockIdx.z, blockIdx.y, c);
 }
 
