@@ -41,7 +41,14 @@ struct CellArgs {
     float v0, dvv, zoff, sod;
     float dx, dy, shx, shy;
     unsigned plane8;  // Nx*Ny*8: byte stride of a z-plane (below 2^24 unless the plan is big)
+    int band;         // cell launch order: 1 = row tiles outermost (tables larger than kCellBandBytes)
 };
+
+// z-prefix tables (one orientation, one batch item) above this size are walked with the
+// row tiles outermost in the cell gathers' launch order: the workgroups in flight then
+// share a band of z-planes, which stays in the 256 MiB last-level cache; smaller tables
+// keep each group quad's row tiles together.
+constexpr uint64_t kCellBandBytes = 128ull << 20;
 
 struct VoxArgs {
     int nx, ny, nz, nu, nv, nview, maxline, nbatch, nchunk;
@@ -156,6 +163,7 @@ inline CellArgs make_cell_args(const Plan3d &p, int nbatch)
     a.v0 = p.v0; a.dvv = p.dvv; a.zoff = p.zoff; a.sod = p.sod;
     a.dx = p.vol.dx; a.dy = p.vol.dy; a.shx = p.vol.shift_x; a.shy = p.vol.shift_y;
     a.plane8 = (unsigned)((uint64_t)p.vol.nx * p.vol.ny * 8u);
+    a.band = (uint64_t)(p.vol.nz + 1) * p.vol.nx * p.vol.ny * 8u > kCellBandBytes ? 1 : 0;
     return a;
 }
 

@@ -224,7 +224,8 @@ __global__ __launch_bounds__(kBlock) void k_fdk_prefix(const ViewHdr *__restrict
 
 // Cell gather over the column groups of one span class: a workgroup is 4 waves on 4
 // consecutive listed groups (view v, 64 columns each), rows r0 .. r0+R-1 of one row tile,
-// one batch item; blockIdx.x = (batch item, group quad, row tile), row tile fastest.
+// one batch item; blockIdx.x = (batch item, group quad, row tile), row tile fastest
+// (a.band: (batch item, row tile, group quad), see kCellBandBytes).
 // Every wave of a launch but the last quad's has work whatever Nv is.  SPAN = the class (2, 3 or 0), so each
 // class runs with the registers of its own paths; SPAN = -1: every class in one launch,
 // each workgroup reading its group's class (small problems, where launches cost more
@@ -243,8 +244,18 @@ __global__ __launch_bounds__(kBlock) void k_cell(const ViewHdr *__restrict__ hdr
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     const int ncg = ceil_div(a.nu, kWave);
     const int nrt = ceil_div(a.nv, R), nquad = ceil_div(nitems, kWavesPerBlock);
-    const int rt = blockIdx.x % nrt, rest = blockIdx.x / nrt;
-    const int quad = rest % nquad, b = rest / nquad;
+    int rt, quad, b;
+    if (a.band) {  // row tiles outermost (large tables)
+        quad = blockIdx.x % nquad;
+        const int rest = blockIdx.x / nquad;
+        rt = rest % nrt;
+        b = rest / nrt;
+    } else {
+        rt = blockIdx.x % nrt;
+        const int rest = blockIdx.x / nrt;
+        quad = rest % nquad;
+        b = rest / nquad;
+    }
     const int idx = quad * kWavesPerBlock + wave;
     if (idx >= nitems) return;  // the whole wave
     const int item = items[idx];
