@@ -297,9 +297,10 @@ template <int GEO>
 __host__ __device__ inline float fdk_weight(float sx, float sy, float px, float py, float sod)
 {
     if (GEO == GEO_FLAT) {
-        const float dd = (sx * px + sy * py) / sod;
-        const float den = sod - dd;
-        return (sod * sod) / (den * den);
+        // sod^2 / (sod - (s.p)/sod)^2 written as 1 / (1 - (s.p)/sod^2)^2: one division per
+        // call (1/sod^2 is loop-invariant and hoisted)
+        const float t = fmaf(-(sx * px + sy * py), 1.0f / (sod * sod), 1.0f);
+        return 1.0f / (t * t);
     } else if (GEO == GEO_ARC) {
         const float ex = sx - px, ey = sy - py;
         return 1.0f / (ex * ex + ey * ey);
