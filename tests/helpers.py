@@ -37,6 +37,14 @@ CASES3D = {
 }
 
 
+# 2D cases: odd sizes, non-square pixels, shifted image and detector.
+CASES2D = {
+    "flat": opts2d(33, 17, 45, 1.0, 1.1, 0.5, 500.0, 1000.0, 1, 0.3, -0.2, 0.25),
+    "arc": opts2d(33, 17, 45, 1.0, 1.1, 2 * np.arctan(1.0 / 2000.0), 500.0, 1000.0, 2, 0.3, -0.2, 0.0003),
+    "par": opts2d(33, 17, 45, 1.0, 1.1, 0.9, 0.0, 0.0, 0, 0.3, -0.2, 0.25),
+}
+
+
 def angles_with_ties(V: int, offset: float = 0.01) -> np.ndarray:
     """V float32 angles over 2 pi with the 45-degree tie views inserted exactly as the
     float32 grid k*2pi/8 produces them."""

@@ -6,16 +6,11 @@ import numpy as np
 import pytest
 
 import oracle
-from helpers import CASES3D, OPS, VOLUME_IN, angles_with_ties, correlate_fp64, emu_available, emu_fdk3d, emu_op3d, \
-    emu_view_spans, opts2d, opts2d_as_3d, opts3d, rel_l2, shapes3d
+from helpers import CASES2D, CASES3D, OPS, VOLUME_IN, angles_with_ties, correlate_fp64, emu_available, emu_fdk3d, emu_op3d, \
+    emu_view_spans, opts2d_as_3d, opts3d, rel_l2, shapes3d
 
 pytestmark = pytest.mark.skipif(not emu_available(), reason="emulator not built (run __graft_entry__.build())")
 
-CASES2D = {
-    "flat": opts2d(33, 17, 45, 1.0, 1.1, 0.5, 500.0, 1000.0, 1, 0.3, -0.2, 0.25),
-    "arc": opts2d(33, 17, 45, 1.0, 1.1, 2 * np.arctan(1.0 / 2000.0), 500.0, 1000.0, 2, 0.3, -0.2, 0.0003),
-    "par": opts2d(33, 17, 45, 1.0, 1.1, 0.9, 0.0, 0.0, 0, 0.3, -0.2, 0.25),
-}
 
 
 @pytest.mark.parametrize("geo", sorted(CASES3D))

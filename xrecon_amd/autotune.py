@@ -9,8 +9,9 @@ Two knobs exist, one per kernel family (``Plan.kernel_kind(op)``):
   Longer chunks amortise each view's column search, shorter ones give more waves.
 
 ``tune`` times the gather pass alone (the prefix pass is shared) for every candidate
-with HIP events on the caller's stream, takes the median of a few launches, and caches
-the winner per (geometry, views, operator, batch).  Tuning launches kernels and
+with HIP events on the caller's stream, takes the median of a few launches, and keeps
+the winner on the plan object itself, per (operator, batch): a plan is one geometry and
+one angle set, and the choice lives and dies with it.  Tuning launches kernels and
 synchronises, so run it before graph capture or timed regions.
 """
 from __future__ import annotations
@@ -20,22 +21,22 @@ from typing import Dict, Tuple
 
 import torch
 
-from . import _backend as be
-
 CELL = 1
 VOXEL = 2
 TILE_ROWS = (1, 2, 4, 8, 16)
 VOXEL_CHUNKS = (1, 4, 8, 16)
 
-_CACHE: Dict[tuple, Tuple[int, int]] = {}
 
-
-def _key(plan, op: int, nbatch: int):
-    return (plan.handle, op, nbatch)
+def _table(plan) -> Dict[tuple, Tuple[int, int]]:
+    table = getattr(plan, "_xr_tuned", None)
+    if table is None:
+        table = {}
+        plan._xr_tuned = table
+    return table
 
 
 def cached(plan, op: int, nbatch: int):
-    return _CACHE.get(_key(plan, op, nbatch))
+    return _table(plan).get((op, nbatch))
 
 
 def candidates(plan, op: int):
@@ -49,9 +50,10 @@ def tune(plan, op: int, x: torch.Tensor, out: torch.Tensor, ws: torch.Tensor, re
     """Pick (tile_rows, voxel_chunk) for `op` on input `x` (shape fixes the batch).
     Returns the cached choice if this (plan, op, batch) was tuned before."""
     nb = x.shape[0] * x.shape[1]
-    key = _key(plan, op, nb)
-    if key in _CACHE:
-        return _CACHE[key]
+    key = (op, nb)
+    table = _table(plan)
+    if key in table:
+        return table[key]
     stream = torch.cuda.current_stream(x.device)
     sid = stream.cuda_stream
     plan.execute_into(op, x, out, ws, sid, 0, 0, 1)  # prefix pass, shared by all candidates
@@ -73,9 +75,10 @@ def tune(plan, op: int, x: torch.Tensor, out: torch.Tensor, ws: torch.Tensor, re
             best, best_t = (rows, chunk), t
     if verbose:
         print("autotune op", op, {k: round(v, 3) for k, v in timings.items()}, "->", best, flush=True)
-    _CACHE[key] = best
+    table[key] = best
     return best
 
 
-def clear() -> None:
-    _CACHE.clear()
+def clear(plan) -> None:
+    """Forget the choices tuned on `plan`."""
+    _table(plan).clear()

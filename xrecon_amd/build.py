@@ -64,12 +64,14 @@ def build_extension(verbose: bool = False) -> str:
     out = ext_path()
     tmp = out + ".tmp"
     abi = int(torch._C._GLIBCXX_USE_CXX11_ABI)
-    inc = [f"-I{p}" for p in ce.include_paths()] + [f"-I{sysconfig.get_paths()['include']}"]
+    # torch's HIP headers (c10/hip: the current stream of a device) need the ROCm include
+    # directory and the AMD platform macro; the HIP runtime itself comes through c10_hip.
+    inc = [f"-I{p}" for p in ce.include_paths(device_type="cuda")] + [f"-I{sysconfig.get_paths()['include']}"]
     libs = [f"-L{p}" for p in ce.library_paths()]
-    cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-w",
+    cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-w", "-D__HIP_PLATFORM_AMD__=1",
            "-DTORCH_EXTENSION_NAME=_C", "-DTORCH_API_INCLUDE_EXTENSION_H", f"-D_GLIBCXX_USE_CXX11_ABI={abi}",
            *inc, os.path.join(CSRC, "torch_binding.cpp"), "-o", tmp,
-           f"-L{LIBDIR}", "-lxrecon_amd", *libs, "-lc10", "-ltorch", "-ltorch_cpu", "-ltorch_python",
+           f"-L{LIBDIR}", "-lxrecon_amd", *libs, "-lc10", "-lc10_hip", "-ltorch", "-ltorch_cpu", "-ltorch_python",
            "-Wl,-rpath,$ORIGIN/_lib"]
     _run(cmd, verbose)
     os.replace(tmp, out)

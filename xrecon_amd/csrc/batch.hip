@@ -82,6 +82,28 @@ __host__ __device__ inline int blocks_split_sum(const BatchDesc &d)
     return d.va.vsplit > 1 ? ceil_div((int64_t)d.nb * d.va.nz * d.va.ny * d.va.nx, kBlock) : 0;
 }
 
+// Host check that every per-instance block count of a launch, computed in 64 bits, stays
+// below 2^31, so that the 32-bit counts above (and the kernels' walk over them) are exact.
+inline bool block_counts_fit(const BatchDesc &d, bool cell, int R, int KC)
+{
+    const int64_t lim = INT32_MAX;
+    auto cd = [](int64_t a, int64_t b) { return (a + b - 1) / b; };
+    const int64_t nb = d.nb;
+    if (cell) {
+        if (cd(nb * d.ca.nx * d.ca.ny, kBlock) >= lim) return false;
+        if (d.any_y && cd(d.ca.nx, 32) * cd(d.ca.ny, 32) * nb * (d.ca.nz + 1) >= lim) return false;
+        for (int k = 0; k < 4; ++k) {
+            const int64_t items = k == 3 ? (int64_t)d.ncls[0] + d.ncls[1] + d.ncls[2] : d.ncls[k];
+            const int rows = k == 3 ? R : cell_class_rows(R, k);
+            if (cd(items, kWavesPerBlock) * cd(d.ca.nv, rows) * nb >= lim) return false;
+        }
+        return true;
+    }
+    if (cd(d.sa.nu, kBlock) * d.sa.nview * nb >= lim) return false;
+    if (cd(d.va.nx, kVoxTile) * cd(d.va.ny, kVoxTile) * nb * cd(d.va.nz, KC) * d.va.vsplit >= lim) return false;
+    return !(d.va.vsplit > 1 && cd(nb * d.va.nz * d.va.ny * d.va.nx, kBlock) >= lim);
+}
+
 // Maps the flat block index to (instance, local block); returns the instance or -1.
 template <typename F>
 __device__ inline int find_instance(const BatchDesc *__restrict__ D, int n, int &bid, F count)
@@ -442,7 +464,10 @@ int xr_batch3d_execute(const xr_batch3d *b, int op, const float *const *in, floa
     const xr::BatchDesc *D = b->d_desc;
     int64_t g_pre = 0, g_tr = 0, g_sum = 0;
     int64_t g_gather[4] = {0, 0, 0, 0};  // voxel gathers: [0]; cell gathers: per span class, [3] = all (big)
-    for (const xr::BatchDesc &d : b->desc) {
+    for (int i = 0; i < n; ++i) {
+        const xr::BatchDesc &d = b->desc[i];
+        if (!xr::block_counts_fit(d, rt.cell, R, KC))
+            return fail(XR_EINVAL, "batch instance " + std::to_string(i) + " is too large for one launch");
         if (rt.cell) {
             g_pre += xr::blocks_vol_prefix(d);
             g_tr += xr::blocks_transpose(d);

@@ -12,9 +12,14 @@
 //  * CPU tensors are rejected: this build computes on the GPU only (the reference falls
 //    back to its std::thread CPU kernels, torch_binding.cpp:421-425).
 #include <torch/extension.h>
+#include <c10/hip/HIPStream.h>
 
+#include <cstring>
+#include <list>
 #include <memory>
+#include <mutex>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/xrecon_amd.h"
@@ -281,9 +286,125 @@ private:
     xr_batch3d *batch_ = nullptr;
 };
 
+// Plans of the reference-named functions, found without rebuilding tables:
+//  * by identity: the (options, angles) tensor objects and their version counters, each
+//    validated through a weak reference, so a repeated call with the same tensors does no
+//    device->host copy, no synchronisation and no allocation beyond its output and
+//    workspace (torch's caching allocator);
+//  * by content: the option values, the angle bytes, the device and the dimension, so
+//    equal geometry in new tensors reuses the plan after one host copy of both.
+// Plans are freed only on eviction (least recently used beyond kMaxPlans), never right
+// after a launch that reads their tables.
+class PlanCache {
+public:
+    static constexpr size_t kMaxPlans = 64;
+
+    std::shared_ptr<Plan> get(const torch::Tensor &options, const torch::Tensor &angles, int device, bool is2d)
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        const std::string ikey = identity_key(options, angles, device, is2d);
+        auto it = by_id_.find(ikey);
+        if (it != by_id_.end()) {
+            IdEntry &e = it->second->second;
+            if (e.opts.lock().get() == options.unsafeGetTensorImpl() &&
+                e.angs.lock().get() == angles.unsafeGetTensorImpl()) {
+                by_id_lru_.splice(by_id_lru_.begin(), by_id_lru_, it->second);
+                return e.plan;
+            }
+            by_id_lru_.erase(it->second);
+            by_id_.erase(it);
+        }
+        const auto o = host_floats(options, is2d ? 12 : 18, "Option");
+        auto a = angles.detach().to(torch::kCPU, torch::kFloat32).contiguous();
+        std::string vkey(reinterpret_cast<const char *>(o.data()), o.size() * sizeof(float));
+        vkey.append(reinterpret_cast<const char *>(a.data_ptr<float>()), (size_t)a.numel() * sizeof(float));
+        vkey.append(reinterpret_cast<const char *>(&device), sizeof(device));
+        vkey.push_back(is2d ? '2' : '3');
+        std::shared_ptr<Plan> plan;
+        auto vt = by_val_.find(vkey);
+        if (vt != by_val_.end()) {
+            by_val_lru_.splice(by_val_lru_.begin(), by_val_lru_, vt->second);
+            plan = vt->second->second;
+        } else {
+            plan = std::make_shared<Plan>(o, a, device, is2d);
+            by_val_lru_.emplace_front(vkey, plan);
+            by_val_[vkey] = by_val_lru_.begin();
+            evict(by_val_, by_val_lru_);
+        }
+        by_id_lru_.emplace_front(ikey, IdEntry{Weak(options.getIntrusivePtr()), Weak(angles.getIntrusivePtr()), plan});
+        by_id_[ikey] = by_id_lru_.begin();
+        evict(by_id_, by_id_lru_);
+        return plan;
+    }
+
+    size_t size() const
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        return by_val_.size();
+    }
+
+    void clear()
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        by_id_.clear();
+        by_id_lru_.clear();
+        by_val_.clear();
+        by_val_lru_.clear();
+    }
+
+private:
+    using Weak = c10::weak_intrusive_ptr<c10::TensorImpl, c10::UndefinedTensorImpl>;
+    struct IdEntry {
+        Weak opts, angs;
+        std::shared_ptr<Plan> plan;
+    };
+    template <class T>
+    using Lru = std::list<std::pair<std::string, T>>;
+    template <class T>
+    using Index = std::unordered_map<std::string, typename Lru<T>::iterator>;
+
+    static std::string identity_key(const torch::Tensor &o, const torch::Tensor &a, int device, bool is2d)
+    {
+        const int64_t f[6] = {reinterpret_cast<int64_t>(o.unsafeGetTensorImpl()), (int64_t)o._version(),
+                              reinterpret_cast<int64_t>(a.unsafeGetTensorImpl()), (int64_t)a._version(),
+                              (int64_t)device, (int64_t)is2d};
+        return std::string(reinterpret_cast<const char *>(f), sizeof(f));
+    }
+
+    template <class T>
+    static void evict(Index<T> &index, Lru<T> &lru)
+    {
+        while (lru.size() > kMaxPlans) {
+            index.erase(lru.back().first);
+            lru.pop_back();
+        }
+    }
+
+    mutable std::mutex mu_;
+    Lru<IdEntry> by_id_lru_;
+    Index<IdEntry> by_id_;
+    Lru<std::shared_ptr<Plan>> by_val_lru_;
+    Index<std::shared_ptr<Plan>> by_val_;
+};
+
+PlanCache &plan_cache()
+{
+    static PlanCache *cache = new PlanCache();  // never destroyed: plans outlive interpreter teardown order
+    return *cache;
+}
+
+// stream < 0: torch's current stream on `device` (the default of the reference-named
+// functions); otherwise the hipStream_t handle as an integer (0 = legacy default stream).
+int64_t resolve_stream(int64_t stream, int device)
+{
+    if (stream >= 0) return stream;
+    return reinterpret_cast<int64_t>(c10::hip::getCurrentHIPStream((c10::DeviceIndex)device).stream());
+}
+
 // Reference-named operator functions: (input, options, angles[, stream]) -> output, as in
-// src/api/torch_binding.cpp:407-523 (3D) and :139-351 (2D).  They build a plan per call
-// (host geometry tables); xrecon_amd.Projector3d/Projector2d cache plans instead.
+// src/api/torch_binding.cpp:407-523 (3D) and :139-351 (2D).  The plan comes from the
+// cache above; the output and workspace from torch's caching allocator; the work goes to
+// torch's current stream unless a stream is given.
 torch::Tensor run_op(int op, bool is2d, const torch::Tensor &x, const torch::Tensor &options,
                      const torch::Tensor &angles, int64_t stream)
 {
@@ -291,11 +412,15 @@ torch::Tensor run_op(int op, bool is2d, const torch::Tensor &x, const torch::Ten
                              "(CPU tensors are not supported by this build)");
     TORCH_CHECK(!angles.is_cuda() || angles.get_device() == x.get_device(),
                 "Expected all tensors on the same device");
+    TORCH_CHECK(!options.is_cuda() || options.get_device() == x.get_device(),
+                "Expected all tensors on the same device");
     TORCH_CHECK(options.scalar_type() == torch::kFloat32, "options must be float32");
     TORCH_CHECK(angles.scalar_type() == torch::kFloat32, "ang must be float32");
     TORCH_CHECK(angles.is_contiguous() && options.is_contiguous(), "Expected all tensors to be contiguous");
-    Plan plan(host_floats(options, is2d ? 12 : 18, "Option"), angles, x.get_device(), is2d);
-    return plan.execute(op, x, stream);
+    TORCH_CHECK(angles.dim() == 1, "Expected angles shape [V], got ", angles.sizes());
+    const int dev = x.get_device();
+    auto plan = plan_cache().get(options, angles, dev, is2d);
+    return plan->execute(op, x, resolve_stream(stream, dev));
 }
 
 // system_matrix2d / system_matrix3d (src/api/torch_binding.cpp:353-405): CPU tensors in,
@@ -340,14 +465,15 @@ torch::Tensor system_matrix(bool is2d, const torch::Tensor &options, const torch
         [](const torch::Tensor &x, const torch::Tensor &o, const torch::Tensor &a, int64_t stream) {         \
             return run_op(op, is2d, x, o, a, stream);                                                        \
         },                                                                                                   \
-        py::arg("input"), py::arg("options"), py::arg("angles"), py::arg("stream") = 0)
+        py::arg("input"), py::arg("options"), py::arg("angles"), py::arg("stream") = -1)
 
 }  // namespace
 
 PYBIND11_MODULE(TORCH_EXTENSION_NAME, m)
 {
     m.doc() = "xrecon_amd native binding over libxrecon_amd.so (gfx950)";
-    py::class_<Plan, std::shared_ptr<Plan>>(m, "Plan")
+    // dynamic_attr: per-plan Python state (the autotuner keeps its choices on the plan).
+    py::class_<Plan, std::shared_ptr<Plan>>(m, "Plan", py::dynamic_attr())
         .def(py::init([](const std::vector<float> &options, const torch::Tensor &angles, int device, bool is2d) {
                  return std::make_shared<Plan>(options, angles, device, is2d);
              }),
@@ -384,6 +510,9 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m)
     m.attr("OP_BACKPROJECTION_T") = XR_OP_BACKPROJECTION_T;
     m.attr("OP_BACKPROJECTION_W_T") = XR_OP_BACKPROJECTION_W_T;
     m.def("version", []() { return std::string(xr_version()); });
+    m.def("plan_cache_size", []() { return (int64_t)plan_cache().size(); },
+          "number of distinct geometries cached by the reference-named functions");
+    m.def("plan_cache_clear", []() { plan_cache().clear(); });
 
     m.def("system_matrix2d", [](const torch::Tensor &o, const torch::Tensor &a) { return system_matrix(true, o, a); },
           "2D distance-driven projection matrix");
