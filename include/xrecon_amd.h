@@ -173,6 +173,21 @@ int xr_batch3d_execute(const xr_batch3d *batch, int op, const float *const *in, 
                        void *workspace, size_t workspace_bytes, int tile_rows, int voxel_chunk,
                        void *stream);
 
+/* ------------------------------------------------- execution without allocation */
+
+/* No counterpart in the reference.  For callers that cannot allocate device memory on the
+ * call path (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc): xr_plan3d_execute with the workspace taken from a fixed device array of
+ * XR_NOALLOC_SCRATCH_BYTES inside the library instead of from the caller.  The caller
+ * still passes the plan (created beforehand), the input and the output.  Operators whose
+ * xr_plan3d_workspace_bytes() exceeds the limit fail with XR_EWORKSPACE and launch
+ * nothing.  Each GPU holds its own array from the first call on it until the process
+ * ends.  Calls that overlap in time on one GPU (several streams or threads) race on that
+ * GPU's array: the caller serialises them. */
+#define XR_NOALLOC_SCRATCH_BYTES ((size_t)256 << 20)
+int xr_plan3d_execute_noalloc(const xr_plan3d *plan, int op, const float *in, float *out, int num_batch,
+                              void *stream);
+
 /* Same as xr_plan3d_execute with explicit tiling parameters (see above); used by the
  * autotuner. */
 int xr_plan3d_execute_tuned(const xr_plan3d *plan, int op, const float *in, float *out,
