@@ -6,6 +6,7 @@
 // against the oracle without a GPU; the GPU suite checks the compiled kernels.  It is
 // not part of the product library and nothing in xrecon_amd loads it.
 #include <algorithm>
+#include <atomic>
 #include <cstring>
 #include <string>
 #include <limits>
@@ -21,6 +22,9 @@ namespace {
 // Cell-gather variant (see run_cell_RV): 0 = as launched, 1 = general 32-bit, 2 = general
 // 64-bit, 3 = the staged wave variant for fixed-span groups.
 int g_cell_variant = 0;
+
+// Number of (workgroup, view) pairs the voxel gather passed over (VoxBox::skip).
+std::atomic<long long> g_voxel_skips{0};
 
 // One wave of the staged fixed-span cell gather, lanes in lockstep per line: the block
 // estimate, every lane's staging share, then every lane's gather -- the order the wave's
@@ -151,6 +155,10 @@ void run_voxel_wg_KC(const Plan3d &p, const float2 *Q, float *out, int nb, int n
                     const float4 *ct = p.d_col + (int64_t)v * (a.nu + 1);
                     const float2 *Qv = view_table(Q, a, b, v);
                     const VoxBox box = voxel_box<KC>(p.d_vtile[((int64_t)v * gy + by) * gx + bx], a, k0);
+                    if (box.skip) {
+                        g_voxel_skips.fetch_add(1);
+                        continue;
+                    }
                     for (int tid = 0; tid < kBlock; ++tid) voxel_stage(Qv, box, tile.data(), a.nu, tid, kBlock);
                     for (int tid = 0; tid < kBlock; ++tid) {
                         int i, j;
@@ -297,6 +305,13 @@ const char *emu_error(void) { return g_err.c_str(); }
 void emu_set_cell_variant(int v) { g_cell_variant = v; }
 
 void emu_set_voxel_variant(int v) { g_voxel_variant = v; }
+
+long long emu_voxel_skips(int reset)
+{
+    const long long n = g_voxel_skips.load();
+    if (reset) g_voxel_skips = 0;
+    return n;
+}
 
 // The fused FDK pass (xr_plan3d_execute_fdk) on the CPU: `in` [nb, V, Nv, Nu], filter taps
 // (scaled, 2 Nu - 1), column weights (Nu), `out` [nb, Nz, Ny, Nx].

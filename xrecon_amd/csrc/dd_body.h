@@ -280,12 +280,24 @@ __host__ __device__ inline float fractf(float x)
 #endif
 }
 
+// 1 / y where an error of about one ulp is harmless: v_rcp_f32 on the device (a true
+// division compiles to a ten-instruction correctly rounded sequence), a division on the
+// host.
+__host__ __device__ inline float approx_rcp(float y)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_rcpf(y);
+#else
+    return 1.0f / y;
+#endif
+}
+
 // x / y where the quotient only seeds a search that is corrected against exact tables:
-// v_rcp_f32 times x on the device (about 1 ulp), a true division on the host.
+// x times v_rcp_f32(y) on the device (about 2 ulp), a true division on the host.
 __host__ __device__ inline float seed_div(float x, float y)
 {
 #if defined(__HIP_DEVICE_COMPILE__)
-    return __fdividef(x, y);
+    return x * __builtin_amdgcn_rcpf(y);
 #else
     return x / y;
 #endif
@@ -870,9 +882,11 @@ __host__ __device__ inline float column_seed(const ViewHdr &h, float a)
 
 // Block of one view's row-prefix table staged in LDS for a voxel-gather workgroup:
 // columns c0 .. c0+nc-1 and rows r0 .. r0+nr-1, stored row-major with an even pitch >= nc
-// (whole 16-byte pairs per row).  nc = 0: nothing staged.
+// (whole 16-byte pairs per row).  nc = 0: nothing staged.  skip = 1: no voxel of the
+// workgroup's tile and slice chunk reaches the detector on this view (bounds, not
+// estimates: see VoxTile), so the workgroup passes over the view.
 struct VoxBox {
-    int c0, nc, r0, nr, pitch;
+    int c0, nc, r0, nr, pitch, skip;
 };
 
 // Capacity of the staged block, in float2 entries (20 KiB of LDS per workgroup).  The
@@ -880,9 +894,14 @@ struct VoxBox {
 // isocentre) detector stay below 2400 entries.
 constexpr int kVoxLds = 2560;
 
-// The tile estimate of VoxTile for voxel columns [i0, i1] x [j0, j1] on view v, from the
-// tile's corner pixels: the axis map of a pixel edge is monotone in the pixel and in the
-// line, and the depth magnification zeta monotone in the line.  Computed once per plan.
+// VoxTile for voxel columns [i0, i1] x [j0, j1] on view v, from the tile's corner pixels:
+// the axis map of a pixel edge is monotone in the pixel and in the line (a projective
+// map away from its pole), and the depth magnification zeta monotone in the line.  The
+// column range carries a one-column margin over the closed-form inverse's rounding, and
+// the depth-scale range is taken over every column of it, so both bound every voxel of
+// the tile (not only the corners): the voxel gather skips a view on them.  nc = -1: the
+// tile's footprint misses the detector (by more than a thousandth of a column).
+// Computed once per plan.
 template <int GEO>
 inline VoxTile voxel_tile(const ViewHdr &h, const float4 *lineV, const float4 *ct, int ny, int nu, int maxline, int v,
                           int i0, int i1, int j0, int j1)
@@ -894,11 +913,20 @@ inline VoxTile voxel_tile(const ViewHdr &h, const float4 *lineV, const float4 *c
     const float4 la = lineV[(int64_t)v * maxline + La], lb = lineV[(int64_t)v * maxline + Lb];
     const float alo = fminf(fmaf((float)mlo, la.y, la.x), fmaf((float)mlo, lb.y, lb.x));
     const float ahi = fmaxf(fmaf((float)(mhi + 1), la.y, la.x), fmaf((float)(mhi + 1), lb.y, lb.x));
-    if (!(ahi > h.dlo && alo < h.dhi)) return t;
+    const float tol = 1e-3f * (h.dhi - h.dlo) / (float)nu;
+    if (!(ahi > h.dlo - tol && alo < h.dhi + tol)) {
+        t.nc = -1;
+        return t;
+    }
     const int c0 = imax((int)clampf(column_seed<GEO>(h, alo), 0.0f, (float)(nu - 1)) - 1, 0);
     const int c1 = imin((int)clampf(column_seed<GEO>(h, ahi), 0.0f, (float)(nu - 1)) + 1, nu - 1);
     if (c1 < c0) return t;
-    const float zs[4] = {la.z * ct[c0].z, la.z * ct[c1].z, lb.z * ct[c0].z, lb.z * ct[c1].z};
+    float klo = ct[c0].z, khi = klo;
+    for (int c = c0 + 1; c <= c1; ++c) {
+        klo = fminf(klo, ct[c].z);
+        khi = fmaxf(khi, ct[c].z);
+    }
+    const float zs[4] = {la.z * klo, la.z * khi, lb.z * klo, lb.z * khi};
     t.c0 = c0;
     t.nc = c1 - c0 + 1;
     t.zk_lo = fminf(fminf(zs[0], zs[1]), fminf(zs[2], zs[3]));
@@ -909,15 +937,24 @@ inline VoxTile voxel_tile(const ViewHdr &h, const float4 *lineV, const float4 *c
 // The block a voxel-gather workgroup stages for slices [k0, k0 + KC): the tile's columns
 // and the rows rho = zeta kappa z - v0/dvv of its chunk's bottom and top boundaries over
 // the tile's depth-scale range, with a margin of one row, clipped to the LDS capacity.
+//
+// skip is set when the tile misses the detector, or when the chunk's rows lie more than
+// one row outside [0, Nv] over the tile's whole depth-scale range (a voxel lane then fails
+// its own test rhi > 0 && rlo < Nv on every column: rounding moves its rows by far less
+// than a row).
 template <int KC>
 __host__ __device__ inline VoxBox voxel_box(const VoxTile &t, const VoxArgs &a, int k0)
 {
-    VoxBox bx{0, 0, 0, 0, 0};
+    VoxBox bx{0, 0, 0, 0, 0, 0};
+    if (t.nc < 0) bx.skip = 1;
     if (t.nc <= 0) return bx;
     const float zlo = fmaf((float)k0, a.dz, a.zb), zhi = fmaf((float)(k0 + KC), a.dz, a.zb);
     float lo = fminf(zlo * t.zk_lo, zlo * t.zk_hi) - a.rv0;
     float hi = fmaxf(zhi * t.zk_lo, zhi * t.zk_hi) - a.rv0;
-    if (!(hi > -1.0f && lo < (float)a.nv + 1.0f)) return bx;
+    if (!(hi > -1.0f && lo < (float)a.nv + 1.0f)) {
+        bx.skip = 1;
+        return bx;
+    }
     lo = fmaxf(lo, -1.0f);
     hi = fminf(hi, (float)a.nv + 1.0f);
     const int r0 = imax((int)floorf(lo) - 1, 0);
@@ -953,9 +990,10 @@ __host__ __device__ inline void voxel_stage(const float2 *__restrict__ Qv, const
     const char *Qb = reinterpret_cast<const char *>(Qv);
     const unsigned nu8 = (unsigned)nu * 8u;
     // (row, pair) of p = tid, advanced by nthreads = dq rows + dc pairs per step.  The
-    // first division: (tid + 1/2) / half stays >= 1/(2 half) from an integer, far above
-    // the rounding of the product.
-    int r = (int)(((float)tid + 0.5f) * (1.0f / (float)half));
+    // first division: (tid + 1/2) / half stays >= 1/(2 half) >= 1/2560 from an integer,
+    // far above the error of the approximate reciprocal and product (tid < 2^10:
+    // below 2^10 * 2^-22).
+    int r = (int)(((float)tid + 0.5f) * approx_rcp((float)half));
     int c = tid - r * half;
     const int dq = nthreads / half, dc = nthreads - dq * half;
     const unsigned cb = (unsigned)bx.c0 * 8u;
@@ -977,8 +1015,80 @@ __host__ __device__ inline void voxel_stage(const float2 *__restrict__ Qv, const
     }
 }
 
+// True on every lane if the predicate holds on any active lane of the wave (device); the
+// predicate itself on the host, where each thread runs alone.
+__host__ __device__ inline bool wave_any(bool p)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_ballot_w64(p) != 0;
+#else
+    return p;
+#endif
+}
+
+// Where one column's row-prefix entries are read from: the staged LDS block (when the
+// column lies in it) or the view's table in HBM.
+struct VoxCol {
+    const char *Qb;   // view's table
+    unsigned nu8;     // byte stride of a table row
+    unsigned c8;      // byte offset of the column in a table row
+    const char *Lb;   // LDS block, or nullptr when the column is not staged
+    unsigned lcb;     // (c - c0) * 8 - r0 * pitch * 8: LDS byte offset of row 0 (mod 2^32)
+    unsigned nc8;     // LDS row pitch in bytes
+    int r0, r1;       // staged rows [r0, r1)
+};
+
+// Slice boundaries q = Q0..Q1 of one column: rows rho_q = clamp(rlo + q step, 0, Nv), one
+// row-prefix lookup each, I(rho) = B_r + rho p_r; accumulates slices Q0..Q1-1 with weight
+// w and returns I at boundary Q1.  With have_first the caller already holds I at Q0
+// (Ifirst) and that lookup is not repeated.  All lookups are independent: the rows are
+// computed, the loads issued back to back, then consumed.
+template <int Q0, int Q1>
+__host__ __device__ inline float voxel_slices(const VoxCol &cs, float rlo, float step, float fnv, float w,
+                                              bool have_first, float Ifirst, float *acc)
+{
+    constexpr int N = Q1 - Q0;
+    int ri[N + 1];
+    float rh[N + 1];
+#pragma unroll
+    for (int q = 0; q <= N; ++q) {
+        rh[q] = clampf(fmaf((float)(Q0 + q), step, rlo), 0.0f, fnv);
+        ri[q] = (int)rh[q];
+    }
+    float2 e[N + 1];
+    // rows increase with q, so the first and last lookups bound the span's rows
+    const bool staged = cs.Lb != nullptr && ri[0] >= cs.r0 && ri[N] < cs.r1;
+    if (staged) {
+        // byte offset (ri - r0) * pitch * 8 + (c - c0) * 8 as one 24-bit multiply-add
+        // (unsigned wrap-around of the constant part is exact)
+        if (!have_first) e[0] = *reinterpret_cast<const float2 *>(cs.Lb + mad24((unsigned)ri[0], cs.nc8, cs.lcb));
+#pragma unroll
+        for (int q = 1; q <= N; ++q)
+            e[q] = *reinterpret_cast<const float2 *>(cs.Lb + mad24((unsigned)ri[q], cs.nc8, cs.lcb));
+    } else {
+        if (!have_first) e[0] = *reinterpret_cast<const float2 *>(cs.Qb + (mul24((unsigned)ri[0], cs.nu8) + cs.c8));
+#pragma unroll
+        for (int q = 1; q <= N; ++q)
+            e[q] = *reinterpret_cast<const float2 *>(cs.Qb + (mul24((unsigned)ri[q], cs.nu8) + cs.c8));
+    }
+    // I(rho) = B_r + rho q_r (see prefix_column)
+    float Ip = have_first ? Ifirst : fmaf(rh[0], e[0].y, e[0].x);
+#pragma unroll
+    for (int q = 0; q < N; ++q) {
+        const float In = fmaf(rh[q + 1], e[q + 1].y, e[q + 1].x);
+        acc[Q0 + q] = fmaf(w, In - Ip, acc[Q0 + q]);
+        Ip = In;
+    }
+    return Ip;
+}
+
 // One view's contribution to the KC slice accumulators of voxel column (i, j).  With a
 // staged block (lds != nullptr), lookups inside it read LDS, the rest the table in HBM.
+//
+// Chunks of 16 slices run as two halves of 8, each skipped when no lane of the wave has
+// a row of that half on the detector: every boundary of a skipped half then clamps to
+// the same row (0 below, Nv above) for every lane, so its differences are exactly zero
+// and skipping it changes no result.
 template <int OPK, int GEO, int KC>
 __host__ __device__ inline void voxel_view(const ViewHdr &h, const float4 *__restrict__ lineV,
                                            const float4 *__restrict__ ct, const float2 *__restrict__ Qv,
@@ -1033,40 +1143,20 @@ __host__ __device__ inline void voxel_view(const ViewHdr &h, const float4 *__res
                 // B: ou * U w sigma dvv / (Delta zeta dz); A^T: ou (normalisation folded
                 // into the prefix table).
                 const float w = (OPK == VOX_AT) ? ou : ou * hB * cc.y;
-                // All KC+1 boundary lookups are independent: compute the rows, issue the
-                // loads back to back, then consume.
-                int ri[KC + 1];
-                float rh[KC + 1];
-#pragma unroll
-                for (int q = 0; q <= KC; ++q) {
-                    rh[q] = clampf(fmaf((float)q, step, rlo), 0.0f, fnv);
-                    ri[q] = (int)rh[q];
-                }
-                float2 e[KC + 1];
-                // rows increase with q, so the first and last lookups bound the column's rows
-                const bool staged = lds != nullptr && (unsigned)(c - bx.c0) < (unsigned)bx.nc && ri[0] >= bx.r0 &&
-                                    ri[KC] < bx.r0 + bx.nr;
-                if (staged) {
-                    // byte offset (ri - r0) * pitch * 8 + (c - c0) * 8 as one 24-bit
-                    // multiply-add (unsigned wrap-around of the constant part is exact)
-                    const char *Lb = reinterpret_cast<const char *>(lds);
-                    const unsigned cb = (unsigned)(c - bx.c0) * 8u - (unsigned)bx.r0 * nc8;
-#pragma unroll
-                    for (int q = 0; q <= KC; ++q)
-                        e[q] = *reinterpret_cast<const float2 *>(Lb + mad24((unsigned)ri[q], nc8, cb));
+                const bool in_lds = lds != nullptr && (unsigned)(c - bx.c0) < (unsigned)bx.nc;
+                const VoxCol cs{Qb, nu8, (unsigned)c * 8u, in_lds ? reinterpret_cast<const char *>(lds) : nullptr,
+                                (unsigned)(c - bx.c0) * 8u - (unsigned)bx.r0 * nc8, nc8, bx.r0, bx.r0 + bx.nr};
+                if (KC < 16) {
+                    voxel_slices<0, KC>(cs, rlo, step, fnv, w, false, 0.0f, acc);
                 } else {
-                    const unsigned c8 = (unsigned)c * 8u;
-#pragma unroll
-                    for (int q = 0; q <= KC; ++q)
-                        e[q] = *reinterpret_cast<const float2 *>(Qb + (mul24((unsigned)ri[q], nu8) + c8));
-                }
-                // I(rho) = B_r + rho q_r (see prefix_column)
-                float Ip = fmaf(rh[0], e[0].y, e[0].x);
-#pragma unroll
-                for (int q = 0; q < KC; ++q) {
-                    const float In = fmaf(rh[q + 1], e[q + 1].y, e[q + 1].x);
-                    acc[q] = fmaf(w, In - Ip, acc[q]);
-                    Ip = In;
+                    constexpr int H = KC / 2;
+                    const float rmid = fmaf((float)H, step, rlo);
+                    // this lane's lower half reaches row 0 iff rmid > 0 (its chunk starts
+                    // below Nv), the upper half iff rmid < Nv (its chunk ends above 0)
+                    const bool lo = wave_any(rmid > 0.0f), hi = wave_any(rmid < fnv);
+                    float Imid = 0.0f;
+                    if (lo) Imid = voxel_slices<0, H>(cs, rlo, step, fnv, w, false, 0.0f, acc);
+                    if (hi) voxel_slices<H, KC>(cs, rlo, step, fnv, w, lo, Imid, acc);
                 }
             }
         }
@@ -1107,7 +1197,7 @@ __host__ __device__ inline void voxel_body(const ViewHdr *__restrict__ hdr, cons
     const float px = voxel_px(a, i), py = voxel_py(a, j);
     float acc[KC];
     for (int q = 0; q < KC; ++q) acc[q] = 0.0f;
-    const VoxBox none{0, 0, 0, 0, 0};
+    const VoxBox none{0, 0, 0, 0, 0, 0};
     for (int v = v0; v < v1; ++v)
         voxel_view<OPK, GEO, KC>(hdr[v], lineV, colt + (int64_t)v * (a.nu + 1), view_table(Q, a, b, v), a, v, i, j,
                                  k0, px, py, acc, none, nullptr);
@@ -1146,6 +1236,7 @@ __device__ inline void voxel_wg(const ViewHdr *__restrict__ hdr, const float4 *_
         const float4 *ct = colt + (int64_t)v * (a.nu + 1);
         const float2 *Qv = view_table(Q, a, b, v);
         const VoxBox bx = voxel_box<KC>(vtile[(int64_t)v * ntile + tile_id], a, k0);
+        if (bx.skip) continue;  // the same for every thread of the workgroup: no barrier is missed
         voxel_stage(Qv, bx, tile, a.nu, threadIdx.x, kBlock);
         __builtin_amdgcn_s_waitcnt(0);  // this wave's direct-to-LDS loads have landed
         __syncthreads();
