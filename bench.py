@@ -239,23 +239,39 @@ def bench_cfg1(dev, reps, with_cpu):
     return out
 
 
-def bench_cfg5(dev, reps):
-    """Cone-arc: 512^2 x 256 at 0.7 mm, 736 x 128 detector (1.2 mm / 1.0 mm), SOD 595 mm,
-    SDD 1085.6 mm (du = 1.105e-3 rad), 1152 views, batch 4, seeded random ellipsoids.
-    A + B and the adjoint pair A + A^T with the default tiles (no autotuning at this size)."""
+CFG5_WORKLOAD = "cone_arc_512x512x256_736x128_1152v"
+
+
+def cfg5_setup(dev):
+    """Config 5: cone-arc 512^2 x 256 at 0.7 mm, arc detector 736 x 128 (1.2 mm / 1.0 mm),
+    SOD 595 mm, SDD 1085.6 mm (du = 1.105e-3 rad), 1152 views over 2 pi, batch 4: four
+    phantoms of 32 seeded random ellipsoids (intensities 0.0-0.05 per mm, seeds 1005..1008)."""
     import torch
 
     import xrecon_amd
-    from xrecon_amd import _backend as be
     from xrecon_amd.phantom import random_ellipsoids
 
     W = H = 512
     D, Nu, Nv, V, NB = 256, 736, 128, 1152, 4
     proj = xrecon_amd.Projector3d((W, H, D), (Nu, Nv), 0.7, (1.2, 1.0), 595.0, 1085.6, scan_type="arc")
     angles = torch.arange(V, dtype=torch.float32) * np.float32(2 * np.pi / V)
-    ph = random_ellipsoids((W, H, D), (0.7, 0.7, 0.7), count=16, semi_axes=(5.0, 120.0), seed=1005)
-    x = torch.from_numpy(ph).to(dev).view(1, 1, D, H, W).expand(NB, 1, D, H, W).contiguous()
-    del ph
+    x = torch.stack([random_ellipsoids((W, H, D), (0.7, 0.7, 0.7), count=32, semi_axes=(5.0, 120.0),
+                                       intensity=(0.0, 0.05), seed=1005 + b, device=dev) for b in range(NB)])
+    return proj, angles, x.view(NB, 1, D, H, W)
+
+
+def bench_cfg5(dev, reps):
+    """Config 5 operators: A + B and the adjoint pair A + A^T at batch 4 with the default
+    tiles (no autotuning at this size); the dominant gather's algorithmic FLOP rate from
+    the counted nnz (an estimate from 36 evenly spaced views, scaled to 1152)."""
+    import torch
+
+    from xrecon_amd import _backend as be
+
+    proj, angles, x = cfg5_setup(dev)
+    NB, _, D, H, W = x.shape
+    Nu, Nv = proj.num_det
+    V = angles.numel()
     plan = proj.plan(angles, dev)
     sid = torch.cuda.current_stream(dev).cuda_stream
     stream = torch.cuda.current_stream(dev)
@@ -268,11 +284,100 @@ def bench_cfg5(dev, reps):
     t_b = _time_gpu(lambda: plan.execute_into(B, sino, rec, ws, sid, 0, 0, 0), reps, stream)
     t_at = _time_gpu(lambda: plan.execute_into(AT, sino, rec, ws, sid, 0, 0, 0), reps, stream)
     vv = W * H * D * V * NB
-    return {"workload": "cone-arc 512^2x256 @0.7mm, 736x128 detector (1.2/1.0 mm), SOD 595, SDD 1085.6, "
-                        "1152 views, B=4, 16 random ellipsoids (seed 1005)",
-            "voxel_views_per_operator": vv, "value": vv / ((t_a + t_b) * 1e-3), "unit": "voxel-views/s",
-            "ms_A": t_a, "ms_B": t_b, "ms_AT": t_at, "adjoint_value": vv / ((t_a + t_at) * 1e-3),
-            "tiles": "defaults (8 rows, 16 slices)"}
+    out = {"workload": "cone-arc 512^2x256 @0.7mm, 736x128 detector (1.2/1.0 mm), SOD 595, SDD 1085.6, "
+                       "1152 views, B=4, 32 random ellipsoids each (intensities 0-0.05/mm, seeds 1005-1008)",
+           "voxel_views_per_operator": vv, "value": vv / ((t_a + t_b) * 1e-3), "unit": "voxel-views/s",
+           "ms_A": t_a, "ms_B": t_b, "ms_AT": t_at, "adjoint_value": vv / ((t_a + t_at) * 1e-3),
+           "tiles": "defaults (8 rows, 16 slices)"}
+    try:
+        nnz = nnz_of(CFG5_WORKLOAD) * NB
+    except (KeyError, OSError):
+        nnz = None
+    if nnz:
+        t_dom, name = max((t_a, "cell_gather(A)"), (t_b, "voxel_gather(B)"))
+        achieved = 4.0 * nnz / (t_dom * 1e-3) / 1e12
+        out["roofline"] = {"bound": "valu", "kernel": name + ", whole pass (prefix included)",
+                           "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                           "frac": achieved / FP32_PEAK_TFLOPS, "frac_nonpacked": achieved / (FP32_PEAK_TFLOPS / 2),
+                           "nnz_per_operator": nnz,
+                           "nnz_note": "counted by the FP64 oracle on 36 evenly spaced views, scaled to 1152, x4"}
+    return out
+
+
+def bench_cfg4(dev, proj, angles_dev, x_true, steps=50):
+    """Config 4: differentiable reconstruction on the cone-flat 256^2 x 64, 720-view
+    geometry.  y = -ln(max(Poisson(I0 exp(-A x*)), 1) / I0) at I0 = 1e5 (the clamp at one
+    count is the recorded choice); then `steps` Adam steps (lr 1e-2) on
+    0.5 ||A x - y||^2 + 1e-3 TV(x), x fp32 with requires_grad, gradients through the
+    hand-written adjoint (A^T) kernels.  Timed on the host clock around synchronisations."""
+    import torch
+
+    from xrecon_amd.recon import reconstruct, simulate_transmission
+
+    y = simulate_transmission(proj, x_true, angles_dev, I0=1e5, seed=1004)
+    losses = []
+    reconstruct(proj, y, angles_dev, steps=2, lr=1e-2, tv_weight=1e-3)  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    rec = reconstruct(proj, y, angles_dev, steps=steps, lr=1e-2, tv_weight=1e-3,
+                      callback=lambda k, loss: losses.append(loss))
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    losses = [float(v) for v in losses]
+    W, H, D = proj.image_size
+    vv = W * H * D * angles_dev.numel()
+    return {"workload": "cone-flat 256^2x64 / 720 views, Poisson I0=1e5 + log (counts clamped at 1), "
+                        "Adam lr 1e-2 on 0.5||Ax-y||^2 + 1e-3 TV(x)",
+            "adam_steps": steps, "ms_per_step": t * 1e3 / steps, "value": vv / (t / steps),
+            "unit": "voxel-views/s (per operator, A + A^T + TV + Adam per step)",
+            "loss_first": losses[0], "loss_last": losses[-1],
+            "loss_decreasing": bool(losses[-1] < losses[0]),
+            "loss_monotone_steps": int(sum(b <= a for a, b in zip(losses, losses[1:]))),
+            "finite": bool(torch.isfinite(rec).all().item())}
+
+
+def bench_cfg5_unrolled(dev, iterations=10):
+    """Config 5 unrolled reconstruction: `iterations` gradient steps
+    x <- x - a_k A^T (A x - y) at batch 4 through autograd (a_k learnable, initialised
+    to 1/L with L from a power iteration on A^T A), loss ||x_K - x*||^2, forward and
+    backward; with per-iteration checkpointing and, for comparison, without.  Reports time
+    and peak device memory of each."""
+    import torch
+
+    from xrecon_amd.recon import UnrolledGradientDescent
+
+    proj, angles, x_true = cfg5_setup(dev)
+    angles = angles.to(dev)
+    with torch.no_grad():
+        y = proj.projection(x_true, angles)
+        v = x_true[:1].clone()
+        for _ in range(6):
+            w = proj.projection_t(proj.projection(v, angles), angles)
+            lip = (w.norm() / v.norm()).item()
+            v = w / w.norm()
+        del v, w
+    out = {"workload": "cone-arc 512^2x256 / 1152 views, batch 4, %d unrolled gradient iterations" % iterations,
+           "step_init": 1.0 / lip}
+    for ckpt in (True, False):
+        net = UnrolledGradientDescent(proj, angles, iterations=iterations, step=1.0 / lip,
+                                      use_checkpoint=ckpt).to(dev)
+        torch.cuda.synchronize()
+        torch.cuda.reset_peak_memory_stats(dev)
+        base = torch.cuda.memory_allocated(dev)
+        t0 = time.perf_counter()
+        rec = net(y)
+        loss = ((rec - x_true) ** 2).sum()
+        loss.backward()
+        torch.cuda.synchronize()
+        t = time.perf_counter() - t0
+        key = "checkpointed" if ckpt else "stored_activations"
+        out[key] = {"seconds_forward_backward": t,
+                    "peak_memory_GiB": (torch.cuda.max_memory_allocated(dev) - base) / 2**30,
+                    "loss": float(loss), "loss_initial": float((x_true ** 2).sum()),
+                    "grad_finite": bool(torch.isfinite(net.steps.grad).all().item())}
+        del net, rec, loss
+        torch.cuda.empty_cache()
+    return out
 
 
 def traffic_from_profiles(kernel):
@@ -476,8 +581,12 @@ def main():
     batched = None if args.no_batched else bench_batched(dev, max(2, K // 2), args.verbose)
     configs = None
     if not args.no_configs:
-        configs = {"cfg1_fan_flat_2d": bench_cfg1(dev, max(3, K), not args.no_cpu),
-                   "cfg5_cone_arc": bench_cfg5(dev, 2)}
+        configs = {"cfg1_fan_flat_2d": bench_cfg1(dev, max(3, K), not args.no_cpu)}
+        configs["cfg4_adam_reconstruction"] = bench_cfg4(dev, proj, angles_dev, x)
+        torch.cuda.empty_cache()
+        configs["cfg5_cone_arc"] = bench_cfg5(dev, 2)
+        torch.cuda.empty_cache()
+        configs["cfg5_unrolled"] = bench_cfg5_unrolled(dev)
         torch.cuda.empty_cache()
     compulsory = 4.0 * (vox_views / wl["V"] + wl["V"] * wl["Nv"] * wl["Nu"])
 

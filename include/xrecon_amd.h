@@ -237,6 +237,44 @@ int xr_op2d(int op, const float *in, float *out, const float *angles_host, const
 int xr_geometry2d_as_3d(const xr_volume2d *vol2, const xr_geometry2d *geo2, xr_volume3d *vol3,
                         xr_geometry3d *geo3);
 
+/* ------------------------------------------------- host execution (CPU tensors) */
+
+/* The operators on host memory, for inputs that live on the CPU: the reference runs its
+ * std::thread CPU kernels for CPU tensors (src/api/torch_binding.cpp:421-425).  Here the
+ * host runs the same per-thread bodies as the GPU kernels on the same FP64-built tables,
+ * on `num_threads` threads (0 = all hardware threads).  Pointers are host pointers. */
+typedef struct xr_host_plan3d xr_host_plan3d;
+int xr_host_plan3d_create(const xr_volume3d *vol, const xr_geometry3d *geo, const float *angles_host,
+                          int num_view, xr_host_plan3d **out);
+void xr_host_plan3d_destroy(xr_host_plan3d *plan);
+int xr_host_plan3d_execute(const xr_host_plan3d *plan, int op, const float *in, float *out, int num_batch,
+                           int num_threads);
+
+/* include/functions.h:249  forward3d_cpu(image, proj, ang, vol, geo, num_batch, num_view) */
+int xr_forward3d_cpu(const float *image, float *proj, const float *angles_host, const xr_volume3d *vol,
+                     const xr_geometry3d *geo, int num_batch, int num_view);
+/* include/functions.h:250  forward3d_t_cpu */
+int xr_forward3d_t_cpu(float *image, const float *proj, const float *angles_host, const xr_volume3d *vol,
+                       const xr_geometry3d *geo, int num_batch, int num_view);
+/* include/functions.h:251  backward3d_cpu */
+int xr_backward3d_cpu(float *image, const float *proj, const float *angles_host, const xr_volume3d *vol,
+                      const xr_geometry3d *geo, int num_batch, int num_view);
+/* include/functions.h:252  weighted_backward3d_cpu */
+int xr_weighted_backward3d_cpu(float *image, const float *proj, const float *angles_host,
+                               const xr_volume3d *vol, const xr_geometry3d *geo, int num_batch,
+                               int num_view);
+/* include/functions.h:253  backward3d_t_cpu(proj, image, ...) */
+int xr_backward3d_t_cpu(float *proj, const float *image, const float *angles_host, const xr_volume3d *vol,
+                        const xr_geometry3d *geo, int num_batch, int num_view);
+/* include/functions.h:254  weighted_backward3d_t_cpu(proj, image, ...) */
+int xr_weighted_backward3d_t_cpu(float *proj, const float *image, const float *angles_host,
+                                 const xr_volume3d *vol, const xr_geometry3d *geo, int num_batch,
+                                 int num_view);
+/* include/functions.h:72-165  forward2d_cpu ... weighted_backward2d_t_cpu: image
+ * [nb, Ny, Nx] <-> sinogram [nb, num_view, Nu] on the host. */
+int xr_op2d_cpu(int op, const float *in, float *out, const float *angles_host, const xr_volume2d *vol,
+                const xr_geometry2d *geo, int num_batch, int num_view);
+
 /* ------------------------------------------------------- system matrix (host) */
 
 /* COO triplets of the forward projector A: rows index the sinogram
@@ -255,6 +293,25 @@ int xr_system_matrix2d(const xr_volume2d *vol, const xr_geometry2d *geo, const f
 int64_t xr_coo_nnz(const xr_coo *coo);
 int xr_coo_copy(const xr_coo *coo, int64_t *rows, int64_t *cols, float *vals);
 void xr_coo_destroy(xr_coo *coo);
+
+/* --------------------------------------------------- system matrix (GPU emission) */
+
+/* The same triplets as xr_system_matrix3d / xr_system_matrix2d -- same order, same
+ * values bit for bit -- emitted into device memory by `device`.  No reference
+ * counterpart: the reference builds the matrix on the CPU only
+ * (src/api/torch_binding.cpp:380-405 rejects GPU tensors).  Creation builds the per-view
+ * tables on the host (every transcendental of the traversal), runs a counting pass and a
+ * scan on `stream` and synchronises it to learn the number of triplets; emission then
+ * writes them into caller-provided device buffers of xr_sysmat_gpu_nnz() entries on
+ * `stream` without synchronising. */
+typedef struct xr_sysmat_gpu xr_sysmat_gpu;
+int xr_system_matrix3d_gpu(const xr_volume3d *vol, const xr_geometry3d *geo, const float *angles_host,
+                           int num_view, int device, void *stream, xr_sysmat_gpu **out);
+int xr_system_matrix2d_gpu(const xr_volume2d *vol, const xr_geometry2d *geo, const float *angles_host,
+                           int num_view, int device, void *stream, xr_sysmat_gpu **out);
+int64_t xr_sysmat_gpu_nnz(const xr_sysmat_gpu *sm);
+int xr_sysmat_gpu_emit(const xr_sysmat_gpu *sm, int64_t *rows, int64_t *cols, float *vals, void *stream);
+void xr_sysmat_gpu_destroy(xr_sysmat_gpu *sm);
 
 #ifdef __cplusplus
 }

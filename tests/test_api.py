@@ -154,14 +154,19 @@ def test_shape_checks_raise_value_error():
         xrecon_amd.Projector3d((16, 16, 4))
 
 
-def test_cpu_tensors_are_rejected_loudly():
-    """No CPU fallback: CPU inputs raise instead of computing elsewhere."""
+def test_cpu_tensors_take_host_plans_and_devices_never_mix():
+    """CPU inputs run the host executor (as the reference runs its CPU kernels for CPU
+    tensors); a host plan refuses GPU-side use and a device plan refuses CPU tensors."""
+    from xrecon_amd import _backend as be
+
     p = xrecon_amd.Projector3d((16, 16, 4), (20, 4), 1.0, 1.0, 500.0, 1000.0)
-    with pytest.raises(RuntimeError, match="GPU"):
-        p.projection(torch.zeros(1, 1, 4, 16, 16), torch.zeros(5))
+    y = p.projection(torch.ones(1, 1, 4, 16, 16), torch.arange(5, dtype=torch.float32))
+    assert y.shape == (1, 1, 5, 4, 20) and not y.is_cuda and float(y.abs().sum()) > 0
+    plan = be.get_plan(p._host_options, torch.arange(5, dtype=torch.float32), torch.empty(0), False)
+    assert type(plan).__name__ == "HostPlan"
     p2 = xrecon_amd.Projector2d(16, 20, 1.0, 1.0, 500.0, 1000.0)
-    with pytest.raises(RuntimeError, match="GPU"):
-        p2.backprojection(torch.zeros(1, 1, 5, 20), torch.zeros(5))
+    b = p2.backprojection(torch.ones(1, 1, 5, 20), torch.arange(5, dtype=torch.float32))
+    assert b.shape == (1, 1, 16, 16) and float(b.abs().sum()) > 0
 
 
 def test_native_extension_loads_and_binds_reference_names():
@@ -175,5 +180,7 @@ def test_native_extension_loads_and_binds_reference_names():
                  "weighted_backward_projection_transpose2d", "back_projection2d", "weighted_back_projection2d"):
         assert hasattr(C, name), name
     opts = torch.tensor([16, 16, 4, 20, 4, 1, 1, 1, 1, 1, 500, 1000, 0, 0, 0, 0, 0, 1], dtype=torch.float32)
-    with pytest.raises(RuntimeError, match="GPU"):
-        C.forward_projection3d(torch.zeros(1, 1, 4, 16, 16), opts, torch.zeros(3))
+    # a CPU tensor computes on the host; shape errors raise as in the reference
+    assert C.forward_projection3d(torch.zeros(1, 1, 4, 16, 16), opts, torch.zeros(3)).shape == (1, 1, 3, 4, 20)
+    with pytest.raises(RuntimeError, match="shape"):
+        C.forward_projection3d(torch.zeros(1, 1, 4, 16, 15), opts, torch.zeros(3))

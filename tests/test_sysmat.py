@@ -11,7 +11,7 @@ import torch
 import oracle
 import xrecon_amd
 from helpers import opts2d, opts3d, rel_l2
-from xrecon_amd.sysmat import system_matrix2d, system_matrix3d
+from xrecon_amd.sysmat import system_matrix2d, system_matrix3d, triplets
 
 
 def _coalesce(rows, cols, vals, shape):
@@ -33,6 +33,21 @@ def test_golden_3d_and_2d():
     _check(system_matrix2d(list(f["sm2d_options"]), f["sm2d_angles"]), f["sm2d_rows"], f["sm2d_cols"], f["sm2d_vals"])
 
 
+def _same_order(ours, rows, cols, vals):
+    r, c, v = (t.numpy() for t in ours)
+    assert np.array_equal(r, rows) and np.array_equal(c, cols), "triplet order or indices differ"
+    assert np.array_equal(v.view(np.int32), np.asarray(vals, np.float32).view(np.int32)), "values differ"
+
+
+def test_golden_triplets_in_traversal_order():
+    """Before coalescing, the host builder's triplets are the reference's, in the same
+    order (the GPU emission is checked against this order)."""
+    f = np.load(__import__("os").path.join(__import__("os").path.dirname(__file__), "golden", "ref_cpu_fixtures.npz"))
+    _same_order(triplets(list(f["sm3d_options"]), f["sm3d_angles"]), f["sm3d_rows"], f["sm3d_cols"], f["sm3d_vals"])
+    _same_order(triplets(list(f["sm2d_options"]), f["sm2d_angles"], is2d=True), f["sm2d_rows"], f["sm2d_cols"],
+                f["sm2d_vals"])
+
+
 @pytest.mark.skipif(not oracle.ref_available(), reason="reference CPU library not built")
 @pytest.mark.parametrize("scan", [0, 1, 2])
 @pytest.mark.parametrize("seed", [0, 1, 2])
@@ -48,6 +63,7 @@ def test_bit_exact_vs_reference_3d(scan, seed):
     ang[3] = np.float32(np.pi / 4)  # a driving-axis tie
     rows, cols, vals = oracle.ref_sm3d(ang, o)
     _check(system_matrix3d(o, ang), rows, cols, vals)
+    _same_order(triplets(o, ang), rows, cols, vals)
 
 
 @pytest.mark.skipif(not oracle.ref_available(), reason="reference CPU library not built")
@@ -59,6 +75,7 @@ def test_bit_exact_vs_reference_2d(scan):
     ang = (np.arange(14, dtype=np.float32) * np.float32(2 * np.pi / 14) + np.float32(0.05)).astype(np.float32)
     rows, cols, vals = oracle.ref_sm2d(ang, o)
     _check(system_matrix2d(o, ang), rows, cols, vals)
+    _same_order(triplets(o, ang, is2d=True), rows, cols, vals)
 
 
 def test_projector_method_and_matvec():

@@ -1,8 +1,11 @@
 """Loader for the native extension and the plan cache.
 
 The native extension (``xrecon_amd/_C*.so`` over ``_lib/libxrecon_amd.so``) is the only
-compute path: if it is missing or cannot be loaded, every operator raises.  There is no
-Python or CPU fallback.
+compute path: if it is missing or cannot be loaded, every operator raises.  GPU tensors
+run the gfx950 kernels through device plans; CPU tensors run the library's host executor
+(the same per-thread bodies on the host's cores) through host plans, as the reference
+runs its CPU kernels for CPU tensors.  There is no Python compute path and no fallback
+from one device to the other.
 """
 from __future__ import annotations
 
@@ -50,6 +53,9 @@ def native():
 
 
 def current_stream(t: torch.Tensor) -> int:
+    """torch's current HIP stream of t's device (0 for CPU tensors, which need none)."""
+    if not t.is_cuda:
+        return 0
     return torch.cuda.current_stream(t.device).cuda_stream
 
 
@@ -87,7 +93,10 @@ class PlanCache:
         with _lock:
             plan = self._by_val.get(val_key)
         if plan is None:
-            plan = native().Plan(list(opts), ang_host, device, is2d)
+            if device < 0:
+                plan = native().HostPlan(list(opts), ang_host, is2d)
+            else:
+                plan = native().Plan(list(opts), ang_host, device, is2d)
             with _lock:
                 self._insert(self._by_val, val_key, plan)
         with _lock:
@@ -105,12 +114,11 @@ PLANS = PlanCache()
 
 
 def get_plan(options, angles: torch.Tensor, ref: torch.Tensor, is2d: bool):
-    if not ref.is_cuda:
-        raise RuntimeError("xrecon_amd computes on the GPU: input must be a GPU tensor "
-                           "(CPU tensors are not supported by this build)")
+    """The plan for `ref`'s device: a device plan for GPU tensors, a host plan (device
+    key -1) for CPU tensors."""
     if angles.dim() != 1:
         raise ValueError(f"Expected angles shape [V], got {tuple(angles.shape)}")
-    return PLANS.get(options, angles, ref.get_device(), is2d)
+    return PLANS.get(options, angles, ref.get_device() if ref.is_cuda else -1, is2d)
 
 
 def options_tuple(values: Sequence[float]):

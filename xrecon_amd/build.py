@@ -22,7 +22,7 @@ LIBDIR = os.path.join(HERE, "_lib")
 LIB = os.path.join(LIBDIR, "libxrecon_amd.so")
 ARCH = os.environ.get("XRECON_AMD_ARCH", "gfx950")
 
-HIP_SOURCES = ["dd_kernels.hip", "batch.hip", "noalloc.hip", "plan.cpp", "capi.cpp", "sysmat.cpp"]
+HIP_SOURCES = ["dd_kernels.hip", "batch.hip", "noalloc.hip", "sysmat_gpu.hip", "plan.cpp", "capi.cpp", "cpu_exec.cpp", "sysmat.cpp"]
 # Packed-FP32 instructions are disabled for all device code (they have produced silent
 # wrong results on this hardware generation in some forms).  No implicit contraction of
 # a * b + c: every fused multiply-add in the kernels is an explicit fmaf, so results do not

@@ -58,6 +58,17 @@ int validate(const xr_volume3d *vol, const xr_geometry3d *geo, int num_view)
     return XR_OK;
 }
 
+}  // namespace
+
+namespace xr {
+int validate_plan_geometry(const xr_volume3d *vol, const xr_geometry3d *geo, int num_view)
+{
+    return validate(vol, geo, num_view);
+}
+}  // namespace xr
+
+namespace {
+
 struct DeviceGuard {
     int prev = -1;
     explicit DeviceGuard(int dev)

@@ -9,8 +9,9 @@
 //    of 18 device->host .item() reads per call (torch_binding.cpp:85-108);
 //  * work goes to the caller's stream (the reference uses the legacy default stream,
 //    src/kernels/forward3d_flat.cu:506), so calls can be captured into a graph;
-//  * CPU tensors are rejected: this build computes on the GPU only (the reference falls
-//    back to its std::thread CPU kernels, torch_binding.cpp:421-425).
+//  * CPU tensors run the library's host executor (the same per-thread bodies as the GPU
+//    kernels on the host's cores), where the reference runs its std::thread CPU kernels
+//    (torch_binding.cpp:421-425); GPU tensors always run the gfx950 kernels.
 #include <torch/extension.h>
 #include <c10/hip/HIPStream.h>
 
@@ -147,8 +148,7 @@ public:
 
     void check_input(int op, const torch::Tensor &x) const
     {
-        TORCH_CHECK(x.is_cuda(), "xrecon_amd computes on the GPU: input must be a GPU tensor "
-                                 "(CPU tensors are not supported by this build)");
+        TORCH_CHECK(x.is_cuda(), "a device plan computes GPU tensors; CPU tensors take a host plan");
         TORCH_CHECK(x.scalar_type() == torch::kFloat32, "input must be float32");
         TORCH_CHECK(x.is_contiguous(), "Expected all tensors to be contiguous");
         TORCH_CHECK(x.get_device() == device_, "input is on device ", x.get_device(), " but the plan on ", device_);
@@ -231,6 +231,74 @@ private:
     int nview_ = 0;
     int device_ = 0;
     xr_plan3d *plan_ = nullptr;
+};
+
+// A host plan: the same operators on CPU tensors (xr_host_plan3d_*), the counterpart of
+// the reference's CPU dispatch (src/api/torch_binding.cpp:421-425).  Only CPU tensors are
+// accepted; a GPU tensor never reaches it.
+class HostPlan {
+public:
+    HostPlan(const std::vector<float> &options, const torch::Tensor &angles, bool is2d)
+    {
+        desc_ = is2d ? unpack2d(options) : unpack3d(options);
+        TORCH_CHECK(angles.dim() == 1, "Expected angles shape [V], got ", angles.sizes());
+        auto a = angles.detach().to(torch::kCPU, torch::kFloat32).contiguous();
+        nview_ = (int)a.numel();
+        const int rc = xr_host_plan3d_create(&desc_.vol, &desc_.geo, a.data_ptr<float>(), nview_, &plan_);
+        if (rc) raise_xr(rc, "xr_host_plan3d_create");
+    }
+    ~HostPlan()
+    {
+        if (plan_) xr_host_plan3d_destroy(plan_);
+    }
+    HostPlan(const HostPlan &) = delete;
+    HostPlan &operator=(const HostPlan &) = delete;
+
+    int num_view() const { return nview_; }
+    bool is2d() const { return desc_.is2d; }
+
+    std::vector<int64_t> volume_shape(int64_t B, int64_t C) const
+    {
+        if (desc_.is2d) return {B, C, desc_.H, desc_.W};
+        return {B, C, desc_.D, desc_.H, desc_.W};
+    }
+    std::vector<int64_t> sino_shape(int64_t B, int64_t C) const
+    {
+        if (desc_.is2d) return {B, C, nview_, desc_.Nu};
+        return {B, C, nview_, desc_.Nv, desc_.Nu};
+    }
+
+    // stream: accepted for the same call signature as Plan::execute, unused on the host.
+    torch::Tensor execute(int op, const torch::Tensor &x, int64_t /*stream*/, int64_t num_threads) const
+    {
+        TORCH_CHECK(!x.is_cuda(), "a host plan computes CPU tensors; GPU tensors take a device plan");
+        TORCH_CHECK(x.scalar_type() == torch::kFloat32, "input must be float32");
+        TORCH_CHECK(x.is_contiguous(), "Expected all tensors to be contiguous");
+        const int nd = desc_.is2d ? 4 : 5;
+        TORCH_CHECK(x.dim() == nd, desc_.is2d ? (volume_in(op) ? "Expected image shape [B, C, H, W]"
+                                                               : "Expected projection shape [B, C, V, Nu]")
+                                              : (volume_in(op) ? "Expected image shape [B, C, D, H, W]"
+                                                               : "Expected projection shape [B, C, V, Nv, Nu]"));
+        const int64_t B = x.size(0), C = x.size(1);
+        const auto want = volume_in(op) ? volume_shape(B, C) : sino_shape(B, C);
+        TORCH_CHECK(x.sizes().vec() == want, volume_in(op) ? "Image shape does not match geometry: "
+                                                           : "Projection shape does not match geometry: ",
+                    x.sizes(), " vs expected ", want);
+        auto out = torch::empty(volume_in(op) ? sino_shape(B, C) : volume_shape(B, C), x.options());
+        int rc;
+        {
+            py::gil_scoped_release nogil;
+            rc = xr_host_plan3d_execute(plan_, op, x.data_ptr<float>(), out.data_ptr<float>(), (int)(B * C),
+                                        (int)num_threads);
+        }
+        if (rc) raise_xr(rc, "xr_host_plan3d_execute");
+        return out;
+    }
+
+private:
+    Desc desc_;
+    int nview_ = 0;
+    xr_host_plan3d *plan_ = nullptr;
 };
 
 // A batch of plans executed by single launches (xr_batch3d_*).
@@ -387,6 +455,45 @@ private:
     Index<std::shared_ptr<Plan>> by_val_;
 };
 
+// Host plans of the reference-named functions on CPU tensors, keyed by content (the
+// options and angles of a CPU call live in host memory: reading them costs nothing).
+class HostPlanCache {
+public:
+    std::shared_ptr<HostPlan> get(const torch::Tensor &options, const torch::Tensor &angles, bool is2d)
+    {
+        const auto o = host_floats(options, is2d ? 12 : 18, "Option");
+        auto a = angles.detach().to(torch::kCPU, torch::kFloat32).contiguous();
+        std::string key(reinterpret_cast<const char *>(o.data()), o.size() * sizeof(float));
+        key.append(reinterpret_cast<const char *>(a.data_ptr<float>()), (size_t)a.numel() * sizeof(float));
+        key.push_back(is2d ? '2' : '3');
+        std::lock_guard<std::mutex> lk(mu_);
+        auto it = index_.find(key);
+        if (it != index_.end()) {
+            lru_.splice(lru_.begin(), lru_, it->second);
+            return it->second->second;
+        }
+        auto plan = std::make_shared<HostPlan>(o, a, is2d);
+        lru_.emplace_front(key, plan);
+        index_[key] = lru_.begin();
+        while (lru_.size() > 16) {
+            index_.erase(lru_.back().first);
+            lru_.pop_back();
+        }
+        return plan;
+    }
+
+private:
+    std::mutex mu_;
+    std::list<std::pair<std::string, std::shared_ptr<HostPlan>>> lru_;
+    std::unordered_map<std::string, std::list<std::pair<std::string, std::shared_ptr<HostPlan>>>::iterator> index_;
+};
+
+HostPlanCache &host_plan_cache()
+{
+    static HostPlanCache *cache = new HostPlanCache();
+    return *cache;
+}
+
 PlanCache &plan_cache()
 {
     static PlanCache *cache = new PlanCache();  // never destroyed: plans outlive interpreter teardown order
@@ -408,8 +515,15 @@ int64_t resolve_stream(int64_t stream, int device)
 torch::Tensor run_op(int op, bool is2d, const torch::Tensor &x, const torch::Tensor &options,
                      const torch::Tensor &angles, int64_t stream)
 {
-    TORCH_CHECK(x.is_cuda(), "xrecon_amd computes on the GPU: input must be a GPU tensor "
-                             "(CPU tensors are not supported by this build)");
+    if (!x.is_cuda()) {
+        // CPU tensors: the host executor, as the reference runs its CPU kernels
+        TORCH_CHECK(!options.is_cuda() && !angles.is_cuda(), "Expected all tensors on the same device");
+        TORCH_CHECK(options.scalar_type() == torch::kFloat32, "options must be float32");
+        TORCH_CHECK(angles.scalar_type() == torch::kFloat32, "ang must be float32");
+        TORCH_CHECK(angles.is_contiguous() && options.is_contiguous(), "Expected all tensors to be contiguous");
+        TORCH_CHECK(angles.dim() == 1, "Expected angles shape [V], got ", angles.sizes());
+        return host_plan_cache().get(options, angles, is2d)->execute(op, x, 0, 0);
+    }
     TORCH_CHECK(!angles.is_cuda() || angles.get_device() == x.get_device(),
                 "Expected all tensors on the same device");
     TORCH_CHECK(!options.is_cuda() || options.get_device() == x.get_device(),
@@ -459,6 +573,70 @@ torch::Tensor system_matrix(bool is2d, const torch::Tensor &options, const torch
     return torch::sparse_coo_tensor(torch::stack({rows, cols}), vals, {nrows, ncols}).coalesce();
 }
 
+// The same triplets emitted on a GPU (xr_system_matrix*_gpu): rows, cols (int64) and
+// values (float32) as device tensors in the host builder's order, before coalescing.
+std::vector<torch::Tensor> system_matrix_triplets_device(bool is2d, const torch::Tensor &options,
+                                                         const torch::Tensor &ang, int64_t device)
+{
+    TORCH_CHECK(options.scalar_type() == torch::kFloat32, "options must be float32");
+    TORCH_CHECK(ang.scalar_type() == torch::kFloat32, "ang must be float32");
+    const auto o = host_floats(options, is2d ? 12 : 18, "Option");
+    auto a = ang.detach().to(torch::kCPU, torch::kFloat32).contiguous();
+    const int V = (int)a.numel();
+    const int dev = (int)device;
+    const int64_t stream = resolve_stream(-1, dev);
+    xr_sysmat_gpu *m = nullptr;
+    int rc;
+    if (is2d) {
+        xr_volume2d v2{(int)o[0], (int)o[1], o[3], o[4], o[8], o[9]};
+        xr_geometry2d g2{(int)o[11], (int)o[2], o[5], o[10], o[6], o[7]};
+        rc = xr_system_matrix2d_gpu(&v2, &g2, a.data_ptr<float>(), V, dev, reinterpret_cast<void *>(stream), &m);
+    } else {
+        const Desc d = unpack3d(o);
+        rc = xr_system_matrix3d_gpu(&d.vol, &d.geo, a.data_ptr<float>(), V, dev, reinterpret_cast<void *>(stream), &m);
+    }
+    if (rc) raise_xr(rc, is2d ? "xr_system_matrix2d_gpu" : "xr_system_matrix3d_gpu");
+    std::unique_ptr<xr_sysmat_gpu, void (*)(xr_sysmat_gpu *)> guard(m, xr_sysmat_gpu_destroy);
+    const int64_t n = xr_sysmat_gpu_nnz(m);
+    const auto dopt = torch::TensorOptions().device(torch::kCUDA, dev);
+    auto rows = torch::empty({n}, dopt.dtype(torch::kInt64)), cols = torch::empty({n}, dopt.dtype(torch::kInt64));
+    auto vals = torch::empty({n}, dopt.dtype(torch::kFloat32));
+    rc = xr_sysmat_gpu_emit(m, rows.data_ptr<int64_t>(), cols.data_ptr<int64_t>(), vals.data_ptr<float>(),
+                            reinterpret_cast<void *>(stream));
+    if (rc) raise_xr(rc, "xr_sysmat_gpu_emit");
+    // the tables and counts the emission reads are freed by `guard`: wait for it first
+    TORCH_CHECK(hipStreamSynchronize(reinterpret_cast<hipStream_t>(stream)) == hipSuccess,
+                "stream synchronisation after the system-matrix emission failed");
+    return {rows, cols, vals};
+}
+
+// Host builder triplets, uncoalesced, in traversal order (tests compare them with the
+// GPU emission).
+std::vector<torch::Tensor> system_matrix_triplets_host(bool is2d, const torch::Tensor &options,
+                                                       const torch::Tensor &ang)
+{
+    const auto o = host_floats(options, is2d ? 12 : 18, "Option");
+    auto a = ang.detach().to(torch::kCPU, torch::kFloat32).contiguous();
+    const int V = (int)a.numel();
+    xr_coo *coo = nullptr;
+    int rc;
+    if (is2d) {
+        xr_volume2d v2{(int)o[0], (int)o[1], o[3], o[4], o[8], o[9]};
+        xr_geometry2d g2{(int)o[11], (int)o[2], o[5], o[10], o[6], o[7]};
+        rc = xr_system_matrix2d(&v2, &g2, a.data_ptr<float>(), V, &coo);
+    } else {
+        const Desc d = unpack3d(o);
+        rc = xr_system_matrix3d(&d.vol, &d.geo, a.data_ptr<float>(), V, &coo);
+    }
+    if (rc) raise_xr(rc, is2d ? "xr_system_matrix2d" : "xr_system_matrix3d");
+    const int64_t n = xr_coo_nnz(coo);
+    auto rows = torch::empty({n}, torch::kInt64), cols = torch::empty({n}, torch::kInt64);
+    auto vals = torch::empty({n}, torch::kFloat32);
+    xr_coo_copy(coo, rows.data_ptr<int64_t>(), cols.data_ptr<int64_t>(), vals.data_ptr<float>());
+    xr_coo_destroy(coo);
+    return {rows, cols, vals};
+}
+
 #define XR_DEF_OP(name, op, is2d)                                                                         \
     m.def(                                                                                                   \
         name,                                                                                                \
@@ -492,6 +670,16 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m)
              py::arg("workspace"), py::arg("stream") = 0, py::arg("tile_rows") = 0, py::arg("voxel_chunk") = 0,
              py::arg("phase") = XR_PHASE_ALL);
 
+    py::class_<HostPlan, std::shared_ptr<HostPlan>>(m, "HostPlan", py::dynamic_attr())
+        .def(py::init([](const std::vector<float> &options, const torch::Tensor &angles, bool is2d) {
+                 return std::make_shared<HostPlan>(options, angles, is2d);
+             }),
+             py::arg("options"), py::arg("angles"), py::arg("is2d") = false)
+        .def_property_readonly("num_view", &HostPlan::num_view)
+        .def_property_readonly("is2d", &HostPlan::is2d)
+        .def("execute", &HostPlan::execute, py::arg("op"), py::arg("input"), py::arg("stream") = 0,
+             py::arg("num_threads") = 0);
+
     py::class_<Batch, std::shared_ptr<Batch>>(m, "Batch")
         .def(py::init([](const std::vector<std::shared_ptr<Plan>> &plans, const std::vector<int> &num_batch) {
                  return std::make_shared<Batch>(plans, num_batch);
@@ -518,6 +706,13 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m)
           "2D distance-driven projection matrix");
     m.def("system_matrix3d", [](const torch::Tensor &o, const torch::Tensor &a) { return system_matrix(false, o, a); },
           "3D distance-driven projection matrix");
+    m.def("system_matrix_triplets",
+          [](const torch::Tensor &o, const torch::Tensor &a, bool is2d, int64_t device) {
+              return device < 0 ? system_matrix_triplets_host(is2d, o, a)
+                                : system_matrix_triplets_device(is2d, o, a, device);
+          },
+          py::arg("options"), py::arg("angles"), py::arg("is2d"), py::arg("device") = -1,
+          "system-matrix triplets (rows, cols, values) before coalescing: host builder (device < 0) or GPU emission");
     XR_DEF_OP("forward_projection2d", XR_OP_PROJECTION, true);
     XR_DEF_OP("backward_projection2d", XR_OP_BACKPROJECTION, true);
     XR_DEF_OP("weighted_backward_projection2d", XR_OP_BACKPROJECTION_W, true);

@@ -42,9 +42,11 @@ static_assert(sizeof(ViewHdr) == 80, "ViewHdr layout");
 
 // Per (view, 16 x 16 tile of voxel columns): the detector columns c0 .. c0+nc-1 the tile's
 // footprint reaches on that view (with a one-column margin), and the range of the depth
-// scale zeta * kappa over the tile's lines and those columns, from which a voxel-gather
-// workgroup derives the rows its slice chunk reads.  nc = 0: the tile misses the
-// detector.  An estimate that decides what is staged in LDS, never a result.
+// scale zeta * kappa over the tile's lines and all of those columns, from which a
+// voxel-gather workgroup derives the rows its slice chunk reads.  Both ranges bound every
+// voxel of the tile, so a workgroup whose chunk's rows lie off the detector passes over
+// the view; what the workgroup stages in LDS from them decides speed only.  nc = -1: the
+// tile misses the detector; nc = 0: no range (nothing staged, nothing skipped).
 struct VoxTile {
     int c0, nc;
     float zk_lo, zk_hi;
@@ -107,6 +109,9 @@ struct PlanTables {
     std::vector<int> citems;  // as Plan3d::d_citems
     int ncls[3] = {0, 0, 0};
 };
+
+// capi.cpp: the geometry checks of xr_plan3d_create (sizes, spacings, index limits).
+int validate_plan_geometry(const xr_volume3d *vol, const xr_geometry3d *geo, int num_view);
 
 // plan.cpp
 int build_tables(Plan3d &p, const float *angles_host, PlanTables &t, std::string &err);

@@ -155,11 +155,13 @@ class Projector3d(nn.Module):
     backprojection_transpose = backprojection_t
     weighted_backprojection_transpose = backprojection_weighted_t
 
-    def system_matrix(self, angles: Tensor) -> Tensor:
+    def system_matrix(self, angles: Tensor, device=None) -> Tensor:
+        """Sparse COO matrix of the projection; built on the host (as the reference), or
+        emitted on ``device`` (a GPU) with the same triplets."""
         from .sysmat import system_matrix3d  # noqa: PLC0415
 
         angles = torch.as_tensor(angles, dtype=torch.float32, device="cpu").contiguous()
-        return system_matrix3d(self._host_options, angles)
+        return system_matrix3d(self._host_options, angles, device)
 
     get_sys_matrix = system_matrix
 
