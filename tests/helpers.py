@@ -109,7 +109,7 @@ def _p(a):
 def emu_op3d(op, x, angles, opts, R=0, KC=0, is2d=False, nthreads=None, cell_variant=0, voxel_variant=0):
     """The product's kernel bodies run on the CPU (float32 in/out).  cell_variant: 0 = the
     cell-gather variant the library launches, 1 = general path with 32-bit offsets,
-    2 = general path with 64-bit offsets, 3 = the wave-staged fixed-span variant.  voxel_variant: 0 = the workgroup path with the
+    2 = general path with 64-bit offsets.  voxel_variant: 0 = the workgroup path with the
     per-view block staged (as launched), 1 = every lookup from the table."""
     lib = _load_emu()
     lib.emu_set_cell_variant(int(cell_variant))

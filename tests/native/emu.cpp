@@ -20,49 +20,15 @@ using namespace xr;
 namespace {
 
 // Cell-gather variant (see run_cell_RV): 0 = as launched, 1 = general 32-bit, 2 = general
-// 64-bit, 3 = the staged wave variant for fixed-span groups.
+// 64-bit.
 int g_cell_variant = 0;
 
 // Number of (workgroup, view) pairs the voxel gather passed over (VoxBox::skip).
 std::atomic<long long> g_voxel_skips{0};
 
-// One wave of the staged fixed-span cell gather, lanes in lockstep per line: the block
-// estimate, every lane's staging share, then every lane's gather -- the order the wave's
-// own wait imposes on the device.
-template <int OPK, int GEO, int R, int SPAN>
-void emu_cell_wave(const Plan3d &p, const CellArgs &a, const float2 *WX, const float2 *WY, float *out, int cg, int r0,
-                   int v, int b)
-{
-    const ViewHdr &h = p.d_hdr[v];
-    const float4 *ct = p.d_col + (int64_t)v * (a.nu + 1);
-    std::vector<CellLane<R>> st(kWave);
-    std::vector<char> ok(kWave);
-    for (int lane = 0; lane < kWave; ++lane) {
-        const int c = cg * kWave + lane;
-        ok[lane] = c < a.nu;
-        cell_init<R>(st[lane], ct, a, ok[lane] ? c : cg * kWave, r0);
-    }
-    const float2 *W = cell_table(WX, WY, h, a, b);
-    const float4 *lt = p.d_lineC + (int64_t)v * a.maxline;
-    const float dfirst = ct[cg * kWave].x, dlast = ct[imin(a.nu, cg * kWave + kWave)].x;
-    const CellGroup grp = p.d_cgrp[(int64_t)v * ceil_div(a.nu, kWave) + cg];
-    std::vector<float2> lds(cell_lds_cap<R>());
-    for (int L = 0; L < h.nline; ++L) {
-        const float4 lc = lt[L];
-        const CellBox bx = cell_box<R, SPAN>(a, lc, dfirst, dlast, grp.sig_lo, grp.sig_hi, r0, h.nalong);
-        if (bx.pitch == 0) continue;
-        for (int lane = 0; lane < kWave; ++lane) cell_stage(W + (int64_t)L * h.nalong, bx, lds.data(), a.plane8, lane);
-        for (int lane = 0; lane < kWave; ++lane)
-            if (ok[lane]) cell_line<OPK, GEO, R, SPAN, false>(st[lane], h, a, lc, L, W, bx, lds.data());
-    }
-    for (int lane = 0; lane < kWave; ++lane)
-        if (ok[lane]) cell_store<OPK, GEO, R>(st[lane], h, a, out, cg * kWave + lane, r0, v, b);
-}
-
 // MODE 0: what the library launches (per column group: the group's class, each line of
 // a wider group on the narrowest path it allows; big plans 64-bit general); 1: general
-// loop, 32-bit offsets, on every line; 2: general loop, 64-bit offsets; 3: the staged
-// wave variant (XR_CELL_STAGED) for fixed-span groups.
+// loop, 32-bit offsets, on every line; 2: general loop, 64-bit offsets.
 template <int OPK, int GEO, int R, int MODE>
 void run_cell_RV(const Plan3d &p, const float2 *WX, const float2 *WY, float *out, int nb, int nthreads)
 {
@@ -79,15 +45,7 @@ void run_cell_RV(const Plan3d &p, const float2 *WX, const float2 *WY, float *out
                 for (int g = 0; g < ngroups; ++g)
                     for (int cg = 0; cg < ncg; ++cg) {
                         const CellGroup grp = p.d_cgrp[(int64_t)v * ncg + cg];
-                        const int span = (MODE == 0 || MODE == 3) && !p.big ? grp.span : 0;
-                        if (MODE == 3 && span == 2) {
-                            emu_cell_wave<OPK, GEO, R, 2>(p, a, WX, WY, out, cg, g * R, v, b);
-                            continue;
-                        }
-                        if (MODE == 3 && span == 3) {
-                            emu_cell_wave<OPK, GEO, R, 3>(p, a, WX, WY, out, cg, g * R, v, b);
-                            continue;
-                        }
+                        const int span = MODE == 0 && !p.big ? grp.span : 0;
                         const float wg = MODE == 0 && !p.big ? grp.wmax : inf;
                         for (int c = cg * kWave; c < std::min(a.nu, (cg + 1) * kWave); ++c) {
                             if (span == 2)
@@ -115,7 +73,6 @@ void run_cell_R(const Plan3d &p, const float2 *WX, const float2 *WY, float *out,
 {
     if (g_cell_variant == 1) run_cell_RV<OPK, GEO, R, 1>(p, WX, WY, out, nb, nt);
     else if (g_cell_variant == 2) run_cell_RV<OPK, GEO, R, 2>(p, WX, WY, out, nb, nt);
-    else if (g_cell_variant == 3) run_cell_RV<OPK, GEO, R, 3>(p, WX, WY, out, nb, nt);
     else run_cell_RV<OPK, GEO, R, 0>(p, WX, WY, out, nb, nt);
 }
 
@@ -135,7 +92,7 @@ int g_voxel_variant = 0;
 
 // Workgroup emulation of the staged voxel gather: per view, the block estimate, all
 // threads' staging shares, then every thread's gather -- the order the barriers impose.
-template <int OPK, int GEO, int KC>
+template <int OPK, int GEO, int KC, bool HALVES>
 void run_voxel_wg_KC(const Plan3d &p, const float2 *Q, float *out, int nb, int nthreads)
 {
     const VoxArgs a = make_vox_args(p, nb, KC);
@@ -164,8 +121,8 @@ void run_voxel_wg_KC(const Plan3d &p, const float2 *Q, float *out, int nb, int n
                         int i, j;
                         voxel_coords(bx, by, tid, i, j);
                         if (i >= a.nx || j >= a.ny) continue;
-                        voxel_view<OPK, GEO, KC>(h, p.d_lineV, ct, Qv, a, v, i, j, k0, voxel_px(a, i), voxel_py(a, j),
-                                                 &acc[(size_t)tid * KC], box, tile.data());
+                        voxel_view<OPK, GEO, KC, HALVES>(h, p.d_lineV, ct, Qv, a, v, i, j, k0, voxel_px(a, i),
+                                                         voxel_py(a, j), &acc[(size_t)tid * KC], box, tile.data());
                     }
                 }
                 for (int tid = 0; tid < kBlock; ++tid) {
@@ -179,11 +136,11 @@ void run_voxel_wg_KC(const Plan3d &p, const float2 *Q, float *out, int nb, int n
     for (auto &x : th) x.join();
 }
 
-template <int OPK, int GEO, int KC>
+template <int OPK, int GEO, int KC, bool HALVES>
 void run_voxel_KC(const Plan3d &p, const float2 *Q, float *out, int nb, int nthreads)
 {
     if (g_voxel_variant == 0 && KC >= kVoxStageMin) {
-        run_voxel_wg_KC<OPK, GEO, KC>(p, Q, out, nb, nthreads);
+        run_voxel_wg_KC<OPK, GEO, KC, HALVES>(p, Q, out, nb, nthreads);
         return;
     }
     const VoxArgs a = make_vox_args(p, nb, KC);
@@ -195,7 +152,8 @@ void run_voxel_KC(const Plan3d &p, const float2 *Q, float *out, int nb, int nthr
                 const int j = it % a.ny, rest = it / a.ny;
                 const int chunk = rest % a.nchunk, b = rest / a.nchunk;
                 for (int i = 0; i < a.nx; ++i)
-                    voxel_body<OPK, GEO, KC>(p.d_hdr, p.d_lineV, p.d_col, Q, out, a, i, j, chunk * KC, b, 0, a.nview);
+                    voxel_body<OPK, GEO, KC, HALVES>(p.d_hdr, p.d_lineV, p.d_col, Q, out, a, i, j, chunk * KC, b, 0,
+                                                     a.nview);
             }
         });
     }
@@ -205,10 +163,11 @@ void run_voxel_KC(const Plan3d &p, const float2 *Q, float *out, int nb, int nthr
 template <int OPK, int GEO>
 void run_voxel(const Plan3d &p, int KC, const float2 *Q, float *out, int nb, int nt)
 {
-    if (KC == 1) run_voxel_KC<OPK, GEO, 1>(p, Q, out, nb, nt);
-    else if (KC == 4) run_voxel_KC<OPK, GEO, 4>(p, Q, out, nb, nt);
-    else if (KC == 8) run_voxel_KC<OPK, GEO, 8>(p, Q, out, nb, nt);
-    else run_voxel_KC<OPK, GEO, 16>(p, Q, out, nb, nt);
+    if (KC == 1) run_voxel_KC<OPK, GEO, 1, false>(p, Q, out, nb, nt);
+    else if (KC == 4) run_voxel_KC<OPK, GEO, 4, false>(p, Q, out, nb, nt);
+    else if (KC == 8) run_voxel_KC<OPK, GEO, 8, false>(p, Q, out, nb, nt);
+    else if (KC == 16) run_voxel_KC<OPK, GEO, 16, false>(p, Q, out, nb, nt);
+    else run_voxel_KC<OPK, GEO, 16, true>(p, Q, out, nb, nt);
 }
 
 template <int GEO>
@@ -324,7 +283,7 @@ int emu_fdk3d(const int *vn, const float *vf, const int *gn, const float *gf, co
     const int rc = make_plan(p, t, vn, vf, gn, gf, angles, nview);
     if (rc) return rc;
     int KC = voxel_chunk;
-    if (KC == 0) KC = p.vol.nz >= 64 ? 16 : p.vol.nz >= 8 ? 8 : p.vol.nz >= 4 ? 4 : 1;
+    if (KC == 0) KC = default_chunk(p);
     if (nthreads < 1) nthreads = 1;
     if (p.kind == GEO_FLAT) fdk_dispatch<GEO_FLAT>(p, KC, in, filt, wgt, out, nbatch, nthreads);
     else if (p.kind == GEO_ARC) fdk_dispatch<GEO_ARC>(p, KC, in, filt, wgt, out, nbatch, nthreads);
@@ -402,7 +361,7 @@ int emu_op3d(int op, const int *vn, const float *vf, const int *gn, const float 
     }
     int R = tile_rows, KC = voxel_chunk;
     if (R == 0) R = p.geo.nv >= 32 ? 8 : p.geo.nv >= 16 ? 4 : p.geo.nv >= 4 ? 2 : 1;
-    if (KC == 0) KC = p.vol.nz >= 64 ? 16 : p.vol.nz >= 8 ? 8 : p.vol.nz >= 4 ? 4 : 1;
+    if (KC == 0) KC = default_chunk(p);
     if (nthreads < 1) nthreads = 1;
     if (p.kind == GEO_FLAT) dispatch<GEO_FLAT>(p, rt, R, KC, in, out, nbatch, nthreads);
     else if (p.kind == GEO_ARC) dispatch<GEO_ARC>(p, rt, R, KC, in, out, nbatch, nthreads);

@@ -125,18 +125,45 @@ def test_parker_weights_properties_and_errors():
     w = p.parker_weight(short)
     assert w.shape == (1, 1, 200, 1, 40)
     assert float(w.min()) >= 0 and float(w.max()) <= 1
-    # Parker's complementarity: the two rays measuring the same line sum to one.  For the
-    # central column (gamma ~ 0) the conjugate of beta is beta + pi.
-    c = 20
-    beta = short.numpy()
-    for bi in (3, 10, 40):
-        b2 = beta[bi] + math.pi - 2 * float(p._fan_angles()[c])
-        j = int(np.argmin(np.abs(beta - b2)))
-        if abs(beta[j] - b2) < 1e-3:
-            assert abs(float(w[0, 0, bi, 0, c] + w[0, 0, j, 0, c]) - 1.0) < 2e-2
     with pytest.raises(ValueError):
         p.parker_weight(torch.linspace(0, 2.0, 50))
     assert p._mean_angle_step(full).item() == pytest.approx(2 * math.pi / 360, rel=1e-5)
+
+
+@pytest.mark.parametrize("scan_type", ["flat", "arc"])
+def test_parker_weights_sum_to_one_over_conjugate_rays(scan_type):
+    """Parker's normalisation (Parker 1982; Silver 2000 for overscan): every line measured
+    twice in a short scan of range pi + 2 delta gets weights summing to one.  The ray of
+    column c at source angle beta and the ray of the mirrored column at beta + pi + 2 gamma_c
+    are the same line -- checked first with the projector's own forward projection of a
+    smooth blob, where the opposite sign is visibly another line -- and their weights are
+    then checked to sum to one for every column, at the minimal range and with overscan."""
+    p = xrecon_amd.Projector3d((64, 64, 1), (96, 1), 1.0, 1.0, 300.0, 600.0, scan_type=scan_type)
+    Nu = 96
+    gam = p._fan_angles(dtype=torch.float64).numpy()
+    y, x = np.meshgrid(np.arange(64) - 31.5, np.arange(64) - 31.5, indexing="ij")
+    blob = torch.from_numpy(np.exp(-((x - 8) ** 2 + (y + 5) ** 2) / 72.0).astype(np.float32)).view(1, 1, 1, 64, 64)
+    for c in (10, 30, 70):
+        b = 0.4
+        ang = torch.tensor([b, b + math.pi + 2 * gam[c], b + math.pi - 2 * gam[c]], dtype=torch.float32)
+        s = p.projection(blob, ang)[0, 0, :, 0, :]
+        assert abs(float(s[1, Nu - 1 - c] / s[0, c]) - 1) < 2e-3
+        assert abs(float(s[2, Nu - 1 - c] / s[0, c]) - 1) > 1e-2
+    gmax = float(np.abs(gam).max())
+    for delta in (gmax, gmax + 0.15):
+        rng = math.pi + 2 * delta
+        pairs = []
+        live = delta - gam > 1e-9  # the end column of a minimal scan has no ramp (its rays are the scan's ends)
+        for c in np.flatnonzero(live)[::5]:
+            for f in (0.1, 0.35, 0.6, 0.9):
+                b = f * 2 * (delta - gam[c])
+                pairs.append((b, b + math.pi + 2 * gam[c], c))
+        ang = sorted({0.0, rng, *(b for b, _, _ in pairs), *(b2 for _, b2, _ in pairs)})
+        w = p.parker_weight(torch.tensor(ang, dtype=torch.float64))[0, 0, :, 0, :].numpy()
+        pos = {a: k for k, a in enumerate(ang)}
+        sums = [w[pos[b], c] + w[pos[b2], Nu - 1 - c] for b, b2, c in pairs]
+        assert np.allclose(sums, 1.0, rtol=0, atol=1e-12)
+        assert np.all(w[0][live] == 0) and np.all(np.abs(w[-1][live[::-1]]) < 1e-24)
 
 
 def test_shape_checks_raise_value_error():

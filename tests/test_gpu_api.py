@@ -105,6 +105,31 @@ def test_fbp2d_config1_vs_fp64(cuda):
     assert rel_l2(rec.cpu().numpy()[0, 0], want[0]) <= 1e-5
 
 
+@pytest.mark.parametrize("scan", ["arc", "parallel"])
+def test_2d_config1_size_fan_arc_and_parallel(cuda, scan):
+    """Config 1's image and sampling (256^2 at 1 mm, 256 detectors, 720 views) on the other
+    2D geometries: A and B_W at full size against the FP64 oracle."""
+    from xrecon_amd.phantom import random_ellipses
+
+    if scan == "arc":  # the 2 mm detector on an arc of radius SDD: 2 mm / 1000 mm per column
+        proj = xrecon_amd.Projector2d(256, 256, 1.0, 2.0, 500.0, 1000.0, scan_type="arc").to(cuda)
+    else:
+        proj = xrecon_amd.Projector2d(256, 256, 1.0, 1.0, 0.0, 0.0, scan_type="parallel").to(cuda)
+    V = 720
+    ang = uniform_angles(V)
+    opts = list(proj._host_options)
+    img = random_ellipses((256, 256), (1.0, 1.0), count=10, semi_axes=(5.0, 60.0), seed=1001)
+    a = torch.from_numpy(ang).to(cuda)
+    print(f"[gpu] 2D {scan} config-1 size A, B_W", flush=True)
+    sino = proj.projection(torch.from_numpy(img).to(cuda).view(1, 1, 256, 256), a)
+    want = oracle.op2d("A", img[None].astype(np.float64), ang, opts)
+    assert rel_l2(sino.cpu().numpy()[0, 0], want[0]) <= 1e-5
+    y = np.random.default_rng(31).random((1, V, 256)).astype(np.float32)
+    got = proj.backprojection_weighted(torch.from_numpy(y).to(cuda).unsqueeze(1), a)
+    want = oracle.op2d("BW", y.astype(np.float64), ang, opts)
+    assert rel_l2(got.cpu().numpy()[0, 0], want[0]) <= 1e-5
+
+
 CFG5 = [512, 512, 256, 736, 128, 0.7, 0.7, 0.7, np.float32(1.2 / 1085.6), 1.0, 595.0, 1085.6, 0, 0, 0, 0, 0, 2]
 
 

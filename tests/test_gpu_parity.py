@@ -11,8 +11,8 @@ import pytest
 import torch
 
 import oracle
-from helpers import CASES3D, OPS, VOLUME_IN, angles_with_ties, emu_available, emu_op3d, rel_l2, shapes3d, \
-    uniform_angles
+from helpers import CASES3D, OP_ID, OPS, VOLUME_IN, angles_with_ties, emu_available, emu_op3d, opts3d, rel_l2, \
+    shapes3d, uniform_angles
 
 pytestmark = pytest.mark.gpu
 
@@ -289,3 +289,36 @@ def test_batch_independence(cuda):
     y4 = plan.execute(0, x4, s)
     assert y4.shape == (2, 2, V, int(opts[4]), int(opts[3]))
     assert torch.equal(y4[1, 0], plan.execute(0, x4[1:2, 0:1].contiguous(), s)[0, 0])
+
+
+@pytest.mark.parametrize("case", ["plane", "table"])
+def test_big_plans_vs_oracle(cuda, case):
+    """Plans whose z-prefix addressing needs 64 bits ("big": a plane of Nx Ny >= 2^21
+    voxels, or a table of one batch item of 4 GiB and more) run the cell gathers'
+    general loop with 64-bit offsets; A, B^T and B_W^T against the oracle, and the voxel
+    gathers of the same plans.  (The CPU emulator checks the same path first,
+    tests/test_emulator.py.)"""
+    if case == "plane":
+        opts = opts3d(1456, 1456, 2, 64, 4, 0.2, 0.2, 1.0, 1.0, 1.0, 500.0, 1000.0, 1, 0.3, -0.2, 0.0, 0.3, 0.1)
+    else:  # (Nz + 1) * Nx * Ny * 8 = 4.6 GB
+        opts = opts3d(800, 800, 900, 48, 24, 0.3, 0.3, 0.3, 1.0, 1.0, 500.0, 1000.0, 1, 0.0, 0.0, 0.0, 0.0, 0.0)
+    ang = np.array([0.1, 0.9, 2.3], dtype=np.float32) if case == "plane" else np.array([0.3, 1.2], dtype=np.float32)
+    ops = ("A", "BT", "BWT", "B", "AT") if case == "plane" else ("A", "B")
+    from xrecon_amd import _backend as be
+
+    plan = be.native().Plan([float(v) for v in opts], torch.from_numpy(ang), 0, False)
+    s = torch.cuda.current_stream().cuda_stream
+    vol, sino = shapes3d(opts, 1, len(ang))
+    rng = np.random.default_rng(13)
+    x = rng.random(vol, dtype=np.float32)
+    y = rng.random(sino).astype(np.float32)
+    xd = torch.from_numpy(x).to(cuda).unsqueeze(1)
+    yd = torch.from_numpy(y).to(cuda).unsqueeze(1)
+    for op in ops:
+        print(f"[gpu] big plan ({case}) {op}", flush=True)
+        inp = (xd, x) if op in ("A", "BT", "BWT") else (yd, y)
+        got = plan.execute(OP_ID[op], inp[0], s).cpu().numpy()[:, 0]
+        want = oracle.op3d(op, inp[1].astype(np.float64), ang, opts)
+        err = rel_l2(got, want)
+        print(f"[gpu] big plan ({case}) {op}: rel L2 {err:.2e}", flush=True)
+        assert err <= 1e-5, op

@@ -5,8 +5,10 @@ Two knobs exist, one per kernel family (``Plan.kernel_kind(op)``):
 * cell gathers (A, B^T, B_W^T): ``tile_rows`` in {1, 2, 4, 8, 16} -- detector rows per
   thread.  More rows amortise each line's column overlaps and share row boundaries,
   but cost registers and therefore resident waves.
-* voxel gathers (B, B_W, A^T): ``voxel_chunk`` in {1, 4, 8, 16} -- slices per thread.
-  Longer chunks amortise each view's column search, shorter ones give more waves.
+* voxel gathers (B, B_W, A^T): ``voxel_chunk`` in {1, 4, 8, 16} -- slices per thread --
+  and 16 | VOXEL_HALVES, 16 slices run as two halves that a wave skips when none of its
+  lanes reaches the detector in them.  Longer chunks amortise each view's column search,
+  shorter ones give more waves; the halves pay on tall volumes.
 
 ``tune`` times the gather pass alone (the prefix pass is shared) for every candidate
 with HIP events on the caller's stream, takes the median of a few launches, and keeps
@@ -24,7 +26,8 @@ import torch
 CELL = 1
 VOXEL = 2
 TILE_ROWS = (1, 2, 4, 8, 16)
-VOXEL_CHUNKS = (1, 4, 8, 16)
+VOXEL_HALVES = 0x100  # XR_VOXEL_HALVES
+VOXEL_CHUNKS = (1, 4, 8, 16, 16 | VOXEL_HALVES)
 
 
 def _table(plan) -> Dict[tuple, Tuple[int, int]]:

@@ -75,7 +75,8 @@ __host__ __device__ inline int blocks_sino_prefix(const BatchDesc &d)
 }
 __host__ __device__ inline int blocks_voxel(const BatchDesc &d, int KC)
 {
-    return ceil_div(d.va.nx, kVoxTile) * ceil_div(d.va.ny, kVoxTile) * d.nb * ceil_div(d.va.nz, KC) * d.va.vsplit;
+    return ceil_div(d.va.nx, kVoxTile) * ceil_div(d.va.ny, kVoxTile) * d.nb * ceil_div(d.va.nz, chunk_slices(KC)) *
+           d.va.vsplit;
 }
 __host__ __device__ inline int blocks_split_sum(const BatchDesc &d)
 {
@@ -100,27 +101,47 @@ inline bool block_counts_fit(const BatchDesc &d, bool cell, int R, int KC)
         return true;
     }
     if (cd(d.sa.nu, kBlock) * d.sa.nview * nb >= lim) return false;
-    if (cd(d.va.nx, kVoxTile) * cd(d.va.ny, kVoxTile) * nb * cd(d.va.nz, KC) * d.va.vsplit >= lim) return false;
+    if (cd(d.va.nx, kVoxTile) * cd(d.va.ny, kVoxTile) * nb * cd(d.va.nz, chunk_slices(KC)) * d.va.vsplit >= lim)
+        return false;
     return !(d.va.vsplit > 1 && cd(nb * d.va.nz * d.va.ny * d.va.nx, kBlock) >= lim);
 }
 
-// Maps the flat block index to (instance, local block); returns the instance or -1.
+// First block of each instance in one launch (start[n] = the launch's block count),
+// computed on the host and passed by value, so that a workgroup finds its instance with
+// a scan over kernel arguments: no divisions and no memory loads (walking the descriptor
+// table instead costs one dependent scalar load per instance before the work starts).
+struct BlockStarts {
+    int start[XR_BATCH_MAX + 1];
+};
+
 template <typename F>
-__device__ inline int find_instance(const BatchDesc *__restrict__ D, int n, int &bid, F count)
+inline BlockStarts block_starts(const std::vector<BatchDesc> &desc, F count)
 {
-    for (int i = 0; i < n; ++i) {
-        const int c = count(D[i]);
-        if (bid < c) return i;
-        bid -= c;
+    BlockStarts st{};
+    int64_t acc = 0;
+    for (size_t i = 0; i < desc.size(); ++i) {
+        st.start[i] = (int)acc;
+        acc += count(desc[i]);
     }
-    return -1;
+    st.start[desc.size()] = (int)acc;  // the caller has checked that every total stays below 2^31
+    return st;
 }
 
-__global__ __launch_bounds__(kBlock) void k_vol_prefix_batch(const BatchDesc *__restrict__ D, int n, IOPtrs io,
-                                                             char *__restrict__ ws)
+// Maps the flat block index to (instance, local block); returns the instance or -1.
+__device__ inline int find_instance(const BlockStarts &st, int n, int &bid)
+{
+    if (bid >= st.start[n]) return -1;
+    int i = 0;
+    while (i + 1 < n && bid >= st.start[i + 1]) ++i;
+    bid -= st.start[i];
+    return i;
+}
+
+__global__ __launch_bounds__(kBlock) void k_vol_prefix_batch(const BatchDesc *__restrict__ D, int n, BlockStarts st,
+                                                             IOPtrs io, char *__restrict__ ws)
 {
     int bid = blockIdx.x;
-    const int i = find_instance(D, n, bid, [](const BatchDesc &d) { return blocks_vol_prefix(d); });
+    const int i = find_instance(st, n, bid);
     if (i < 0) return;
     const BatchDesc &d = D[i];
     const int64_t plane = (int64_t)d.ca.nx * d.ca.ny;
@@ -135,12 +156,12 @@ __global__ __launch_bounds__(kBlock) void k_vol_prefix_batch(const BatchDesc *__
         vol_prefix_body(io.in[i], reinterpret_cast<float2 *>(ws + d.ws_volx), plane, d.ca.nz, b, idx - b * plane);
 }
 
-__global__ __launch_bounds__(kBlock) void k_transpose_batch(const BatchDesc *__restrict__ D, int n,
+__global__ __launch_bounds__(kBlock) void k_transpose_batch(const BatchDesc *__restrict__ D, int n, BlockStarts st,
                                                             char *__restrict__ ws)
 {
     __shared__ float2 tile[32][33];
     int bid = blockIdx.x;
-    const int i = find_instance(D, n, bid, [](const BatchDesc &d) { return blocks_transpose(d); });
+    const int i = find_instance(st, n, bid);
     if (i < 0) return;
     const BatchDesc &d = D[i];
     const int nx = d.ca.nx, ny = d.ca.ny;
@@ -169,14 +190,11 @@ __global__ __launch_bounds__(kBlock) void k_transpose_batch(const BatchDesc *__r
 // items.  One launch per class, as for a single plan.
 template <int OPK, int GEO, int R, int K, bool BIG>
 __global__ __launch_bounds__(kBlock) void k_cell_batch(
-    const BatchDesc *__restrict__ D, int n, IOPtrs io, char *__restrict__ ws)
+    const BatchDesc *__restrict__ D, int n, BlockStarts st, IOPtrs io, char *__restrict__ ws)
 {
-#if XR_CELL_STAGED
-    __shared__ __attribute__((aligned(16))) float2 lds[kWavesPerBlock][cell_lds_cap<R>()];
-#endif
     constexpr int SPAN = K == 0 ? 2 : K == 1 ? 3 : 0;
     int bid = blockIdx.x;
-    const int i = find_instance(D, n, bid, [](const BatchDesc &d) { return blocks_cell(d, R, K); });
+    const int i = find_instance(st, n, bid);
     if (i < 0) return;
     const BatchDesc &d = D[i];
     const CellArgs a = d.ca;
@@ -195,22 +213,15 @@ __global__ __launch_bounds__(kBlock) void k_cell_batch(
     const float2 *WX = reinterpret_cast<const float2 *>(ws + d.ws_volx);
     const float2 *WY = reinterpret_cast<const float2 *>(ws + d.ws_voly);
     const float wgrp = BIG ? INFINITY : d.cgrp[item].wmax;
-#if XR_CELL_STAGED
-    if (!BIG && (SPAN == 2 || SPAN == 3)) {
-        cell_wave<OPK, GEO, R, (SPAN == 2 ? 2 : 3)>(d.hdr, d.lineC, d.col, d.cgrp, WX, WY, io.out[i], a, c, c < a.nu,
-                                                     cg, r0, v, b, lds[wave]);
-        return;
-    }
-#endif
     if (c < a.nu) cell_body<OPK, GEO, R, SPAN, BIG>(d.hdr, d.lineC, d.col, WX, WY, io.out[i], a, c, r0, v, b, wgrp);
 }
 
 template <int MODE, int GEO>
-__global__ __launch_bounds__(kBlock) void k_sino_prefix_batch(const BatchDesc *__restrict__ D, int n, IOPtrs io,
-                                                              char *__restrict__ ws)
+__global__ __launch_bounds__(kBlock) void k_sino_prefix_batch(const BatchDesc *__restrict__ D, int n, BlockStarts st,
+                                                              IOPtrs io, char *__restrict__ ws)
 {
     int bid = blockIdx.x;
-    const int i = find_instance(D, n, bid, [](const BatchDesc &d) { return blocks_sino_prefix(d); });
+    const int i = find_instance(st, n, bid);
     if (i < 0) return;
     const BatchDesc &d = D[i];
     const int gx = ceil_div(d.sa.nu, kBlock);
@@ -223,13 +234,13 @@ __global__ __launch_bounds__(kBlock) void k_sino_prefix_batch(const BatchDesc *_
         sino_prefix_body<MODE, GEO>(d.hdr, d.col, io.in[i], reinterpret_cast<float2 *>(ws + d.ws_sino), d.sa, b, v, c);
 }
 
-template <int OPK, int GEO, int KC>
-__global__ __launch_bounds__(kBlock) void k_voxel_batch(const BatchDesc *__restrict__ D, int n, IOPtrs io,
-                                                        char *__restrict__ ws)
+template <int OPK, int GEO, int KC, bool HALVES>
+__global__ __launch_bounds__(kBlock) void k_voxel_batch(const BatchDesc *__restrict__ D, int n, BlockStarts st,
+                                                        IOPtrs io, char *__restrict__ ws)
 {
     __shared__ __attribute__((aligned(16))) float2 tile[kVoxLds];
     int bid = blockIdx.x;
-    const int i = find_instance(D, n, bid, [](const BatchDesc &d) { return blocks_voxel(d, KC); });
+    const int i = find_instance(st, n, bid);
     if (i < 0) return;
     const BatchDesc &d = D[i];
     VoxArgs a = d.va;
@@ -243,17 +254,17 @@ __global__ __launch_bounds__(kBlock) void k_voxel_batch(const BatchDesc *__restr
                      : io.out[i];
     int ii, j;
     voxel_coords(bx, by, threadIdx.x, ii, j);
-    voxel_wg<OPK, GEO, KC>(d.hdr, d.lineV, d.col, d.vtile, reinterpret_cast<const float2 *>(ws + d.ws_sino), dst, a, ii,
-                           j, ii < a.nx && j < a.ny, by * gx + bx, (rest % a.nchunk) * KC, rest / a.nchunk, v0, v1,
-                           tile);
+    voxel_wg<OPK, GEO, KC, HALVES>(d.hdr, d.lineV, d.col, d.vtile, reinterpret_cast<const float2 *>(ws + d.ws_sino),
+                                   dst, a, ii, j, ii < a.nx && j < a.ny, by * gx + bx, (rest % a.nchunk) * KC,
+                                   rest / a.nchunk, v0, v1, tile);
 }
 
 // Sum of the view-set partial volumes of every split instance, in set order.
-__global__ __launch_bounds__(kBlock) void k_split_sum_batch(const BatchDesc *__restrict__ D, int n, IOPtrs io,
-                                                            const char *__restrict__ ws)
+__global__ __launch_bounds__(kBlock) void k_split_sum_batch(const BatchDesc *__restrict__ D, int n, BlockStarts st,
+                                                            IOPtrs io, const char *__restrict__ ws)
 {
     int bid = blockIdx.x;
-    const int i = find_instance(D, n, bid, [](const BatchDesc &d) { return blocks_split_sum(d); });
+    const int i = find_instance(st, n, bid);
     if (i < 0) return;
     const BatchDesc &d = D[i];
     const int64_t total = (int64_t)d.nb * d.va.nz * d.va.ny * d.va.nx;
@@ -266,10 +277,13 @@ __global__ __launch_bounds__(kBlock) void k_split_sum_batch(const BatchDesc *__r
 }
 
 template <int OPK, int GEO, int K>
-void launch_cell_batch_class(int R, int grid, const BatchDesc *D, int n, const IOPtrs &io, char *ws, hipStream_t s)
+void launch_cell_batch_class(int R, const BlockStarts &st, const BatchDesc *D, int n, const IOPtrs &io, char *ws,
+                             hipStream_t s)
 {
+    const int grid = st.start[n];
     if (grid <= 0) return;
-#define XR_CELLB(RR) hipLaunchKernelGGL((k_cell_batch<OPK, GEO, RR, K, false>), dim3(grid), dim3(kBlock), 0, s, D, n, io, ws)
+#define XR_CELLB(RR) \
+    hipLaunchKernelGGL((k_cell_batch<OPK, GEO, RR, K, false>), dim3(grid), dim3(kBlock), 0, s, D, n, st, io, ws)
     switch (R) {
     case 1: XR_CELLB(1); break;
     case 2: XR_CELLB(2); break;
@@ -283,52 +297,57 @@ void launch_cell_batch_class(int R, int grid, const BatchDesc *D, int n, const I
 #undef XR_CELLB
 }
 
-// grids[k]: blocks of span-class index k (k = 3: every group, big batches).
+// gs[k]: block starts of span-class index k (k = 3: every group, big batches).
 template <int OPK, int GEO>
-void launch_cell_batch(int R, bool big, const int64_t *grids, const BatchDesc *D, int n, const IOPtrs &io, char *ws,
+void launch_cell_batch(int R, bool big, const BlockStarts *gs, const BatchDesc *D, int n, const IOPtrs &io, char *ws,
                        hipStream_t s)
 {
     if (big) {
-        hipLaunchKernelGGL((k_cell_batch<OPK, GEO, 8, 3, true>), dim3((int)grids[3]), dim3(kBlock), 0, s, D, n, io, ws);
+        hipLaunchKernelGGL((k_cell_batch<OPK, GEO, 8, 3, true>), dim3(gs[3].start[n]), dim3(kBlock), 0, s, D, n, gs[3],
+                           io, ws);
         return;
     }
-    launch_cell_batch_class<OPK, GEO, 0>(R, (int)grids[0], D, n, io, ws, s);
-    launch_cell_batch_class<OPK, GEO, 1>(cell_class_rows(R, 1), (int)grids[1], D, n, io, ws, s);
-    launch_cell_batch_class<OPK, GEO, 2>(cell_class_rows(R, 2), (int)grids[2], D, n, io, ws, s);
+    launch_cell_batch_class<OPK, GEO, 0>(R, gs[0], D, n, io, ws, s);
+    launch_cell_batch_class<OPK, GEO, 1>(cell_class_rows(R, 1), gs[1], D, n, io, ws, s);
+    launch_cell_batch_class<OPK, GEO, 2>(cell_class_rows(R, 2), gs[2], D, n, io, ws, s);
 }
 
 template <int OPK, int GEO>
-void launch_voxel_batch(int KC, int grid, const BatchDesc *D, int n, const IOPtrs &io, char *ws, hipStream_t s)
+void launch_voxel_batch(int KC, const BlockStarts &st, const BatchDesc *D, int n, const IOPtrs &io, char *ws,
+                        hipStream_t s)
 {
+    const dim3 grid(st.start[n]);
     switch (KC) {
-    case 1: hipLaunchKernelGGL((k_voxel_batch<OPK, GEO, 1>), dim3(grid), dim3(kBlock), 0, s, D, n, io, ws); break;
-    case 4: hipLaunchKernelGGL((k_voxel_batch<OPK, GEO, 4>), dim3(grid), dim3(kBlock), 0, s, D, n, io, ws); break;
-    case 8: hipLaunchKernelGGL((k_voxel_batch<OPK, GEO, 8>), dim3(grid), dim3(kBlock), 0, s, D, n, io, ws); break;
-    default: hipLaunchKernelGGL((k_voxel_batch<OPK, GEO, 16>), dim3(grid), dim3(kBlock), 0, s, D, n, io, ws); break;
+    case 1: hipLaunchKernelGGL((k_voxel_batch<OPK, GEO, 1, false>), grid, dim3(kBlock), 0, s, D, n, st, io, ws); break;
+    case 4: hipLaunchKernelGGL((k_voxel_batch<OPK, GEO, 4, false>), grid, dim3(kBlock), 0, s, D, n, st, io, ws); break;
+    case 8: hipLaunchKernelGGL((k_voxel_batch<OPK, GEO, 8, false>), grid, dim3(kBlock), 0, s, D, n, st, io, ws); break;
+    case 16: hipLaunchKernelGGL((k_voxel_batch<OPK, GEO, 16, false>), grid, dim3(kBlock), 0, s, D, n, st, io, ws); break;
+    default: hipLaunchKernelGGL((k_voxel_batch<OPK, GEO, 16, true>), grid, dim3(kBlock), 0, s, D, n, st, io, ws); break;
     }
 }
 
 template <int GEO>
-void dispatch_gather(const Route &rt, int R, bool big, int KC, const int64_t *grids, const BatchDesc *D, int n,
+void dispatch_gather(const Route &rt, int R, bool big, int KC, const BlockStarts *gs, const BatchDesc *D, int n,
                      const IOPtrs &io, char *ws, hipStream_t s)
 {
-    const int grid = (int)grids[0];
     if (rt.cell) {
-        if (rt.kop == CELL_A) launch_cell_batch<CELL_A, GEO>(R, big, grids, D, n, io, ws, s);
-        else if (rt.kop == CELL_BT) launch_cell_batch<CELL_BT, GEO>(R, big, grids, D, n, io, ws, s);
-        else launch_cell_batch<CELL_BWT, GEO>(R, big, grids, D, n, io, ws, s);
+        if (rt.kop == CELL_A) launch_cell_batch<CELL_A, GEO>(R, big, gs, D, n, io, ws, s);
+        else if (rt.kop == CELL_BT) launch_cell_batch<CELL_BT, GEO>(R, big, gs, D, n, io, ws, s);
+        else launch_cell_batch<CELL_BWT, GEO>(R, big, gs, D, n, io, ws, s);
     } else {
-        if (rt.kop == VOX_B) launch_voxel_batch<VOX_B, GEO>(KC, grid, D, n, io, ws, s);
-        else if (rt.kop == VOX_BW) launch_voxel_batch<VOX_BW, GEO>(KC, grid, D, n, io, ws, s);
-        else launch_voxel_batch<VOX_AT, GEO>(KC, grid, D, n, io, ws, s);
+        if (rt.kop == VOX_B) launch_voxel_batch<VOX_B, GEO>(KC, gs[0], D, n, io, ws, s);
+        else if (rt.kop == VOX_BW) launch_voxel_batch<VOX_BW, GEO>(KC, gs[0], D, n, io, ws, s);
+        else launch_voxel_batch<VOX_AT, GEO>(KC, gs[0], D, n, io, ws, s);
     }
 }
 
 template <int GEO>
-void dispatch_sino_prefix(bool at, int grid, const BatchDesc *D, int n, const IOPtrs &io, char *ws, hipStream_t s)
+void dispatch_sino_prefix(bool at, const BlockStarts &st, const BatchDesc *D, int n, const IOPtrs &io, char *ws,
+                          hipStream_t s)
 {
-    if (at) hipLaunchKernelGGL((k_sino_prefix_batch<SINO_AT, GEO>), dim3(grid), dim3(kBlock), 0, s, D, n, io, ws);
-    else hipLaunchKernelGGL((k_sino_prefix_batch<SINO_PLAIN, GEO>), dim3(grid), dim3(kBlock), 0, s, D, n, io, ws);
+    const dim3 grid(st.start[n]);
+    if (at) hipLaunchKernelGGL((k_sino_prefix_batch<SINO_AT, GEO>), grid, dim3(kBlock), 0, s, D, n, st, io, ws);
+    else hipLaunchKernelGGL((k_sino_prefix_batch<SINO_PLAIN, GEO>), grid, dim3(kBlock), 0, s, D, n, st, io, ws);
 }
 
 }  // namespace
@@ -455,7 +474,7 @@ int xr_batch3d_execute(const xr_batch3d *b, int op, const float *const *in, floa
         for (const xr::Plan3d *p : b->plans) KC = std::max(KC, xr::default_chunk(*p));
     }
     if (!xr::valid_rows(R)) return fail(XR_EINVAL, "tile_rows must be 1, 2, 4, 8 or 16");
-    if (!xr::valid_chunk(KC)) return fail(XR_EINVAL, "voxel_chunk must be 1, 4, 8 or 16");
+    if (!xr::valid_chunk(KC)) return fail(XR_EINVAL, "voxel_chunk must be 1, 4, 8, 16 or 16 | XR_VOXEL_HALVES");
     bool big = false;
     for (const xr::BatchDesc &d : b->desc) big = big || d.big;
     if (big) R = 8;  // tables of 4 GiB and more: the 64-bit general variant with 8-row tiles
@@ -482,20 +501,37 @@ int xr_batch3d_execute(const xr_batch3d *b, int op, const float *const *in, floa
     if (g_pre >= INT32_MAX || g_tr >= INT32_MAX || g_sum >= INT32_MAX || g_gather[0] >= INT32_MAX ||
         g_gather[1] >= INT32_MAX || g_gather[2] >= INT32_MAX || g_gather[3] >= INT32_MAX)
         return fail(XR_EINVAL, "batch too large for one launch");
+    // per-launch first block of every instance (kernel arguments)
+    using xr::BatchDesc;
+    xr::BlockStarts gs[4];
     const int kind = p0.kind;
     if (rt.cell) {
-        hipLaunchKernelGGL(xr::k_vol_prefix_batch, dim3((int)g_pre), dim3(xr::kBlock), 0, s, D, n, io, ws);
-        if (g_tr > 0) hipLaunchKernelGGL(xr::k_transpose_batch, dim3((int)g_tr), dim3(xr::kBlock), 0, s, D, n, ws);
+        const xr::BlockStarts pre = xr::block_starts(b->desc, [](const BatchDesc &d) { return xr::blocks_vol_prefix(d); });
+        const xr::BlockStarts tr = xr::block_starts(b->desc, [](const BatchDesc &d) { return xr::blocks_transpose(d); });
+        for (int k = 0; k < 4; ++k) {
+            const int rows = k == 3 ? R : xr::cell_class_rows(R, k);
+            gs[k] = xr::block_starts(b->desc, [rows, k](const BatchDesc &d) { return xr::blocks_cell(d, rows, k); });
+        }
+        hipLaunchKernelGGL(xr::k_vol_prefix_batch, dim3((int)g_pre), dim3(xr::kBlock), 0, s, D, n, pre, io, ws);
+        if (g_tr > 0)
+            hipLaunchKernelGGL(xr::k_transpose_batch, dim3((int)g_tr), dim3(xr::kBlock), 0, s, D, n, tr, ws);
     } else {
         const bool at = rt.kop == xr::VOX_AT;
-        if (kind == xr::GEO_FLAT) xr::dispatch_sino_prefix<xr::GEO_FLAT>(at, (int)g_pre, D, n, io, ws, s);
-        else if (kind == xr::GEO_ARC) xr::dispatch_sino_prefix<xr::GEO_ARC>(at, (int)g_pre, D, n, io, ws, s);
-        else xr::dispatch_sino_prefix<xr::GEO_PAR>(at, (int)g_pre, D, n, io, ws, s);
+        const xr::BlockStarts pre =
+            xr::block_starts(b->desc, [](const BatchDesc &d) { return xr::blocks_sino_prefix(d); });
+        gs[0] = xr::block_starts(b->desc, [KC](const BatchDesc &d) { return xr::blocks_voxel(d, KC); });
+        if (kind == xr::GEO_FLAT) xr::dispatch_sino_prefix<xr::GEO_FLAT>(at, pre, D, n, io, ws, s);
+        else if (kind == xr::GEO_ARC) xr::dispatch_sino_prefix<xr::GEO_ARC>(at, pre, D, n, io, ws, s);
+        else xr::dispatch_sino_prefix<xr::GEO_PAR>(at, pre, D, n, io, ws, s);
     }
-    if (kind == xr::GEO_FLAT) xr::dispatch_gather<xr::GEO_FLAT>(rt, R, big, KC, g_gather, D, n, io, ws, s);
-    else if (kind == xr::GEO_ARC) xr::dispatch_gather<xr::GEO_ARC>(rt, R, big, KC, g_gather, D, n, io, ws, s);
-    else xr::dispatch_gather<xr::GEO_PAR>(rt, R, big, KC, g_gather, D, n, io, ws, s);
-    if (g_sum > 0) hipLaunchKernelGGL(xr::k_split_sum_batch, dim3((int)g_sum), dim3(xr::kBlock), 0, s, D, n, io, ws);
+    if (kind == xr::GEO_FLAT) xr::dispatch_gather<xr::GEO_FLAT>(rt, R, big, KC, gs, D, n, io, ws, s);
+    else if (kind == xr::GEO_ARC) xr::dispatch_gather<xr::GEO_ARC>(rt, R, big, KC, gs, D, n, io, ws, s);
+    else xr::dispatch_gather<xr::GEO_PAR>(rt, R, big, KC, gs, D, n, io, ws, s);
+    if (g_sum > 0) {
+        const xr::BlockStarts sum =
+            xr::block_starts(b->desc, [](const BatchDesc &d) { return xr::blocks_split_sum(d); });
+        hipLaunchKernelGGL(xr::k_split_sum_batch, dim3((int)g_sum), dim3(xr::kBlock), 0, s, D, n, sum, io, ws);
+    }
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(XR_EHIP, std::string("batched launch failed: ") + hipGetErrorString(e));
     return XR_OK;

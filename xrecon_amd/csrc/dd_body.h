@@ -53,6 +53,7 @@ constexpr uint64_t kCellBandBytes = 128ull << 20;
 struct VoxArgs {
     int nx, ny, nz, nu, nv, nview, maxline, nbatch, nchunk;
     float zb, dz, rv0, sod;
+    float zc, kmid;  // volume centre z and Nz / 2: slice boundary k at (k - kmid) dz + zc
     float dx, dy, shx, shy;
     float dvv;
     float hscale;  // dvv / dz
@@ -143,9 +144,17 @@ inline int default_rows(const Plan3d &p)
     return 1;
 }
 
+// Voxel-gather chunk: slices per thread in the low bits, XR_VOXEL_HALVES for 16-slice
+// chunks run as two wave-culled halves.  The halves pay where many chunks straddle the
+// detector's edge rows (tall volumes: config 5's 256 slices, 46.6 -> 29.8 ms for B), and
+// cost where few do (config 2's 64 slices, 2.79 -> 3.03 ms); the autotuner tries both.
+inline int chunk_slices(int k) { return k & 0xff; }
+inline bool chunk_halves(int k) { return (k & XR_VOXEL_HALVES) != 0; }
+
 inline int default_chunk(const Plan3d &p)
 {
     const int nz = p.vol.nz;
+    if (nz >= 128) return 16 | XR_VOXEL_HALVES;
     if (nz >= 64) return 16;
     if (nz >= 8) return 8;
     if (nz >= 4) return 4;
@@ -153,7 +162,7 @@ inline int default_chunk(const Plan3d &p)
 }
 
 inline bool valid_rows(int r) { return r == 1 || r == 2 || r == 4 || r == 8 || r == 16; }
-inline bool valid_chunk(int k) { return k == 1 || k == 4 || k == 8 || k == 16; }
+inline bool valid_chunk(int k) { return k == 1 || k == 4 || k == 8 || k == 16 || k == (16 | XR_VOXEL_HALVES); }
 
 inline CellArgs make_cell_args(const Plan3d &p, int nbatch)
 {
@@ -172,8 +181,10 @@ inline VoxArgs make_vox_args(const Plan3d &p, int nbatch, int KC)
     VoxArgs a;
     a.nx = p.vol.nx; a.ny = p.vol.ny; a.nz = p.vol.nz; a.nu = p.geo.nu; a.nv = p.geo.nv;
     a.nview = p.nview; a.maxline = p.maxline; a.nbatch = nbatch;
+    KC = chunk_slices(KC);
     a.nchunk = (a.nz + KC - 1) / KC;
     a.zb = p.zb; a.dz = p.dz; a.rv0 = p.rv0; a.sod = p.sod;
+    a.zc = p.zc; a.kmid = 0.5f * (float)a.nz;
     a.dx = p.vol.dx; a.dy = p.vol.dy; a.shx = p.vol.shift_x; a.shy = p.vol.shift_y;
     a.dvv = p.dvv;
     a.hscale = p.dvv / p.dz;
@@ -527,100 +538,14 @@ __host__ __device__ inline void cell_init(CellLane<R> &st, const float4 *__restr
     for (int q = 0; q < R; ++q) st.acc[q] = 0.0f;
 }
 
-// Block of z-prefix entries a wave stages in LDS for one line: planes Z0 .. Z0+NZ-1,
-// pixels P0 .. P0+pitch-1 (row-major, pitch even).  pitch = 0: nothing staged.
-struct CellBox {
-    int P0, pitch, Z0, NZ;
-};
-
-// Capacity of one wave's staged block in float2 entries: 8 KiB for 8-row tiles (16 waves
-// per CU fit), 12 KiB for 16-row tiles (at most 12 waves per CU by registers).
-template <int R>
-__host__ __device__ constexpr int cell_lds_cap()
-{
-    return R <= 8 ? 1024 : 1536;
-}
-
-// The block a wave needs on the line with table coefficients lc: pixels from the first
-// column's left boundary to SPAN-1 past the last column's right boundary, planes over
-// the wave's rows at the extremes of the column depth scale sigma of its 64 columns.
-// Lanes whose lookups leave it read the table in HBM, so it decides speed, not results.
-template <int R, int SPAN>
-__host__ __device__ inline CellBox cell_box(const CellArgs &a, const float4 &lc, float dfirst, float dlast,
-                                            float sig_lo, float sig_hi, int r0, int nalong)
-{
-    CellBox bx{0, 0, 0, 0};
-    const float t0 = fmaf(dfirst, lc.y, lc.x), t1 = fmaf(dlast, lc.y, lc.x);
-    const float fal = (float)nalong;
-    if (!(t1 > 0.0f && t0 < fal)) return bx;
-    const int P0 = t0 > 0.0f ? (int)t0 : 0;
-    const int Pend = imin((t1 < fal ? (int)t1 : nalong) + SPAN, nalong + 1);
-    const int pitch = (Pend - P0 + 1) & ~1;
-    const float va = fmaf((float)r0, a.dvv, a.v0), vb = fmaf((float)(r0 + R), a.dvv, a.v0);
-    const float hmin = fminf(fminf(sig_lo * va, sig_lo * vb), fminf(sig_hi * va, sig_hi * vb));
-    const float hmax = fmaxf(fmaxf(sig_lo * va, sig_lo * vb), fmaxf(sig_hi * va, sig_hi * vb));
-    const float fnz = (float)a.nz;
-    const int Z0 = (int)clampf(fmaf(hmin, lc.z, a.zoff), 0.0f, fnz);
-    int NZ = (int)clampf(fmaf(hmax, lc.z, a.zoff), 0.0f, fnz) - Z0 + 1;
-    constexpr int cap = cell_lds_cap<R>();
-    if (pitch <= 0 || pitch > cap) return bx;
-    if (NZ * pitch > cap) NZ = cap / pitch;
-    bx.P0 = P0;
-    bx.pitch = pitch;
-    bx.Z0 = Z0;
-    bx.NZ = NZ;
-    return bx;
-}
-
-// Lane `lane`'s share of copying a wave's block from the line's z-prefix entries (Wl =
-// entry (plane 0, this line, pixel 0); plane stride plane8 bytes) into the wave's LDS
-// block, in 16-byte pixel pairs: pair p = lane + 64 s goes to lds[2p, 2p+1].  A pair may
-// end one entry past the line (the next line's first entry, or the pad after the table);
-// that entry is only ever weighted by zero.  On the device the copies are direct
-// global-to-LDS loads (lane l writes the wave's LDS base + 16 l); the caller waits for
-// them (s_waitcnt) before reading the block.
-__host__ __device__ inline void cell_stage(const float2 *__restrict__ Wl, const CellBox &bx, float2 *__restrict__ lds,
-                                           unsigned plane8, int lane)
-{
-    const int half = bx.pitch >> 1;
-    const int E2 = bx.NZ * half;
-    if (E2 <= 0) return;
-    // (plane, pair) of p = lane, advanced by 64 = dq planes + dc pairs per step; the
-    // divisions by half are exact (quotients stay >= 1/(2 half) from integers).
-    const float inv = 1.0f / (float)half;
-    int r = (int)(((float)lane + 0.5f) * inv);
-    int c = lane - r * half;
-    const int dq = (int)(((float)kWave + 0.5f) * inv), dc = kWave - dq * half;
-    const char *Wb = reinterpret_cast<const char *>(Wl);
-    for (int s = 0; s * kWave < E2; ++s) {
-        const int p = lane + s * kWave;
-        const unsigned off = p < E2 ? mad24((unsigned)(bx.Z0 + r), plane8, (unsigned)(bx.P0 + 2 * c) * 8u) : 0u;
-#if defined(__HIP_DEVICE_COMPILE__)
-        __builtin_amdgcn_global_load_lds(Wb + off, (__attribute__((address_space(3))) void *)(lds + 2 * s * kWave), 16,
-                                         0, 0);
-#else
-        if (p < E2) *reinterpret_cast<Pair2 *>(lds + 2 * p) = *reinterpret_cast<const Pair2 *>(Wb + off);
-#endif
-        r += dq;
-        c += dc;
-        if (c >= half) {
-            c -= half;
-            ++r;
-        }
-    }
-}
-
 // One line L (table coefficients lc) of the cell gather for this thread.  W: the batch
 // item's z-prefix table in the view's orientation.  SPAN = 2, 3 or 4: the thread's column
 // overlaps at most SPAN consecutive pixels of this line (the caller's span test) and the
-// line takes a fixed SPAN-pixel path, reading the wave's staged block `lds` when given
-// (and when the lane's lookups fall in it; two- and three-pixel paths only); SPAN = 0:
-// the general pixel loop.  BIG: 64-bit
+// line takes a fixed SPAN-pixel path; SPAN = 0: the general pixel loop.  BIG: 64-bit
 // table offsets (entries), otherwise 32-bit byte offsets over the wave-uniform line base.
 template <int OPK, int GEO, int R, int SPAN, bool BIG>
 __host__ __device__ inline void cell_line(CellLane<R> &st, const ViewHdr &h, const CellArgs &a, const float4 &lc,
-                                          int L, const float2 *__restrict__ W, const CellBox &bx,
-                                          const float2 *__restrict__ lds)
+                                          int L, const float2 *__restrict__ W)
 {
     const int nalong = wave_uniform(h.nalong);
     const float fal = (float)nalong;
@@ -681,33 +606,18 @@ __host__ __device__ inline void cell_line(CellLane<R> &st, const ViewHdr &h, con
             }
         }
         // Lookups per row boundary: entries m0, m0+1 (one 16-byte pair), and m0+2 (three
-        // pixels) or m0+2, m0+3 (a second pair, four pixels), from the wave's staged block
-        // when they all fall in it, else from the table.
+        // pixels) or m0+2, m0+3 (a second pair, four pixels).
         Pair2 e[R + 1];
         float2 g[R + 1];
         Pair2 e2[R + 1];
-        const bool staged = SPAN != 4 && !BIG && lds != nullptr && bx.pitch > 0 && ki[0] >= bx.Z0 &&
-                            ki[R] < bx.Z0 + bx.NZ && m0 + SPAN - 1 < bx.P0 + bx.pitch;
-        if (staged) {
-            const unsigned p8 = (unsigned)bx.pitch * 8u;
-            const unsigned base = (unsigned)(m0 - bx.P0) * 8u - (unsigned)bx.Z0 * p8;
-            const char *Lb = reinterpret_cast<const char *>(lds);
 #pragma unroll
-            for (int q = 0; q <= R; ++q) {
-                const char *pq = Lb + mad24((unsigned)ki[q], p8, base);
-                e[q] = *reinterpret_cast<const Pair2 *>(pq);
-                if (SPAN == 3) g[q] = *reinterpret_cast<const float2 *>(pq + 16);
-            }
-        } else {
-#pragma unroll
-            for (int q = 0; q <= R; ++q) {
-                const float2 *pq;
-                if (BIG) pq = Wl + ((int64_t)mul24((unsigned)ki[q], (unsigned)plane) + m0);
-                else pq = reinterpret_cast<const float2 *>(Wb + mad24((unsigned)ki[q], a.plane8, (unsigned)m0 * 8u));
-                e[q] = *reinterpret_cast<const Pair2 *>(pq);
-                if (SPAN == 3) g[q] = pq[2];
-                if (SPAN == 4) e2[q] = *reinterpret_cast<const Pair2 *>(pq + 2);
-            }
+        for (int q = 0; q <= R; ++q) {
+            const float2 *pq;
+            if (BIG) pq = Wl + ((int64_t)mul24((unsigned)ki[q], (unsigned)plane) + m0);
+            else pq = reinterpret_cast<const float2 *>(Wb + mad24((unsigned)ki[q], a.plane8, (unsigned)m0 * 8u));
+            e[q] = *reinterpret_cast<const Pair2 *>(pq);
+            if (SPAN == 3) g[q] = pq[2];
+            if (SPAN == 4) e2[q] = *reinterpret_cast<const Pair2 *>(pq + 2);
         }
         float J0 = fmaf(fr[0], e[0].y0, e[0].x0);
         float J1 = fmaf(fr[0], e[0].y1, e[0].x1);
@@ -811,61 +721,20 @@ __host__ __device__ inline void cell_body(const ViewHdr *__restrict__ hdr, const
     cell_init<R>(st, colt + (int64_t)v * (a.nu + 1), a, c, r0);
     const float2 *W = cell_table(WX, WY, h, a, b);
     const float4 *lt = lineC + (int64_t)v * a.maxline;
-    const CellBox none{0, 0, 0, 0};
     for (int L = 0; L < h.nline; ++L) {
         const float4 lc = lt[L];
         if (SPAN == 2) {
-            cell_line<OPK, GEO, R, 2, BIG>(st, h, a, lc, L, W, none, nullptr);
+            cell_line<OPK, GEO, R, 2, BIG>(st, h, a, lc, L, W);
             continue;
         }
         // wgrp / Delta_L: the group's widest column in pixels of this line (wave-uniform)
         const float s = wgrp * lc.y;
-        if (s < 0.99f) cell_line<OPK, GEO, R, 2, BIG>(st, h, a, lc, L, W, none, nullptr);
-        else if (SPAN == 3 || s < 1.99f) cell_line<OPK, GEO, R, 3, BIG>(st, h, a, lc, L, W, none, nullptr);
-        else if (s < 2.99f) cell_line<OPK, GEO, R, 4, BIG>(st, h, a, lc, L, W, none, nullptr);
-        else cell_line<OPK, GEO, R, 0, BIG>(st, h, a, lc, L, W, none, nullptr);
+        if (s < 0.99f) cell_line<OPK, GEO, R, 2, BIG>(st, h, a, lc, L, W);
+        else if (SPAN == 3 || s < 1.99f) cell_line<OPK, GEO, R, 3, BIG>(st, h, a, lc, L, W);
+        else if (s < 2.99f) cell_line<OPK, GEO, R, 4, BIG>(st, h, a, lc, L, W);
+        else cell_line<OPK, GEO, R, 0, BIG>(st, h, a, lc, L, W);
     }
     cell_store<OPK, GEO, R>(st, h, a, out, c, r0, v, b);
-}
-
-// Whether the cell kernels stage each line's block in LDS per wave (cell_wave) or read
-// the table directly (cell_body).  Staging as written waits for each line's copy before
-// gathering it, which exposes one memory round trip per line and measured 2.7x slower
-// (10.9 ms against 4.0 ms for cfg2 A); it stays here, tested through the emulator, off.
-#ifndef XR_CELL_STAGED
-#define XR_CELL_STAGED 0
-#endif
-
-// A wave of the cell gather with per-line LDS staging (device only): columns
-// cg*64 .. cg*64+63 (this lane: c, valid when `lane_ok`), rows r0..r0+R-1 of view v,
-// batch b, fixed span SPAN (2 or 3).  Per line the wave copies the block of the z-prefix
-// table its lanes read into its own LDS block `lds` (no other wave touches it, so no
-// barrier: the wave waits for its own loads), then every lane gathers from it.
-template <int OPK, int GEO, int R, int SPAN>
-__device__ inline void cell_wave(const ViewHdr *__restrict__ hdr, const float4 *__restrict__ lineC,
-                                 const float4 *__restrict__ colt, const CellGroup *__restrict__ cgrp,
-                                 const float2 *__restrict__ WX, const float2 *__restrict__ WY,
-                                 float *__restrict__ out, const CellArgs &a, int c, bool lane_ok, int cg, int r0,
-                                 int v, int b, float2 *__restrict__ lds)
-{
-    const ViewHdr h = hdr[v];
-    const float4 *ct = colt + (int64_t)v * (a.nu + 1);
-    CellLane<R> st;
-    cell_init<R>(st, ct, a, lane_ok ? c : cg * kWave, r0);
-    const float2 *W = cell_table(WX, WY, h, a, b);
-    const float4 *lt = lineC + (int64_t)v * a.maxline;
-    const float dfirst = ct[cg * kWave].x, dlast = ct[imin(a.nu, cg * kWave + kWave)].x;
-    const CellGroup grp = cgrp[(int64_t)v * ceil_div(a.nu, kWave) + cg];
-    const int lane = threadIdx.x & (kWave - 1);
-    for (int L = 0; L < h.nline; ++L) {
-        const float4 lc = lt[L];
-        const CellBox bx = cell_box<R, SPAN>(a, lc, dfirst, dlast, grp.sig_lo, grp.sig_hi, r0, h.nalong);
-        if (bx.pitch == 0) continue;  // no column of the wave meets this line
-        cell_stage(W + (int64_t)L * h.nalong, bx, lds, a.plane8, lane);
-        __builtin_amdgcn_s_waitcnt(0);  // this wave's direct-to-LDS loads have landed
-        if (lane_ok) cell_line<OPK, GEO, R, SPAN, false>(st, h, a, lc, L, W, bx, lds);
-    }
-    if (lane_ok) cell_store<OPK, GEO, R>(st, h, a, out, c, r0, v, b);
 }
 
 // ---------------------------------------------------------------- voxel gather
@@ -1026,13 +895,37 @@ __host__ __device__ inline bool wave_any(bool p)
 #endif
 }
 
+// A float2 at byte offset off of an LDS block / of a table in global memory.  On the
+// device the loads name their address space, so that the compiler keeps them apart
+// (ds_read vs global_load) instead of merging the two into generic "flat" loads through
+// a selected pointer.
+__host__ __device__ inline float2 lds_float2(const char *base, unsigned off)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return *(const __attribute__((address_space(3))) float2 *)((const __attribute__((address_space(3))) char *)base + off);
+#else
+    return *reinterpret_cast<const float2 *>(base + off);
+#endif
+}
+__host__ __device__ inline float2 global_float2(const char *base, unsigned off)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return *(const __attribute__((address_space(1))) float2 *)((const __attribute__((address_space(1))) char *)base + off);
+#else
+    return *reinterpret_cast<const float2 *>(base + off);
+#endif
+}
+
 // Where one column's row-prefix entries are read from: the staged LDS block (when the
-// column lies in it) or the view's table in HBM.
+// column lies in it) or the view's table in HBM.  Lb is the LDS block itself (never
+// replaced by a null pointer: a pointer that may be either would compile to generic
+// "flat" loads for the LDS reads); in_lds says whether this column is staged.
 struct VoxCol {
     const char *Qb;   // view's table
     unsigned nu8;     // byte stride of a table row
     unsigned c8;      // byte offset of the column in a table row
-    const char *Lb;   // LDS block, or nullptr when the column is not staged
+    const char *Lb;   // LDS block (nullptr only where no block exists at all)
+    bool in_lds;      // the column lies in the staged block
     unsigned lcb;     // (c - c0) * 8 - r0 * pitch * 8: LDS byte offset of row 0 (mod 2^32)
     unsigned nc8;     // LDS row pitch in bytes
     int r0, r1;       // staged rows [r0, r1)
@@ -1057,19 +950,17 @@ __host__ __device__ inline float voxel_slices(const VoxCol &cs, float rlo, float
     }
     float2 e[N + 1];
     // rows increase with q, so the first and last lookups bound the span's rows
-    const bool staged = cs.Lb != nullptr && ri[0] >= cs.r0 && ri[N] < cs.r1;
+    const bool staged = cs.in_lds && ri[0] >= cs.r0 && ri[N] < cs.r1;
     if (staged) {
         // byte offset (ri - r0) * pitch * 8 + (c - c0) * 8 as one 24-bit multiply-add
         // (unsigned wrap-around of the constant part is exact)
-        if (!have_first) e[0] = *reinterpret_cast<const float2 *>(cs.Lb + mad24((unsigned)ri[0], cs.nc8, cs.lcb));
+        if (!have_first) e[0] = lds_float2(cs.Lb, mad24((unsigned)ri[0], cs.nc8, cs.lcb));
 #pragma unroll
-        for (int q = 1; q <= N; ++q)
-            e[q] = *reinterpret_cast<const float2 *>(cs.Lb + mad24((unsigned)ri[q], cs.nc8, cs.lcb));
+        for (int q = 1; q <= N; ++q) e[q] = lds_float2(cs.Lb, mad24((unsigned)ri[q], cs.nc8, cs.lcb));
     } else {
-        if (!have_first) e[0] = *reinterpret_cast<const float2 *>(cs.Qb + (mul24((unsigned)ri[0], cs.nu8) + cs.c8));
+        if (!have_first) e[0] = global_float2(cs.Qb, mul24((unsigned)ri[0], cs.nu8) + cs.c8);
 #pragma unroll
-        for (int q = 1; q <= N; ++q)
-            e[q] = *reinterpret_cast<const float2 *>(cs.Qb + (mul24((unsigned)ri[q], cs.nu8) + cs.c8));
+        for (int q = 1; q <= N; ++q) e[q] = global_float2(cs.Qb, mul24((unsigned)ri[q], cs.nu8) + cs.c8);
     }
     // I(rho) = B_r + rho q_r (see prefix_column)
     float Ip = have_first ? Ifirst : fmaf(rh[0], e[0].y, e[0].x);
@@ -1085,11 +976,11 @@ __host__ __device__ inline float voxel_slices(const VoxCol &cs, float rlo, float
 // One view's contribution to the KC slice accumulators of voxel column (i, j).  With a
 // staged block (lds != nullptr), lookups inside it read LDS, the rest the table in HBM.
 //
-// Chunks of 16 slices run as two halves of 8, each skipped when no lane of the wave has
-// a row of that half on the detector: every boundary of a skipped half then clamps to
-// the same row (0 below, Nv above) for every lane, so its differences are exactly zero
-// and skipping it changes no result.
-template <int OPK, int GEO, int KC>
+// With HALVES, chunks of 16 slices run as two halves of 8, each skipped when no lane of
+// the wave has a row of that half on the detector: every boundary of a skipped half then
+// clamps to the same row (0 below, Nv above) for every lane, so its differences are
+// exactly zero and skipping it changes no result.
+template <int OPK, int GEO, int KC, bool HALVES = false>
 __host__ __device__ inline void voxel_view(const ViewHdr &h, const float4 *__restrict__ lineV,
                                            const float4 *__restrict__ ct, const float2 *__restrict__ Qv,
                                            const VoxArgs &a, int v, int i, int j, int k0, float px, float py,
@@ -1120,7 +1011,11 @@ __host__ __device__ inline void voxel_view(const ViewHdr &h, const float4 *__res
     float4 cc = ct[c], cn = ct[c + 1];
     while (c > 0 && cc.x > a0) { --c; cn = cc; cc = ct[c]; }
 
-    const float zbz = lv.z * a.zb, zsz = lv.z * a.dz;
+    // Slice boundaries are measured from the volume centre, (k - Nz/2) dz + zc: the rows
+    // of the slices that reach the detector then come out of small products instead of
+    // the difference of two large ones (from the volume bottom, a tall volume lost about
+    // Nz/2 x 2^-24 rows to cancellation).
+    const float zcz = lv.z * a.zc, zsz = lv.z * a.dz;
     // Byte offsets below 2^32 within one view's table ((Nv+1)*Nu*8, checked at plan
     // creation) over a wave-uniform base.
     const char *Qb = reinterpret_cast<const char *>(Qv);
@@ -1136,17 +1031,17 @@ __host__ __device__ inline void voxel_view(const ViewHdr &h, const float4 *__res
             // Row coordinate of slice boundary k: rho_k = (zeta z_k / sigma - v0) / dvv.
             const float kap = cc.z;
             const float step = zsz * kap;
-            const float base = fmaf(zbz, kap, -a.rv0);
-            const float rlo = fmaf((float)k0, step, base);
-            const float rhi = fmaf((float)(k0 + KC), step, base);
+            const float base = fmaf(zcz, kap, -a.rv0);
+            const float rlo = fmaf((float)k0 - a.kmid, step, base);
+            const float rhi = fmaf((float)(k0 + KC) - a.kmid, step, base);
             if (rhi > 0.0f && rlo < fnv) {
                 // B: ou * U w sigma dvv / (Delta zeta dz); A^T: ou (normalisation folded
                 // into the prefix table).
                 const float w = (OPK == VOX_AT) ? ou : ou * hB * cc.y;
                 const bool in_lds = lds != nullptr && (unsigned)(c - bx.c0) < (unsigned)bx.nc;
-                const VoxCol cs{Qb, nu8, (unsigned)c * 8u, in_lds ? reinterpret_cast<const char *>(lds) : nullptr,
+                const VoxCol cs{Qb, nu8, (unsigned)c * 8u, reinterpret_cast<const char *>(lds), in_lds,
                                 (unsigned)(c - bx.c0) * 8u - (unsigned)bx.r0 * nc8, nc8, bx.r0, bx.r0 + bx.nr};
-                if (KC < 16) {
+                if (!HALVES || KC < 16) {
                     voxel_slices<0, KC>(cs, rlo, step, fnv, w, false, 0.0f, acc);
                 } else {
                     constexpr int H = KC / 2;
@@ -1188,7 +1083,7 @@ __host__ __device__ inline void voxel_store(float *__restrict__ out, const VoxAr
 
 // One thread, reading the tables in HBM only: voxel column (i, j), slices k0..k0+KC-1,
 // batch b, views [v0, v1).
-template <int OPK, int GEO, int KC>
+template <int OPK, int GEO, int KC, bool HALVES = false>
 __host__ __device__ inline void voxel_body(const ViewHdr *__restrict__ hdr, const float4 *__restrict__ lineV,
                                            const float4 *__restrict__ colt, const float2 *__restrict__ Q,
                                            float *__restrict__ out, const VoxArgs &a, int i, int j, int k0, int b,
@@ -1199,8 +1094,8 @@ __host__ __device__ inline void voxel_body(const ViewHdr *__restrict__ hdr, cons
     for (int q = 0; q < KC; ++q) acc[q] = 0.0f;
     const VoxBox none{0, 0, 0, 0, 0, 0};
     for (int v = v0; v < v1; ++v)
-        voxel_view<OPK, GEO, KC>(hdr[v], lineV, colt + (int64_t)v * (a.nu + 1), view_table(Q, a, b, v), a, v, i, j,
-                                 k0, px, py, acc, none, nullptr);
+        voxel_view<OPK, GEO, KC, HALVES>(hdr[v], lineV, colt + (int64_t)v * (a.nu + 1), view_table(Q, a, b, v), a, v,
+                                         i, j, k0, px, py, acc, none, nullptr);
     voxel_store<KC>(out, a, i, j, k0, b, acc);
 }
 
@@ -1216,7 +1111,7 @@ __host__ __device__ inline void voxel_body(const ViewHdr *__restrict__ hdr, cons
 // serves too few lookups to repay its copy and the two barriers per view.
 constexpr int kVoxStageMin = 8;
 
-template <int OPK, int GEO, int KC>
+template <int OPK, int GEO, int KC, bool HALVES = false>
 __device__ inline void voxel_wg(const ViewHdr *__restrict__ hdr, const float4 *__restrict__ lineV,
                                 const float4 *__restrict__ colt, const VoxTile *__restrict__ vtile,
                                 const float2 *__restrict__ Q, float *__restrict__ out, const VoxArgs &a, int i, int j,
@@ -1224,7 +1119,7 @@ __device__ inline void voxel_wg(const ViewHdr *__restrict__ hdr, const float4 *_
                                 float2 *__restrict__ tile)
 {
     if (KC < kVoxStageMin) {
-        if (active) voxel_body<OPK, GEO, KC>(hdr, lineV, colt, Q, out, a, i, j, k0, b, v0, v1);
+        if (active) voxel_body<OPK, GEO, KC, HALVES>(hdr, lineV, colt, Q, out, a, i, j, k0, b, v0, v1);
         return;
     }
     const float px = voxel_px(a, i), py = voxel_py(a, j);
@@ -1240,7 +1135,7 @@ __device__ inline void voxel_wg(const ViewHdr *__restrict__ hdr, const float4 *_
         voxel_stage(Qv, bx, tile, a.nu, threadIdx.x, kBlock);
         __builtin_amdgcn_s_waitcnt(0);  // this wave's direct-to-LDS loads have landed
         __syncthreads();
-        if (active) voxel_view<OPK, GEO, KC>(h, lineV, ct, Qv, a, v, i, j, k0, px, py, acc, bx, tile);
+        if (active) voxel_view<OPK, GEO, KC, HALVES>(h, lineV, ct, Qv, a, v, i, j, k0, px, py, acc, bx, tile);
         __syncthreads();
     }
     if (active) voxel_store<KC>(out, a, i, j, k0, b, acc);

@@ -237,9 +237,6 @@ __global__ __launch_bounds__(kBlock) void k_cell(const ViewHdr *__restrict__ hdr
                                                  const float2 *__restrict__ WX, const float2 *__restrict__ WY,
                                                  float *__restrict__ out, CellArgs a)
 {
-#if XR_CELL_STAGED
-    __shared__ __attribute__((aligned(16))) float2 lds[kWavesPerBlock][cell_lds_cap<R>()];
-#endif
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     const int ncg = ceil_div(a.nu, kWave);
@@ -263,18 +260,6 @@ __global__ __launch_bounds__(kBlock) void k_cell(const ViewHdr *__restrict__ hdr
     const int c = cg * kWave + lane;
     const int r0 = rt * R;
     const float wgrp = BIG ? INFINITY : cgrp[item].wmax;
-#if XR_CELL_STAGED
-    {
-        const int span = SPAN >= 0 ? SPAN : cgrp[item].span;
-        if (span == 2 || span == 3) {
-            if (span == 2)
-                cell_wave<OPK, GEO, R, 2>(hdr, lineC, colt, cgrp, WX, WY, out, a, c, c < a.nu, cg, r0, v, b, lds[wave]);
-            else
-                cell_wave<OPK, GEO, R, 3>(hdr, lineC, colt, cgrp, WX, WY, out, a, c, c < a.nu, cg, r0, v, b, lds[wave]);
-            return;
-        }
-    }
-#endif
     if (c >= a.nu) return;
     if (SPAN >= 0) {
         cell_body<OPK, GEO, R, (SPAN >= 0 ? SPAN : 0), BIG>(hdr, lineC, colt, WX, WY, out, a, c, r0, v, b, wgrp);
@@ -286,7 +271,7 @@ __global__ __launch_bounds__(kBlock) void k_cell(const ViewHdr *__restrict__ hdr
     else cell_body<OPK, GEO, R, 0, false>(hdr, lineC, colt, WX, WY, out, a, c, r0, v, b, wgrp);
 }
 
-template <int OPK, int GEO, int KC>
+template <int OPK, int GEO, int KC, bool HALVES>
 __global__ __launch_bounds__(kBlock) void k_voxel(const ViewHdr *__restrict__ hdr, const float4 *__restrict__ lineV,
                                                   const float4 *__restrict__ colt, const VoxTile *__restrict__ vtile,
                                                   const float2 *__restrict__ Q, float *__restrict__ out,
@@ -301,8 +286,8 @@ __global__ __launch_bounds__(kBlock) void k_voxel(const ViewHdr *__restrict__ hd
     const int v0 = set * a.vper, v1 = imin(a.nview, v0 + a.vper);
     // one view set: its own partial volume; no split: the output itself
     float *dst = a.vsplit > 1 ? part + (int64_t)set * a.nbatch * a.nz * a.ny * a.nx : out;
-    voxel_wg<OPK, GEO, KC>(hdr, lineV, colt, vtile, Q, dst, a, i, j, i < a.nx && j < a.ny,
-                           blockIdx.y * gridDim.x + blockIdx.x, chunk * KC, b, v0, v1, tile);
+    voxel_wg<OPK, GEO, KC, HALVES>(hdr, lineV, colt, vtile, Q, dst, a, i, j, i < a.nx && j < a.ny,
+                                   blockIdx.y * gridDim.x + blockIdx.x, chunk * KC, b, v0, v1, tile);
 }
 
 // out[e] = sum over sets s = 0..nset-1 of part[s * n + e], in set order.
@@ -377,10 +362,11 @@ void launch_voxel(const Plan3d &p, int KC, const float2 *Q, float *out, float *p
     const VoxArgs a = make_vox_args(p, nbatch, KC);
     dim3 grid(ceil_div(a.nx, kVoxTile), ceil_div(a.ny, kVoxTile), nbatch * a.nchunk * a.vsplit);
     switch (KC) {
-    case 1: hipLaunchKernelGGL((k_voxel<OPK, GEO, 1>), grid, dim3(kBlock), 0, s, p.d_hdr, p.d_lineV, p.d_col, p.d_vtile, Q, out, part, a); break;
-    case 4: hipLaunchKernelGGL((k_voxel<OPK, GEO, 4>), grid, dim3(kBlock), 0, s, p.d_hdr, p.d_lineV, p.d_col, p.d_vtile, Q, out, part, a); break;
-    case 8: hipLaunchKernelGGL((k_voxel<OPK, GEO, 8>), grid, dim3(kBlock), 0, s, p.d_hdr, p.d_lineV, p.d_col, p.d_vtile, Q, out, part, a); break;
-    default: hipLaunchKernelGGL((k_voxel<OPK, GEO, 16>), grid, dim3(kBlock), 0, s, p.d_hdr, p.d_lineV, p.d_col, p.d_vtile, Q, out, part, a); break;
+    case 1: hipLaunchKernelGGL((k_voxel<OPK, GEO, 1, false>), grid, dim3(kBlock), 0, s, p.d_hdr, p.d_lineV, p.d_col, p.d_vtile, Q, out, part, a); break;
+    case 4: hipLaunchKernelGGL((k_voxel<OPK, GEO, 4, false>), grid, dim3(kBlock), 0, s, p.d_hdr, p.d_lineV, p.d_col, p.d_vtile, Q, out, part, a); break;
+    case 8: hipLaunchKernelGGL((k_voxel<OPK, GEO, 8, false>), grid, dim3(kBlock), 0, s, p.d_hdr, p.d_lineV, p.d_col, p.d_vtile, Q, out, part, a); break;
+    case 16: hipLaunchKernelGGL((k_voxel<OPK, GEO, 16, false>), grid, dim3(kBlock), 0, s, p.d_hdr, p.d_lineV, p.d_col, p.d_vtile, Q, out, part, a); break;
+    default: hipLaunchKernelGGL((k_voxel<OPK, GEO, 16, true>), grid, dim3(kBlock), 0, s, p.d_hdr, p.d_lineV, p.d_col, p.d_vtile, Q, out, part, a); break;
     }
 }
 
@@ -471,7 +457,7 @@ int execute_fdk(const Plan3d &p, const float *in, float *out, int nbatch, const 
     if (ws_bytes < need || !ws)
         return fail(XR_EWORKSPACE, "workspace of " + std::to_string(ws_bytes) + " bytes, need " + std::to_string(need));
     if (voxel_chunk == 0) voxel_chunk = default_chunk(p);
-    if (!valid_chunk(voxel_chunk)) return fail(XR_EINVAL, "voxel_chunk must be 1, 4, 8 or 16");
+    if (!valid_chunk(voxel_chunk)) return fail(XR_EINVAL, "voxel_chunk must be 1, 4, 8, 16 or 16 | XR_VOXEL_HALVES");
     float2 *Q = static_cast<float2 *>(ws);
     // the taps live where the view-split partial volumes go later (workspace_bytes
     // reserves room for both); the gather runs after the filter stage on the same stream
@@ -541,7 +527,7 @@ int execute_phase(const Plan3d &p, int op, int phase, const float *in, float *ou
     if (tile_rows == 0) tile_rows = default_rows(p);
     if (voxel_chunk == 0) voxel_chunk = default_chunk(p);
     if (!valid_rows(tile_rows)) return fail(XR_EINVAL, "tile_rows must be 1, 2, 4, 8 or 16");
-    if (!valid_chunk(voxel_chunk)) return fail(XR_EINVAL, "voxel_chunk must be 1, 4, 8 or 16");
+    if (!valid_chunk(voxel_chunk)) return fail(XR_EINVAL, "voxel_chunk must be 1, 4, 8, 16 or 16 | XR_VOXEL_HALVES");
 
     if (rt.cell && cell_blocks(make_cell_args(p, nbatch), 1, p.ncls[0] + p.ncls[1] + p.ncls[2], nbatch) >=
                        ((int64_t)1 << 31))

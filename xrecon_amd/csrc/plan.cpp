@@ -71,6 +71,7 @@ int build_tables(Plan3d &p, const float *angles, PlanTables &t, std::string &err
     p.dvv = (float)dvv;
     p.rv0 = (float)(v0 / dvv);
     p.zb = (float)zb;
+    p.zc = (float)(zb + 0.5 * (is2d ? 1 : nz) * dzz);
     p.dz = (float)dzz;
     p.zoff = (float)(-zb / dzz);
     p.sod = (float)sod;
@@ -270,7 +271,7 @@ int build_tables(Plan3d &p, const float *angles, PlanTables &t, std::string &err
     // Voxel gathers on volumes too small to fill the GPU split their views into sets.
     const int gx = ceil_div(nx, kVoxTile), gy = ceil_div(ny, kVoxTile);
     {
-        const int64_t groups = (int64_t)gx * gy * ceil_div(nz, default_chunk(p));
+        const int64_t groups = (int64_t)gx * gy * ceil_div(nz, chunk_slices(default_chunk(p)));
         int sp = groups >= kVoxMinGroups ? 1 : (int)std::min<int64_t>(16, ceil_div(kVoxMinGroups, groups));
         p.vsplit = std::max(1, std::min(sp, V));
     }

@@ -54,7 +54,7 @@ struct VoxTile {
 static_assert(sizeof(VoxTile) == 16, "VoxTile layout");
 
 // Per (view, 64-column group of the cell gather): the range of the depth scale sigma over
-// the group's columns (staged variant), the group's span class -- 2 / 3: each of its
+// the group's columns (descriptive; no kernel reads it), the group's span class -- 2 / 3: each of its
 // columns overlaps at most 2 / 3 consecutive pixels of any line, 0: no bound -- and its
 // widest column on the axis, wmax.  On a diagonal view the edge columns of a flat panel
 // are wider on the axis than the central ones, so most groups of a three-pixel view
@@ -83,6 +83,7 @@ struct Plan3d {
     int vsplit = 1;          // voxel gathers: views split over this many workgroup sets (summed after)
     // Scalars shared by all views.
     float zb = 0, dz = 1;    // volume bottom z and slice thickness
+    float zc = 0;            // volume centre z (slice boundary k sits at (k - Nz/2) dz + zc)
     float v0 = 0, dvv = 1;   // first row boundary and row pitch on the iso plane
     float rv0 = 0;           // v0 / dvv
     float zoff = 0;          // -zb / dz
