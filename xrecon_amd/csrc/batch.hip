@@ -206,13 +206,13 @@ __global__ __launch_bounds__(kBlock) void k_cell_batch(
     const int quad = rest % nquad, b = rest / nquad;
     const int idx = quad * kWavesPerBlock + wave;
     if (idx >= nk) return;  // the whole wave
-    const int item = d.citems[cell_class_offset(d, K) + idx];
+    const int item = ro_load(d.citems + cell_class_offset(d, K) + idx);
     const int v = item / ncg, cg = item - v * ncg;
     const int c = cg * kWave + lane;
     const int r0 = rt * R;
     const float2 *WX = reinterpret_cast<const float2 *>(ws + d.ws_volx);
     const float2 *WY = reinterpret_cast<const float2 *>(ws + d.ws_voly);
-    const float wgrp = BIG ? INFINITY : d.cgrp[item].wmax;
+    const float wgrp = BIG ? INFINITY : ro_load(&d.cgrp[item].wmax);
     if (c < a.nu) cell_body<OPK, GEO, R, SPAN, BIG>(d.hdr, d.lineC, d.col, WX, WY, io.out[i], a, c, r0, v, b, wgrp);
 }
 

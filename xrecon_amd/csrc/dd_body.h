@@ -36,6 +36,50 @@ __host__ __device__ inline void voxel_coords(int bx, int by, int tid, int &i, in
     j = by * kVoxTile + (wave >> 1) * 8 + (lane >> 3);
 }
 
+// A wave-uniform read of a plan table (per-view headers, per-line entries, column-group
+// lists): read-only for the whole launch, so on the device it is read through the
+// constant address space, which the compiler serves with scalar loads.  The per-instance
+// kernels get those from their __restrict__ kernel arguments anyway; the batched kernels
+// take their table pointers from a descriptor in memory, where a plain dereference would
+// be a generic vector ("flat") load that every wave waits on.
+// The copy goes word by word: a struct with its own copy constructor (float4) would
+// otherwise bind the source to a generic reference and lose the address space.
+template <class T>
+__host__ __device__ inline T ro_load(const T *p)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    static_assert(sizeof(T) % 4 == 0, "tables of 32-bit words");
+    const __attribute__((address_space(4))) unsigned *q = (const __attribute__((address_space(4))) unsigned *)p;
+    unsigned w[sizeof(T) / 4];
+#pragma unroll
+    for (unsigned k = 0; k < sizeof(T) / 4; ++k) w[k] = q[k];
+    T r;
+    __builtin_memcpy(&r, w, sizeof(T));
+    return r;
+#else
+    return *p;
+#endif
+}
+
+// A per-lane read of a plan table in global memory (the same word-by-word copy, through
+// the global address space): vector loads, never generic ones, in the batched kernels.
+template <class T>
+__host__ __device__ inline T g_load(const T *p)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    static_assert(sizeof(T) % 4 == 0, "tables of 32-bit words");
+    const __attribute__((address_space(1))) unsigned *q = (const __attribute__((address_space(1))) unsigned *)p;
+    unsigned w[sizeof(T) / 4];
+#pragma unroll
+    for (unsigned k = 0; k < sizeof(T) / 4; ++k) w[k] = q[k];
+    T r;
+    __builtin_memcpy(&r, w, sizeof(T));
+    return r;
+#else
+    return *p;
+#endif
+}
+
 struct CellArgs {
     int nx, ny, nz, nu, nv, nview, maxline, nbatch;
     float v0, dvv, zoff, sod;
@@ -531,9 +575,9 @@ template <int R>
 __host__ __device__ inline void cell_init(CellLane<R> &st, const float4 *__restrict__ ct, const CellArgs &a, int c,
                                           int r0)
 {
-    st.cc = ct[c];
+    st.cc = g_load(ct + c);
     st.d0 = st.cc.x;
-    st.d1 = ct[c + 1].x;
+    st.d1 = g_load(&ct[c + 1].x);
     for (int q = 0; q <= R; ++q) st.hv[q] = st.cc.y * (a.v0 + (float)(r0 + q) * a.dvv);
     for (int q = 0; q < R; ++q) st.acc[q] = 0.0f;
 }
@@ -716,13 +760,13 @@ __host__ __device__ inline void cell_body(const ViewHdr *__restrict__ hdr, const
                                           const float2 *__restrict__ WY, float *__restrict__ out, const CellArgs &a,
                                           int c, int r0, int v, int b, float wgrp)
 {
-    const ViewHdr h = hdr[v];
+    const ViewHdr h = ro_load(hdr + v);
     CellLane<R> st;
     cell_init<R>(st, colt + (int64_t)v * (a.nu + 1), a, c, r0);
     const float2 *W = cell_table(WX, WY, h, a, b);
     const float4 *lt = lineC + (int64_t)v * a.maxline;
     for (int L = 0; L < h.nline; ++L) {
-        const float4 lc = lt[L];
+        const float4 lc = ro_load(lt + L);
         if (SPAN == 2) {
             cell_line<OPK, GEO, R, 2, BIG>(st, h, a, lc, L, W);
             continue;
@@ -990,7 +1034,7 @@ __host__ __device__ inline void voxel_view(const ViewHdr &h, const float4 *__res
     const int nu = a.nu;
     int L, m;
     if (h.xdrv) { L = j; m = i; } else { L = i; m = a.ny - 1 - j; }
-    const float4 lv = lineV[(int64_t)v * a.maxline + L];  // {P0, Delta, zeta, 1/(Delta zeta)}
+    const float4 lv = g_load(lineV + (int64_t)v * a.maxline + L);  // {P0, Delta, zeta, 1/(Delta zeta)}
     const float a0 = fmaf((float)m, lv.y, lv.x);
     const float a1 = a0 + lv.y;
     if (!(a1 > h.dlo && a0 < h.dhi)) return;
@@ -1008,8 +1052,8 @@ __host__ __device__ inline void voxel_view(const ViewHdr &h, const float4 *__res
     // have no overlap and are skipped by the walk below, so the result is exact w.r.t. the
     // FP32 boundaries whatever the seed's rounding.
     int c = (int)clampf(column_seed<GEO>(h, a0), 0.0f, (float)(nu - 1));
-    float4 cc = ct[c], cn = ct[c + 1];
-    while (c > 0 && cc.x > a0) { --c; cn = cc; cc = ct[c]; }
+    float4 cc = g_load(ct + c), cn = g_load(ct + c + 1);
+    while (c > 0 && cc.x > a0) { --c; cn = cc; cc = g_load(ct + c); }
 
     // Slice boundaries are measured from the volume centre, (k - Nz/2) dz + zc: the rows
     // of the slices that reach the detector then come out of small products instead of
@@ -1025,7 +1069,7 @@ __host__ __device__ inline void voxel_view(const ViewHdr &h, const float4 *__res
     while (c < nu && cc.x < a1) {
         // prefetch the next column's table entry one iteration ahead (32-bit byte offset
         // over the view's table: (Nu+1) * 16 bytes)
-        const float4 cnn = *reinterpret_cast<const float4 *>(ctb + (unsigned)imin(c + 2, nu) * 16u);
+        const float4 cnn = g_load(reinterpret_cast<const float4 *>(ctb + (unsigned)imin(c + 2, nu) * 16u));
         const float ou = minf_finite(a1, cn.x) - maxf_finite(a0, cc.x);
         if (ou > 0.0f) {
             // Row coordinate of slice boundary k: rho_k = (zeta z_k / sigma - v0) / dvv.
@@ -1127,10 +1171,10 @@ __device__ inline void voxel_wg(const ViewHdr *__restrict__ hdr, const float4 *_
     for (int q = 0; q < KC; ++q) acc[q] = 0.0f;
     const int ntile = ceil_div(a.nx, kVoxTile) * ceil_div(a.ny, kVoxTile);
     for (int v = v0; v < v1; ++v) {
-        const ViewHdr h = hdr[v];
+        const ViewHdr h = ro_load(hdr + v);
         const float4 *ct = colt + (int64_t)v * (a.nu + 1);
         const float2 *Qv = view_table(Q, a, b, v);
-        const VoxBox bx = voxel_box<KC>(vtile[(int64_t)v * ntile + tile_id], a, k0);
+        const VoxBox bx = voxel_box<KC>(ro_load(vtile + (int64_t)v * ntile + tile_id), a, k0);
         if (bx.skip) continue;  // the same for every thread of the workgroup: no barrier is missed
         voxel_stage(Qv, bx, tile, a.nu, threadIdx.x, kBlock);
         __builtin_amdgcn_s_waitcnt(0);  // this wave's direct-to-LDS loads have landed
