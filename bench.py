@@ -288,7 +288,7 @@ def bench_cfg5(dev, reps):
                        "1152 views, B=4, 32 random ellipsoids each (intensities 0-0.05/mm, seeds 1005-1008)",
            "voxel_views_per_operator": vv, "value": vv / ((t_a + t_b) * 1e-3), "unit": "voxel-views/s",
            "ms_A": t_a, "ms_B": t_b, "ms_AT": t_at, "adjoint_value": vv / ((t_a + t_at) * 1e-3),
-           "tiles": "defaults (8 rows, 16 slices)"}
+           "tiles": "defaults (8 rows; 32 slices in halves of 16)"}
     try:
         nnz = nnz_of(CFG5_WORKLOAD) * NB
     except (KeyError, OSError):

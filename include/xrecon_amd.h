@@ -116,8 +116,9 @@ int xr_plan3d_execute(const xr_plan3d *plan, int op, const float *in, float *out
 #define XR_PHASE_PREFIX 1
 #define XR_PHASE_GATHER 2
 /* voxel_chunk: slices per voxel-gather thread, 1, 4, 8 or 16 (0 = the plan's default);
- * 16 | XR_VOXEL_HALVES runs the 16 slices as two 8-slice halves, each skipped by a wave
- * whose lanes have no row of it on the detector (pays on tall volumes). */
+ * 16 | XR_VOXEL_HALVES runs the 16 slices as two 8-slice halves, 32 | XR_VOXEL_HALVES 32
+ * slices as two 16-slice halves, each half skipped by a wave whose lanes have no row of
+ * it on the detector (pays on tall volumes). */
 #define XR_VOXEL_HALVES 0x100
 int xr_plan3d_execute_phase(const xr_plan3d *plan, int op, int phase, const float *in, float *out,
                             int num_batch, void *workspace, size_t workspace_bytes, int tile_rows,

@@ -101,7 +101,7 @@ void run_voxel_wg_KC(const Plan3d &p, const float2 *Q, float *out, int nb, int n
     std::vector<std::thread> th;
     for (int t = 0; t < nthreads; ++t) {
         th.emplace_back([&, t]() {
-            std::vector<float2> tile(kVoxLds);
+            std::vector<float2> tile(vox_lds(KC));
             std::vector<float> acc((size_t)kBlock * KC);
             for (int it = t; it < items; it += nthreads) {
                 const int bx = it % gx, by = (it / gx) % gy, bz = it / (gx * gy);
@@ -167,7 +167,8 @@ void run_voxel(const Plan3d &p, int KC, const float2 *Q, float *out, int nb, int
     else if (KC == 4) run_voxel_KC<OPK, GEO, 4, false>(p, Q, out, nb, nt);
     else if (KC == 8) run_voxel_KC<OPK, GEO, 8, false>(p, Q, out, nb, nt);
     else if (KC == 16) run_voxel_KC<OPK, GEO, 16, false>(p, Q, out, nb, nt);
-    else run_voxel_KC<OPK, GEO, 16, true>(p, Q, out, nb, nt);
+    else if (KC == (16 | XR_VOXEL_HALVES)) run_voxel_KC<OPK, GEO, 16, true>(p, Q, out, nb, nt);
+    else run_voxel_KC<OPK, GEO, 32, true>(p, Q, out, nb, nt);
 }
 
 template <int GEO>

@@ -214,6 +214,10 @@ def test_voxel_gather_skips_views_off_the_detector(geo):
         # 16 slices as two halves, each passed over where no row of it is on the detector
         assert np.array_equal(emu_op3d(op, y, ang, opts, KC=16 | 0x100), staged), op
         assert np.array_equal(emu_op3d(op, y, ang, opts, KC=16 | 0x100, voxel_variant=1), staged), op
+        # 32 slices as two halves of 16 (chunks start at other slices: equal to tolerance)
+        wide = emu_op3d(op, y, ang, opts, KC=32 | 0x100)
+        assert np.array_equal(wide, emu_op3d(op, y, ang, opts, KC=32 | 0x100, voxel_variant=1)), op
+        assert rel_l2(wide, staged) <= 1e-6, op
 
 
 def test_big_plan_cell_gather_vs_oracle():

@@ -6,8 +6,8 @@ Two knobs exist, one per kernel family (``Plan.kernel_kind(op)``):
   thread.  More rows amortise each line's column overlaps and share row boundaries,
   but cost registers and therefore resident waves.
 * voxel gathers (B, B_W, A^T): ``voxel_chunk`` in {1, 4, 8, 16} -- slices per thread --
-  and 16 | VOXEL_HALVES, 16 slices run as two halves that a wave skips when none of its
-  lanes reaches the detector in them.  Longer chunks amortise each view's column search,
+  and 16 | VOXEL_HALVES, 32 | VOXEL_HALVES: 16 or 32 slices run as two halves that a wave
+  skips when none of its lanes reaches the detector in them.  Longer chunks amortise each view's column search,
   shorter ones give more waves; the halves pay on tall volumes.
 
 ``tune`` times the gather pass alone (the prefix pass is shared) for every candidate
@@ -27,7 +27,7 @@ CELL = 1
 VOXEL = 2
 TILE_ROWS = (1, 2, 4, 8, 16)
 VOXEL_HALVES = 0x100  # XR_VOXEL_HALVES
-VOXEL_CHUNKS = (1, 4, 8, 16, 16 | VOXEL_HALVES)
+VOXEL_CHUNKS = (1, 4, 8, 16, 16 | VOXEL_HALVES, 32 | VOXEL_HALVES)
 
 
 def _table(plan) -> Dict[tuple, Tuple[int, int]]:

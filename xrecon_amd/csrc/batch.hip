@@ -235,10 +235,10 @@ __global__ __launch_bounds__(kBlock) void k_sino_prefix_batch(const BatchDesc *_
 }
 
 template <int OPK, int GEO, int KC, bool HALVES>
-__global__ __launch_bounds__(kBlock) void k_voxel_batch(const BatchDesc *__restrict__ D, int n, BlockStarts st,
+__global__ __launch_bounds__(kBlock, KC >= 32 ? 4 : 1) void k_voxel_batch(const BatchDesc *__restrict__ D, int n, BlockStarts st,
                                                         IOPtrs io, char *__restrict__ ws)
 {
-    __shared__ __attribute__((aligned(16))) float2 tile[kVoxLds];
+    __shared__ __attribute__((aligned(16))) float2 tile[vox_lds(KC)];
     int bid = blockIdx.x;
     const int i = find_instance(st, n, bid);
     if (i < 0) return;
@@ -322,7 +322,8 @@ void launch_voxel_batch(int KC, const BlockStarts &st, const BatchDesc *D, int n
     case 4: hipLaunchKernelGGL((k_voxel_batch<OPK, GEO, 4, false>), grid, dim3(kBlock), 0, s, D, n, st, io, ws); break;
     case 8: hipLaunchKernelGGL((k_voxel_batch<OPK, GEO, 8, false>), grid, dim3(kBlock), 0, s, D, n, st, io, ws); break;
     case 16: hipLaunchKernelGGL((k_voxel_batch<OPK, GEO, 16, false>), grid, dim3(kBlock), 0, s, D, n, st, io, ws); break;
-    default: hipLaunchKernelGGL((k_voxel_batch<OPK, GEO, 16, true>), grid, dim3(kBlock), 0, s, D, n, st, io, ws); break;
+    case 16 | XR_VOXEL_HALVES: hipLaunchKernelGGL((k_voxel_batch<OPK, GEO, 16, true>), grid, dim3(kBlock), 0, s, D, n, st, io, ws); break;
+    default: hipLaunchKernelGGL((k_voxel_batch<OPK, GEO, 32, true>), grid, dim3(kBlock), 0, s, D, n, st, io, ws); break;
     }
 }
 
@@ -474,7 +475,7 @@ int xr_batch3d_execute(const xr_batch3d *b, int op, const float *const *in, floa
         for (const xr::Plan3d *p : b->plans) KC = std::max(KC, xr::default_chunk(*p));
     }
     if (!xr::valid_rows(R)) return fail(XR_EINVAL, "tile_rows must be 1, 2, 4, 8 or 16");
-    if (!xr::valid_chunk(KC)) return fail(XR_EINVAL, "voxel_chunk must be 1, 4, 8, 16 or 16 | XR_VOXEL_HALVES");
+    if (!xr::valid_chunk(KC)) return fail(XR_EINVAL, "voxel_chunk must be 1, 4, 8, 16, 16 | XR_VOXEL_HALVES or 32 | XR_VOXEL_HALVES");
     bool big = false;
     for (const xr::BatchDesc &d : b->desc) big = big || d.big;
     if (big) R = 8;  // tables of 4 GiB and more: the 64-bit general variant with 8-row tiles

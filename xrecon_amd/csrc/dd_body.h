@@ -188,17 +188,20 @@ inline int default_rows(const Plan3d &p)
     return 1;
 }
 
-// Voxel-gather chunk: slices per thread in the low bits, XR_VOXEL_HALVES for 16-slice
-// chunks run as two wave-culled halves.  The halves pay where many chunks straddle the
-// detector's edge rows (tall volumes: config 5's 256 slices, 46.6 -> 29.8 ms for B), and
-// cost where few do (config 2's 64 slices, 2.79 -> 3.03 ms); the autotuner tries both.
+// Voxel-gather chunk: slices per thread in the low bits, XR_VOXEL_HALVES for chunks run
+// as two wave-culled halves (16 slices: halves of 8; 32 slices: halves of 16).  Halves
+// pay where many chunks straddle the detector's edge rows, and 32-slice chunks halve the
+// per-view work per slice (column search, weights, staging) on volumes tall enough to
+// keep the GPU busy: B per item of the 512^2 x 256 cone-arc workload 32.1 ms (16), 28.4
+// (16 in halves), 23.4 (32 in halves); on 256^2 x 64 plain 16-slice chunks win (2.71 /
+// 2.85 / 2.99 ms).  The autotuner tries every form.
 inline int chunk_slices(int k) { return k & 0xff; }
 inline bool chunk_halves(int k) { return (k & XR_VOXEL_HALVES) != 0; }
 
 inline int default_chunk(const Plan3d &p)
 {
     const int nz = p.vol.nz;
-    if (nz >= 128) return 16 | XR_VOXEL_HALVES;
+    if (nz >= 128) return 32 | XR_VOXEL_HALVES;
     if (nz >= 64) return 16;
     if (nz >= 8) return 8;
     if (nz >= 4) return 4;
@@ -206,7 +209,10 @@ inline int default_chunk(const Plan3d &p)
 }
 
 inline bool valid_rows(int r) { return r == 1 || r == 2 || r == 4 || r == 8 || r == 16; }
-inline bool valid_chunk(int k) { return k == 1 || k == 4 || k == 8 || k == 16 || k == (16 | XR_VOXEL_HALVES); }
+inline bool valid_chunk(int k)
+{
+    return k == 1 || k == 4 || k == 8 || k == 16 || k == (16 | XR_VOXEL_HALVES) || k == (32 | XR_VOXEL_HALVES);
+}
 
 inline CellArgs make_cell_args(const Plan3d &p, int nbatch)
 {
@@ -802,10 +808,11 @@ struct VoxBox {
     int c0, nc, r0, nr, pitch, skip;
 };
 
-// Capacity of the staged block, in float2 entries (20 KiB of LDS per workgroup).  The
-// blocks of 16 x 16 voxel columns x 16 slices of a 1 mm volume seen by a 0.5 mm (at the
-// isocentre) detector stay below 2400 entries.
+// Capacity of the staged block, in float2 entries (20 KiB of LDS per workgroup; 40 KiB
+// for chunks of 32 slices).  The blocks of 16 x 16 voxel columns x 16 slices of a 1 mm
+// volume seen by a 0.5 mm (at the isocentre) detector stay below 2400 entries.
 constexpr int kVoxLds = 2560;
+__host__ __device__ constexpr int vox_lds(int KC) { return KC >= 32 ? 2 * kVoxLds : kVoxLds; }
 
 // VoxTile for voxel columns [i0, i1] x [j0, j1] on view v, from the tile's corner pixels:
 // the axis map of a pixel edge is monotone in the pixel and in the line (a projective
@@ -874,8 +881,9 @@ __host__ __device__ inline VoxBox voxel_box(const VoxTile &t, const VoxArgs &a, 
     const int r1 = imin((int)floorf(hi) + 1, a.nv);
     const int pitch = (t.nc + 1) & ~1;
     int nr = r1 - r0 + 1;
-    if (nr <= 0 || pitch > kVoxLds) return bx;
-    if (pitch * nr > kVoxLds) nr = kVoxLds / pitch;
+    constexpr int cap = vox_lds(KC);
+    if (nr <= 0 || pitch > cap) return bx;
+    if (pitch * nr > cap) nr = cap / pitch;
     bx.c0 = t.c0;
     bx.nc = t.nc;
     bx.r0 = r0;
@@ -886,7 +894,7 @@ __host__ __device__ inline VoxBox voxel_box(const VoxTile &t, const VoxArgs &a, 
 
 // Thread tid's share of copying the block from view v's row-prefix table Qv into LDS, in
 // 16-byte pairs of adjacent columns: pair p = tid + s * nthreads goes to lds[2p, 2p+1]
-// (at most kVoxLds / 2 pairs, so s < kVoxLds / 2 / nthreads).  A pair may end one entry
+// (at most vox_lds(KC) / 2 pairs, so s < vox_lds(KC) / 2 / nthreads).  A pair may end one entry
 // past the view's last column (the next row's first entry, or the pad after the table);
 // that entry is never looked up.
 //
@@ -903,7 +911,7 @@ __host__ __device__ inline void voxel_stage(const float2 *__restrict__ Qv, const
     const char *Qb = reinterpret_cast<const char *>(Qv);
     const unsigned nu8 = (unsigned)nu * 8u;
     // (row, pair) of p = tid, advanced by nthreads = dq rows + dc pairs per step.  The
-    // first division: (tid + 1/2) / half stays >= 1/(2 half) >= 1/2560 from an integer,
+    // first division: (tid + 1/2) / half stays >= 1/(2 half) >= 1/5120 from an integer,
     // far above the error of the approximate reciprocal and product (tid < 2^10:
     // below 2^10 * 2^-22).
     int r = (int)(((float)tid + 0.5f) * approx_rcp((float)half));
@@ -1030,6 +1038,7 @@ __host__ __device__ inline void voxel_view(const ViewHdr &h, const float4 *__res
                                            const VoxArgs &a, int v, int i, int j, int k0, float px, float py,
                                            float *acc, const VoxBox &bx, const float2 *__restrict__ lds)
 {
+    static_assert(KC <= 16 || HALVES, "chunks beyond 16 slices run in halves");
     const float fnv = (float)a.nv;
     const int nu = a.nu;
     int L, m;

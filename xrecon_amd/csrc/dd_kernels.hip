@@ -271,13 +271,15 @@ __global__ __launch_bounds__(kBlock) void k_cell(const ViewHdr *__restrict__ hdr
     else cell_body<OPK, GEO, R, 0, false>(hdr, lineC, colt, WX, WY, out, a, c, r0, v, b, wgrp);
 }
 
+// 32-slice chunks: at most 128 registers (4 waves per SIMD); uncapped they take 133 and
+// run at 3 waves, 24 % slower on the 512^2 x 256 cone-arc workload.
 template <int OPK, int GEO, int KC, bool HALVES>
-__global__ __launch_bounds__(kBlock) void k_voxel(const ViewHdr *__restrict__ hdr, const float4 *__restrict__ lineV,
+__global__ __launch_bounds__(kBlock, KC >= 32 ? 4 : 1) void k_voxel(const ViewHdr *__restrict__ hdr, const float4 *__restrict__ lineV,
                                                   const float4 *__restrict__ colt, const VoxTile *__restrict__ vtile,
                                                   const float2 *__restrict__ Q, float *__restrict__ out,
                                                   float *__restrict__ part, VoxArgs a)
 {
-    __shared__ __attribute__((aligned(16))) float2 tile[kVoxLds];
+    __shared__ __attribute__((aligned(16))) float2 tile[vox_lds(KC)];
     int i, j;
     voxel_coords(blockIdx.x, blockIdx.y, threadIdx.x, i, j);
     const int set = blockIdx.z % a.vsplit, rest = blockIdx.z / a.vsplit;
@@ -366,7 +368,8 @@ void launch_voxel(const Plan3d &p, int KC, const float2 *Q, float *out, float *p
     case 4: hipLaunchKernelGGL((k_voxel<OPK, GEO, 4, false>), grid, dim3(kBlock), 0, s, p.d_hdr, p.d_lineV, p.d_col, p.d_vtile, Q, out, part, a); break;
     case 8: hipLaunchKernelGGL((k_voxel<OPK, GEO, 8, false>), grid, dim3(kBlock), 0, s, p.d_hdr, p.d_lineV, p.d_col, p.d_vtile, Q, out, part, a); break;
     case 16: hipLaunchKernelGGL((k_voxel<OPK, GEO, 16, false>), grid, dim3(kBlock), 0, s, p.d_hdr, p.d_lineV, p.d_col, p.d_vtile, Q, out, part, a); break;
-    default: hipLaunchKernelGGL((k_voxel<OPK, GEO, 16, true>), grid, dim3(kBlock), 0, s, p.d_hdr, p.d_lineV, p.d_col, p.d_vtile, Q, out, part, a); break;
+    case 16 | XR_VOXEL_HALVES: hipLaunchKernelGGL((k_voxel<OPK, GEO, 16, true>), grid, dim3(kBlock), 0, s, p.d_hdr, p.d_lineV, p.d_col, p.d_vtile, Q, out, part, a); break;
+    default: hipLaunchKernelGGL((k_voxel<OPK, GEO, 32, true>), grid, dim3(kBlock), 0, s, p.d_hdr, p.d_lineV, p.d_col, p.d_vtile, Q, out, part, a); break;
     }
 }
 
@@ -457,7 +460,7 @@ int execute_fdk(const Plan3d &p, const float *in, float *out, int nbatch, const 
     if (ws_bytes < need || !ws)
         return fail(XR_EWORKSPACE, "workspace of " + std::to_string(ws_bytes) + " bytes, need " + std::to_string(need));
     if (voxel_chunk == 0) voxel_chunk = default_chunk(p);
-    if (!valid_chunk(voxel_chunk)) return fail(XR_EINVAL, "voxel_chunk must be 1, 4, 8, 16 or 16 | XR_VOXEL_HALVES");
+    if (!valid_chunk(voxel_chunk)) return fail(XR_EINVAL, "voxel_chunk must be 1, 4, 8, 16, 16 | XR_VOXEL_HALVES or 32 | XR_VOXEL_HALVES");
     float2 *Q = static_cast<float2 *>(ws);
     // the taps live where the view-split partial volumes go later (workspace_bytes
     // reserves room for both); the gather runs after the filter stage on the same stream
@@ -527,7 +530,7 @@ int execute_phase(const Plan3d &p, int op, int phase, const float *in, float *ou
     if (tile_rows == 0) tile_rows = default_rows(p);
     if (voxel_chunk == 0) voxel_chunk = default_chunk(p);
     if (!valid_rows(tile_rows)) return fail(XR_EINVAL, "tile_rows must be 1, 2, 4, 8 or 16");
-    if (!valid_chunk(voxel_chunk)) return fail(XR_EINVAL, "voxel_chunk must be 1, 4, 8, 16 or 16 | XR_VOXEL_HALVES");
+    if (!valid_chunk(voxel_chunk)) return fail(XR_EINVAL, "voxel_chunk must be 1, 4, 8, 16, 16 | XR_VOXEL_HALVES or 32 | XR_VOXEL_HALVES");
 
     if (rt.cell && cell_blocks(make_cell_args(p, nbatch), 1, p.ncls[0] + p.ncls[1] + p.ncls[2], nbatch) >=
                        ((int64_t)1 << 31))
