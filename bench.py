@@ -567,7 +567,9 @@ def main():
             symbol = [f"k_cell<0, 1, {r}, {span}, false>" for r, span, n in zip(rows, (2, 3, 0), ncls) if n > 0]
         dominant, t_dom = "cell_gather(A)", t_cell
     else:
-        dominant, t_dom, symbol = "voxel_gather(B)", t_vox, f"k_voxel<0, 1, {tune_b[1]}>"
+        kc = tune_b[1]
+        dominant, t_dom = "voxel_gather(B)", t_vox
+        symbol = f"k_voxel<0, 1, {kc & 0xff}, {'true' if kc & 0x100 else 'false'}>"
     achieved = flops / (t_dom * 1e-3) / 1e12
     traffic, traffic_src = traffic_from_profiles(symbol)
 
