@@ -61,22 +61,46 @@ __host__ __device__ inline T ro_load(const T *p)
 #endif
 }
 
-// A per-lane read of a plan table in global memory (the same word-by-word copy, through
-// the global address space): vector loads, never generic ones, in the batched kernels.
+// A per-lane read of a plan table in global memory, through the global address space:
+// vector loads, never generic ones, in the batched kernels.  16-byte entries (aligned in
+// their tables) are read as one 16-byte vector, so that the load stays one instruction
+// whichever of its words the caller uses.
 template <class T>
 __host__ __device__ inline T g_load(const T *p)
 {
 #if defined(__HIP_DEVICE_COMPILE__)
-    static_assert(sizeof(T) % 4 == 0, "tables of 32-bit words");
-    const __attribute__((address_space(1))) unsigned *q = (const __attribute__((address_space(1))) unsigned *)p;
-    unsigned w[sizeof(T) / 4];
-#pragma unroll
-    for (unsigned k = 0; k < sizeof(T) / 4; ++k) w[k] = q[k];
+    typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
     T r;
-    __builtin_memcpy(&r, w, sizeof(T));
+    if constexpr (sizeof(T) % 16 == 0) {
+        const __attribute__((address_space(1))) u32x4 *q = (const __attribute__((address_space(1))) u32x4 *)p;
+        u32x4 w[sizeof(T) / 16];
+#pragma unroll
+        for (unsigned k = 0; k < sizeof(T) / 16; ++k) w[k] = q[k];
+        __builtin_memcpy(&r, w, sizeof(T));
+    } else {
+        static_assert(sizeof(T) % 4 == 0, "tables of 32-bit words");
+        const __attribute__((address_space(1))) unsigned *q = (const __attribute__((address_space(1))) unsigned *)p;
+        unsigned w[sizeof(T) / 4];
+#pragma unroll
+        for (unsigned k = 0; k < sizeof(T) / 4; ++k) w[k] = q[k];
+        __builtin_memcpy(&r, w, sizeof(T));
+    }
     return r;
 #else
     return *p;
+#endif
+}
+
+// The first three words {d, sigma, kappa} of a column-table entry (w = 0): what the voxel
+// gathers use of it, as one 12-byte load.
+__host__ __device__ inline float4 col_load3(const float4 *p)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    typedef float f32x3 __attribute__((ext_vector_type(3)));
+    const f32x3 v = *(const __attribute__((address_space(1))) f32x3 *)p;
+    return make_float4(v.x, v.y, v.z, 0.0f);
+#else
+    return make_float4(p->x, p->y, p->z, 0.0f);
 #endif
 }
 
@@ -1061,8 +1085,8 @@ __host__ __device__ inline void voxel_view(const ViewHdr &h, const float4 *__res
     // have no overlap and are skipped by the walk below, so the result is exact w.r.t. the
     // FP32 boundaries whatever the seed's rounding.
     int c = (int)clampf(column_seed<GEO>(h, a0), 0.0f, (float)(nu - 1));
-    float4 cc = g_load(ct + c), cn = g_load(ct + c + 1);
-    while (c > 0 && cc.x > a0) { --c; cn = cc; cc = g_load(ct + c); }
+    float4 cc = col_load3(ct + c), cn = col_load3(ct + c + 1);
+    while (c > 0 && cc.x > a0) { --c; cn = cc; cc = col_load3(ct + c); }
 
     // Slice boundaries are measured from the volume centre, (k - Nz/2) dz + zc: the rows
     // of the slices that reach the detector then come out of small products instead of
@@ -1078,7 +1102,7 @@ __host__ __device__ inline void voxel_view(const ViewHdr &h, const float4 *__res
     while (c < nu && cc.x < a1) {
         // prefetch the next column's table entry one iteration ahead (32-bit byte offset
         // over the view's table: (Nu+1) * 16 bytes)
-        const float4 cnn = g_load(reinterpret_cast<const float4 *>(ctb + (unsigned)imin(c + 2, nu) * 16u));
+        const float4 cnn = col_load3(reinterpret_cast<const float4 *>(ctb + (unsigned)imin(c + 2, nu) * 16u));
         const float ou = minf_finite(a1, cn.x) - maxf_finite(a0, cc.x);
         if (ou > 0.0f) {
             // Row coordinate of slice boundary k: rho_k = (zeta z_k / sigma - v0) / dvv.
