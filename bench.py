@@ -380,6 +380,31 @@ def bench_cfg5_unrolled(dev, iterations=10):
     return out
 
 
+# Sources that decide the single-plan gather kernels' code and tables: a traffic figure
+# measured on other sources is reported as not current.
+KERNEL_SOURCES = ("xrecon_amd/csrc/dd_body.h", "xrecon_amd/csrc/dd_kernels.hip", "xrecon_amd/csrc/xr_internal.h",
+                  "xrecon_amd/csrc/plan.cpp")
+
+
+def kernel_sources_digest():
+    import hashlib
+
+    h = hashlib.sha256()
+    for rel in KERNEL_SOURCES:
+        with open(os.path.join(ROOT, rel), "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()[:16]
+
+
+def profile_digest(path):
+    """The kernel-sources digest a profile summary was taken with ("# kernel sources: D")."""
+    with open(path) as f:
+        for line in f:
+            if line.startswith("# kernel sources:"):
+                return line.split(":", 1)[1].strip()
+    return None
+
+
 def traffic_from_profiles(kernel):
     """HBM bytes per launch of `kernel` (a symbol, or a list of symbols whose launches
     together make one pass: their bytes add) from the committed rocprofv3 summaries: a
@@ -572,6 +597,11 @@ def main():
         symbol = f"k_voxel<0, 1, {kc & 0xff}, {'true' if kc & 0x100 else 'false'}>"
     achieved = flops / (t_dom * 1e-3) / 1e12
     traffic, traffic_src = traffic_from_profiles(symbol)
+    # the summary's kernels are these only if it was taken on the same kernel sources
+    traffic_current = None
+    if traffic_src:
+        traffic_current = all(profile_digest(os.path.join(ROOT, src.strip())) == kernel_sources_digest()
+                              for src in traffic_src.split(","))
 
     cpu = None
     if not args.no_cpu:
@@ -611,7 +641,7 @@ def main():
         "roofline": {"bound": "valu", "kernel": dominant, "symbol": symbol, "achieved": achieved,
                      "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS,
                      "peak_nonpacked": FP32_PEAK_TFLOPS / 2, "frac_nonpacked": achieved / (FP32_PEAK_TFLOPS / 2),
-                     "traffic": traffic, "traffic_source": traffic_src,
+                     "traffic": traffic, "traffic_source": traffic_src, "traffic_current": traffic_current,
                      "hbm_frac": None if traffic is None else traffic / (t_dom * 1e-3) / 8e12,
                      "compulsory_bytes": compulsory,
                      "compulsory_hbm_frac": compulsory / (t_dom * 1e-3) / 8e12,

@@ -71,7 +71,8 @@ def test_small_vs_reference_cpu(cuda, geo, op):
     assert rel_l2(got, ref) <= tol
 
 
-@pytest.mark.parametrize("tile_rows,voxel_chunk", [(1, 1), (2, 4), (4, 8), (8, 16), (16, 16)])
+@pytest.mark.parametrize("tile_rows,voxel_chunk", [(1, 1), (2, 4), (4, 8), (8, 16), (16, 16), (16, 16 | 0x100),
+                                                   (16, 32 | 0x100)])
 def test_tilings_agree(cuda, tile_rows, voxel_chunk):
     C = _native()
     opts = CASES3D["flat"]
