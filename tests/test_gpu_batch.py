@@ -29,18 +29,23 @@ def _instances(scan="flat"):
     return projs, angs, [s[4] for s in SHAPES]
 
 
+# (cell rows, voxel chunk): plain tiles, and 16 / 32 slices in wave-culled halves
+TILES = [(4, 4), (16, 16 | 0x100), (8, 32 | 0x100)]
+
+
 @pytest.mark.parametrize("scan", ["flat", "arc"])
 @pytest.mark.parametrize("op", [be.OP_PROJECTION, be.OP_BACKPROJECTION, be.OP_PROJECTION_T,
                                 be.OP_BACKPROJECTION_W, be.OP_BACKPROJECTION_T])
-def test_batched_equals_per_instance(cuda, scan, op):
+@pytest.mark.parametrize("tile", range(len(TILES)))
+def test_batched_equals_per_instance(cuda, scan, op, tile):
     projs, angs, nbs = _instances(scan)
     bp = BatchedProjector3d(projs, angs, nbs, device=cuda)
     gen = torch.Generator(device=cuda).manual_seed(op)
     cell = bp.plans[0].kernel_kind(op) == 1
     inputs = [torch.rand(bp.volume_shape(i) if cell else bp.sino_shape(i), device=cuda, generator=gen)
               for i in range(len(projs))]
-    tiles = (4, 0) if cell else (0, 4)
-    print(f"[gpu] batch {scan} op {op}", flush=True)
+    tiles = (TILES[tile][0], 0) if cell else (0, TILES[tile][1])
+    print(f"[gpu] batch {scan} op {op} tiles {tiles}", flush=True)
     outs = bp.run(op, inputs, tiles=tiles)
     s = torch.cuda.current_stream().cuda_stream
     for i, plan in enumerate(bp.plans):
