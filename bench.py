@@ -439,6 +439,32 @@ def traffic_from_profiles(kernel):
     return None, None
 
 
+def pipes_from_profiles(kernels):
+    """VALU issue, TA busy and L2 hit rate of each kernel from the committed pipe-counter
+    summary (lines "pipe <kernel> valu_issue=V ta_busy=T l2_hit=H"), with whether that
+    summary was taken on the current kernel sources; None when one is missing."""
+    pdir = os.path.join(ROOT, "profiles")
+    found, current = {}, True
+    if not os.path.isdir(pdir):
+        return None
+    for name in sorted(os.listdir(pdir)):
+        if not name.endswith(".txt"):
+            continue
+        path = os.path.join(pdir, name)
+        with open(path) as f:
+            for line in f:
+                if not line.startswith("pipe "):
+                    continue
+                head, _, rest = line[len("pipe "):].partition(" valu_issue=")
+                if head.strip() in kernels:
+                    found[head.strip()] = {k: float(v) for k, v in
+                                           (kv.split("=", 1) for kv in ("valu_issue=" + rest).split())}
+                    current = current and profile_digest(path) == kernel_sources_digest()
+    if any(k not in found for k in kernels):
+        return None
+    return {"kernels": found, "current": current}
+
+
 def calibrate_copy():
     import torch
 
@@ -642,6 +668,7 @@ def main():
                      "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS,
                      "peak_nonpacked": FP32_PEAK_TFLOPS / 2, "frac_nonpacked": achieved / (FP32_PEAK_TFLOPS / 2),
                      "traffic": traffic, "traffic_source": traffic_src, "traffic_current": traffic_current,
+                     "pipes": pipes_from_profiles(symbol if isinstance(symbol, list) else [symbol]),
                      "hbm_frac": None if traffic is None else traffic / (t_dom * 1e-3) / 8e12,
                      "compulsory_bytes": compulsory,
                      "compulsory_hbm_frac": compulsory / (t_dom * 1e-3) / 8e12,
