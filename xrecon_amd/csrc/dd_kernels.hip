@@ -223,7 +223,8 @@ __global__ __launch_bounds__(kBlock) void k_fdk_prefix(const ViewHdr *__restrict
 }
 
 // Cell gather over the column groups of one span class: a workgroup is 4 waves on 4
-// consecutive listed groups (view v, 64 columns each), rows r0 .. r0+R-1 of one row tile,
+// consecutive listed groups (64 columns each; one column group on adjacent views, as the
+// lists run), rows r0 .. r0+R-1 of one row tile,
 // one batch item; blockIdx.x = (batch item, group quad, row tile), row tile fastest
 // (a.band: (batch item, row tile, group quad), see kCellBandBytes).
 // Every wave of a launch but the last quad's has work whatever Nv is.  SPAN = the class (2, 3 or 0), so each

@@ -291,13 +291,18 @@ int build_tables(Plan3d &p, const float *angles, PlanTables &t, std::string &err
         }
 
     // The cell gather launches one grid per span class over that class's groups, so that
-    // the narrow classes run with the registers of their own path only.
+    // the narrow classes run with the registers of their own path only.  Within a class
+    // the groups are listed column group by column group, views in order: a cell
+    // workgroup takes 4 consecutive entries, i.e. one group of columns on (mostly) 4
+    // adjacent views, which read nearly the same z-prefix entries (512^2 x 256 cone-arc
+    // A per item 40.1 -> 35.7 ms against view-major lists, where the 4 waves took 4
+    // neighbouring groups of one view; the 256^2 x 64 workload is unchanged).
     t.citems.clear();
     const int classes[3] = {2, 3, 0};
     for (int k = 0; k < 3; ++k) {
         t.ncls[k] = 0;
-        for (int v = 0; v < V; ++v)
-            for (int g = 0; g < ncg; ++g)
+        for (int g = 0; g < ncg; ++g)
+            for (int v = 0; v < V; ++v)
                 if (t.cgrp[(size_t)v * ncg + g].span == classes[k]) {
                     t.citems.push_back(v * ncg + g);
                     ++t.ncls[k];

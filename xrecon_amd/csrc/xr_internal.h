@@ -97,7 +97,8 @@ struct Plan3d {
     float4 *d_col = nullptr;    // [nview][Nu+1]    {d, sigma, kappa, W}
     VoxTile *d_vtile = nullptr; // [nview][ceil(Ny/16)][ceil(Nx/16)]: voxel-gather staging estimates
     CellGroup *d_cgrp = nullptr; // [nview][ceil(Nu/64)]: cell-gather column groups
-    int *d_citems = nullptr;     // the groups v * ceil(Nu/64) + g listed by span class: 2, then 3, then 0
+    int *d_citems = nullptr;     // the groups v * ceil(Nu/64) + g listed by span class (2, then 3, then 0),
+                                 // within a class by g, then v
     int ncls[3] = {0, 0, 0};     // number of groups of span class 2, 3 and 0
 };
 
