@@ -51,6 +51,8 @@ def parse():
     ap.add_argument("--verbose", action="store_true", help="print autotuner timings")
     ap.add_argument("--no-batched", action="store_true", help="skip the 16-instance batched section")
     ap.add_argument("--no-configs", action="store_true", help="skip the 2D fan-flat and cone-arc sections")
+    ap.add_argument("--cone-arc-gathers", action="store_true",
+                    help="only run A, B and A^T of the cone-arc 512^2x256 workload twice (for counter passes)")
     return ap.parse_args()
 
 
@@ -518,6 +520,11 @@ def main():
         return
     if args.summarize:
         summarize_rocpd(args.summarize)
+        return
+    if args.cone_arc_gathers:
+        import torch
+
+        print(json.dumps(bench_cfg5(torch.device("cuda:0"), 1)))
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build runs on a single MI355X and has no multi-GPU path "
