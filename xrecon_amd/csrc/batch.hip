@@ -323,7 +323,8 @@ void launch_voxel_batch(int KC, const BlockStarts &st, const BatchDesc *D, int n
     case 8: hipLaunchKernelGGL((k_voxel_batch<OPK, GEO, 8, false>), grid, dim3(kBlock), 0, s, D, n, st, io, ws); break;
     case 16: hipLaunchKernelGGL((k_voxel_batch<OPK, GEO, 16, false>), grid, dim3(kBlock), 0, s, D, n, st, io, ws); break;
     case 16 | XR_VOXEL_HALVES: hipLaunchKernelGGL((k_voxel_batch<OPK, GEO, 16, true>), grid, dim3(kBlock), 0, s, D, n, st, io, ws); break;
-    default: hipLaunchKernelGGL((k_voxel_batch<OPK, GEO, 32, true>), grid, dim3(kBlock), 0, s, D, n, st, io, ws); break;
+    default:  // 32 | XR_VOXEL_HALVES: valid_chunk admits nothing else
+        hipLaunchKernelGGL((k_voxel_batch<OPK, GEO, 32, true>), grid, dim3(kBlock), 0, s, D, n, st, io, ws); break;
     }
 }
 

@@ -370,7 +370,8 @@ void launch_voxel(const Plan3d &p, int KC, const float2 *Q, float *out, float *p
     case 8: hipLaunchKernelGGL((k_voxel<OPK, GEO, 8, false>), grid, dim3(kBlock), 0, s, p.d_hdr, p.d_lineV, p.d_col, p.d_vtile, Q, out, part, a); break;
     case 16: hipLaunchKernelGGL((k_voxel<OPK, GEO, 16, false>), grid, dim3(kBlock), 0, s, p.d_hdr, p.d_lineV, p.d_col, p.d_vtile, Q, out, part, a); break;
     case 16 | XR_VOXEL_HALVES: hipLaunchKernelGGL((k_voxel<OPK, GEO, 16, true>), grid, dim3(kBlock), 0, s, p.d_hdr, p.d_lineV, p.d_col, p.d_vtile, Q, out, part, a); break;
-    default: hipLaunchKernelGGL((k_voxel<OPK, GEO, 32, true>), grid, dim3(kBlock), 0, s, p.d_hdr, p.d_lineV, p.d_col, p.d_vtile, Q, out, part, a); break;
+    default:  // 32 | XR_VOXEL_HALVES: valid_chunk admits nothing else
+        hipLaunchKernelGGL((k_voxel<OPK, GEO, 32, true>), grid, dim3(kBlock), 0, s, p.d_hdr, p.d_lineV, p.d_col, p.d_vtile, Q, out, part, a); break;
     }
 }
 
