@@ -389,12 +389,17 @@ KERNEL_SOURCES = ("xrecon_amd/csrc/dd_body.h", "xrecon_amd/csrc/dd_kernels.hip",
 
 
 def kernel_sources_digest():
+    """Digest of the code of KERNEL_SOURCES, comments and layout left out (a comment edit
+    does not make a measurement stale)."""
     import hashlib
+    import re
 
     h = hashlib.sha256()
     for rel in KERNEL_SOURCES:
-        with open(os.path.join(ROOT, rel), "rb") as f:
-            h.update(f.read())
+        with open(os.path.join(ROOT, rel)) as f:
+            text = re.sub(r"/\*.*?\*/", " ", f.read(), flags=re.S)
+        text = re.sub(r"//[^\n]*", " ", text)
+        h.update(" ".join(text.split()).encode())
     return h.hexdigest()[:16]
 
 
