@@ -323,3 +323,33 @@ def test_big_plans_vs_oracle(cuda, case):
         err = rel_l2(got, want)
         print(f"[gpu] big plan ({case}) {op}: rel L2 {err:.2e}", flush=True)
         assert err <= 1e-5, op
+
+
+def test_invalid_tile_shapes_are_refused(cuda):
+    """Tile shapes outside the supported set fail with the library's message before any
+    kernel runs, for a plan and for a batch (which takes no paired or unknown chunks)."""
+    from xrecon_amd.batched import BatchedProjector3d
+    import xrecon_amd
+
+    C = _native()
+    opts = CASES3D["flat"]
+    V = 8
+    ang = angles_with_ties(V)
+    plan = C.Plan([float(v) for v in opts], torch.from_numpy(ang), 0, False)
+    vol, sino = shapes3d(opts, 1, V)
+    y = torch.rand(sino, device=cuda).unsqueeze(1)
+    x = torch.rand(vol, device=cuda).unsqueeze(1)
+    s = torch.cuda.current_stream().cuda_stream
+    out_v = torch.empty(plan.execute(2, y, s).shape, device=cuda)
+    out_s = torch.empty(plan.execute(0, x, s).shape, device=cuda)
+    ws = torch.empty(max(plan.workspace_bytes(2, 1), plan.workspace_bytes(0, 1)), dtype=torch.uint8, device=cuda)
+    for chunk in (2, 3, 32, 64, 8 | 0x100, 16 | 0x200, 0x100):
+        with pytest.raises(RuntimeError, match="voxel_chunk must be"):
+            plan.execute_into(2, y, out_v, ws, s, 0, chunk, 0)
+    for rows in (3, 32):
+        with pytest.raises(RuntimeError, match="tile_rows must be"):
+            plan.execute_into(0, x, out_s, ws, s, rows, 0, 0)
+    proj = xrecon_amd.Projector3d((24, 20, 8), (40, 8), 1.0, 1.0, 500.0, 1000.0)
+    bp = BatchedProjector3d([proj], [torch.from_numpy(uniform_angles(9))], device=cuda)
+    with pytest.raises(RuntimeError, match="voxel_chunk must be"):
+        bp.run(2, [torch.rand(bp.sino_shape(0), device=cuda)], tiles=(0, 5))
