@@ -290,7 +290,7 @@ def bench_cfg5(dev, reps):
                        "1152 views, B=4, 32 random ellipsoids each (intensities 0-0.05/mm, seeds 1005-1008)",
            "voxel_views_per_operator": vv, "value": vv / ((t_a + t_b) * 1e-3), "unit": "voxel-views/s",
            "ms_A": t_a, "ms_B": t_b, "ms_AT": t_at, "adjoint_value": vv / ((t_a + t_at) * 1e-3),
-           "tiles": "defaults (8 rows; 32 slices in halves of 16)"}
+           "tiles": "defaults (8 rows, LDS-staged cell gather; 32 slices in halves of 16)"}
     try:
         nnz = nnz_of(CFG5_WORKLOAD) * NB
     except (KeyError, OSError):
@@ -623,11 +623,16 @@ def main():
         # one launch per span class of the column groups (two-pixel at the tuned rows,
         # three-pixel and general at most 8 rows)
         ncls = plan.cell_classes()
-        rows = (tune_a[0], min(tune_a[0], 8), min(tune_a[0], 8))
-        if -(-sum(ncls) // 4) * -(-wl["Nv"] // tune_a[0]) < 4096:  # small grids: one launch for all classes
-            symbol = [f"k_cell<0, 1, {tune_a[0]}, -1, false>"]
+        r_a = tune_a[0] & 0xFF
+        rows = (r_a, min(r_a, 8), min(r_a, 8))
+        # LDS-staged classes 2 and 3 at 4 rows and more: as tuned (this table is below the
+        # 128 MiB the library stages by default)
+        staged = bool(tune_a[0] & autotune.CELL_STAGED)
+        if -(-sum(ncls) // 4) * -(-wl["Nv"] // r_a) < 4096:  # small grids: one launch for all classes
+            symbol = [f"k_cell<0, 1, {r_a}, -1, false, false>"]
         else:
-            symbol = [f"k_cell<0, 1, {r}, {span}, false>" for r, span, n in zip(rows, (2, 3, 0), ncls) if n > 0]
+            symbol = [f"k_cell<0, 1, {r}, {span}, false, {'true' if staged and span and r >= 4 else 'false'}>"
+                      for r, span, n in zip(rows, (2, 3, 0), ncls) if n > 0]
         dominant, t_dom = "cell_gather(A)", t_cell
     else:
         kc = tune_b[1]
@@ -687,7 +692,8 @@ def main():
                      "note": "FP32 vector-ALU roof (packed FP32 peak); no dense contraction, so no MFMA; "
                              "F = 4*nnz per launch; traffic = HBM bytes per launch (rocprofv3 FETCH_SIZE x2.0 "
                              "+ WRITE_SIZE x1.0, copy-kernel calibrated)"},
-        "tiles": {"A_tile_rows": tune_a[0], "B_voxel_chunk": tune_b[1]},
+        "tiles": {"A_tile_rows": tune_a[0] & 0xFF, "A_cell_staged": bool(tune_a[0] & autotune.CELL_STAGED),
+                  "B_voxel_chunk": tune_b[1]},
         "kernels_ms": {"cell_gather_A": t_cell, "voxel_gather_B": t_vox,
                        "prefix_passes": ms_step - t_cell - t_vox},
         "adjoint": {"ms_per_pair": ms_adj, "value": vox_views / (ms_adj * 1e-3), "unit": "voxel-views/s",

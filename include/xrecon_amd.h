@@ -120,6 +120,15 @@ int xr_plan3d_execute(const xr_plan3d *plan, int op, const float *in, float *out
  * slices as two 16-slice halves, each half skipped by a wave whose lanes have no row of
  * it on the detector (pays on tall volumes). */
 #define XR_VOXEL_HALVES 0x100
+/* tile_rows: detector rows per cell-gather thread, 1, 2, 4, 8 or 16 (0 = the plan's
+ * default).  With 4 rows and more a cell gather may stage, per line, the block of the
+ * z-prefix table each workgroup looks up in LDS (three lines ahead of its use) instead of
+ * reading every lookup from device memory; the results are the same bit for bit.  By
+ * default it stages where the z-prefix table of one batch item exceeds 128 MiB (its
+ * lookups then miss the last-level caches most); tile_rows | XR_CELL_DIRECT never
+ * stages, tile_rows | XR_CELL_STAGED always does (the autotuner tries both). */
+#define XR_CELL_DIRECT 0x100
+#define XR_CELL_STAGED 0x200
 int xr_plan3d_execute_phase(const xr_plan3d *plan, int op, int phase, const float *in, float *out,
                             int num_batch, void *workspace, size_t workspace_bytes, int tile_rows,
                             int voxel_chunk, void *stream);
@@ -172,8 +181,9 @@ void xr_batch3d_destroy(xr_batch3d *batch);
 /* Workspace (device bytes) that xr_batch3d_execute needs for `op`. */
 size_t xr_batch3d_workspace_bytes(const xr_batch3d *batch, int op);
 /* in[i] / out[i]: instance i's device buffers in the layouts above.  `tile_rows`
- * (1, 2, 4, 8, 16; 0 = default) and `voxel_chunk` (1, 4, 8, 16; 0 = default) apply to all
- * instances of the launch. */
+ * and `voxel_chunk` (values as for xr_plan3d_execute_phase; 0 = default) apply to all
+ * instances of the launch; the batched cell gather reads its lookups from device memory
+ * (XR_CELL_DIRECT and XR_CELL_STAGED are accepted and change nothing). */
 int xr_batch3d_execute(const xr_batch3d *batch, int op, const float *const *in, float *const *out,
                        void *workspace, size_t workspace_bytes, int tile_rows, int voxel_chunk,
                        void *stream);

@@ -107,9 +107,10 @@ def _p(a):
 
 
 def emu_op3d(op, x, angles, opts, R=0, KC=0, is2d=False, nthreads=None, cell_variant=0, voxel_variant=0):
-    """The product's kernel bodies run on the CPU (float32 in/out).  cell_variant: 0 = the
-    cell-gather variant the library launches, 1 = general path with 32-bit offsets,
-    2 = general path with 64-bit offsets.  voxel_variant: 0 = the workgroup path with the
+    """The product's kernel bodies run on the CPU (float32 in/out).  cell_variant: 0 = each
+    thread reading the table (the direct launch), 1 = general path with 32-bit offsets,
+    2 = general path with 64-bit offsets, 3 = the LDS-staged workgroups (tiles of 4 rows
+    and more).  voxel_variant: 0 = the workgroup path with the
     per-view block staged (as launched), 1 = every lookup from the table."""
     lib = _load_emu()
     lib.emu_set_cell_variant(int(cell_variant))
@@ -128,6 +129,33 @@ def emu_op3d(op, x, angles, opts, R=0, KC=0, is2d=False, nthreads=None, cell_var
     if rc != 0:
         raise RuntimeError(lib.emu_error().decode())
     return out
+
+
+def emu_cell_lines(reset=True):
+    """The staged cell gather's counts since the last reset (emulator variant 3):
+    (thread, line) pairs served from an LDS block, lookups outside their block (must be
+    0), workgroups staged, workgroups direct."""
+    lib = _load_emu()
+    counts = np.zeros(4, dtype=np.int64)
+    lib.emu_cell_lines(counts.ctypes.data_as(ctypes.c_void_p), int(reset))
+    return counts
+
+
+def emu_cell_box_stats(angles, opts, R2, R3):
+    """Block statistics of the staged cell gather over a plan (tiles of R2 rows for the
+    two-pixel class, R3 for the three-pixel one): [blocks, blocks over a slot, workgroups
+    staged, workgroups direct, entries staged, largest block, blocks over 1024, over 1360,
+    workgroups with a block over 1024 / 1360 / 1536, (workgroup, row tile) pairs, pairs
+    with a block over a slot]."""
+    lib = _load_emu()
+    vn, vf, gn, gf = oracle.split3d(opts)
+    angles = np.ascontiguousarray(angles, dtype=np.float32)
+    st = np.zeros(13, dtype=np.int64)
+    rc = lib.emu_cell_box_stats(_p(vn), _p(vf), _p(gn), _p(gf), _p(angles), angles.shape[0], int(R2), int(R3),
+                                st.ctypes.data_as(ctypes.c_void_p))
+    if rc != 0:
+        raise RuntimeError(lib.emu_error().decode())
+    return st
 
 
 def emu_fdk3d(y, angles, opts, filt, weight, is2d=False, KC=0, nthreads=None):

@@ -19,9 +19,13 @@ using namespace xr;
 
 namespace {
 
-// Cell-gather variant (see run_cell_RV): 0 = as launched, 1 = general 32-bit, 2 = general
-// 64-bit.
+// Cell-gather variant (see run_cell_RV): 0 = each thread from the table, 1 = general
+// 32-bit, 2 = general 64-bit, 3 = the staged workgroups as launched (run_cell_staged).
 int g_cell_variant = 0;
+
+// Staged cell gather: (thread, line) pairs served from an LDS block; workgroups
+// staged and not staged (the plan's flags).
+std::atomic<long long> g_cell_lds_lines{0}, g_cell_wg_staged{0}, g_cell_wg_direct{0};
 
 // Number of (workgroup, view) pairs the voxel gather passed over (VoxBox::skip).
 std::atomic<long long> g_voxel_skips{0};
@@ -68,10 +72,117 @@ void run_cell_RV(const Plan3d &p, const float2 *WX, const float2 *WY, float *out
     for (auto &x : th) x.join();
 }
 
+// One staged workgroup of k_cell (class SPAN 2 or 3, quad q of the class's list `items`
+// of n entries, batch item b), row tile by row tile, line by line as cell_wg runs it: the
+// line's block copied into a one-buffer "LDS" by every thread's share of voxel_stage, the
+// rest of the buffer NaN (a lookup outside the staged entries would show in the output),
+// then each thread's line.
+template <int OPK, int GEO, int R, int SPAN>
+void run_cell_quad(const Plan3d &p, const CellArgs &a, const int *items, int n, const QuadLine *ql, int q, int b,
+                   const float2 *WX, const float2 *WY, float *out)
+{
+    const int ncg = ceil_div(a.nu, kWave);
+    const float nan = std::numeric_limits<float>::quiet_NaN();
+    std::vector<float2> buf(kCellLds);
+    std::vector<CellLane<R>> st(kBlock);
+    for (int r0 = 0; r0 < a.nv; r0 += R) {
+        int v[kWavesPerBlock];
+        bool on[kBlock];
+        for (int w = 0; w < kWavesPerBlock; ++w) {
+            const int idx = q * kWavesPerBlock + w;
+            const int item = items[idx < n ? idx : q * kWavesPerBlock];
+            v[w] = item / ncg;
+            for (int l = 0; l < kWave; ++l) {
+                const int c = (item - v[w] * ncg) * kWave + l;
+                on[w * kWave + l] = idx < n && c < a.nu;
+                if (on[w * kWave + l]) cell_init<R>(st[w * kWave + l], p.d_col + (int64_t)v[w] * (a.nu + 1), a, c, r0);
+            }
+        }
+        const ViewHdr &h0 = p.d_hdr[v[0]];
+        const float2 *W = cell_table(WX, WY, h0, a, b);
+        const int plane = h0.nline * h0.nalong;
+        for (int L = 0; L < h0.nline; ++L) {
+            const VoxBox bx = cell_box<R>(ql[L], a, r0);
+            std::fill(buf.begin(), buf.end(), float2{nan, nan});
+            for (int tid = 0; tid < kBlock; ++tid) voxel_stage(W + (int64_t)L * h0.nalong, bx, buf.data(), plane, tid, kBlock);
+            for (int t = 0; t < kBlock; ++t) {
+                if (!on[t]) continue;
+                const int w = t / kWave;
+                const int item = items[q * kWavesPerBlock + w];
+                const ViewHdr &h = p.d_hdr[v[w]];
+                const float4 lc = p.d_lineC[(int64_t)v[w] * a.maxline + L];
+                cell_line_staged<OPK, GEO, R, SPAN>(st[t], h, a, lc, L, W, p.d_cgrp[item].wmax,
+                                                    reinterpret_cast<const char *>(buf.data()), bx);
+                if (bx.nc > 0) g_cell_lds_lines.fetch_add(1);
+            }
+        }
+        for (int t = 0; t < kBlock; ++t)
+            if (on[t]) {
+                const int w = t / kWave;
+                const int item = items[q * kWavesPerBlock + w];
+                cell_store<OPK, GEO, R>(st[t], p.d_hdr[v[w]], a, out, (item - v[w] * ncg) * kWave + t % kWave, r0, v[w],
+                                        b);
+            }
+    }
+}
+
+// The cell gather as launched with staging (tiles of 4 rows and more): the workgroups of
+// classes 2 and 3 that the plan marks staged, the others and class 0 directly (k_cell).
+template <int OPK, int GEO, int R>
+void run_cell_staged(const Plan3d &p, const float2 *WX, const float2 *WY, float *out, int nb, int nthreads)
+{
+    const CellArgs a = make_cell_args(p, nb);
+    const int ncg = ceil_div(a.nu, kWave);
+    struct Job {
+        int k, q, b;
+    };
+    std::vector<Job> jobs;
+    for (int k = 0; k < 3; ++k)
+        for (int q = 0; q * kWavesPerBlock < p.ncls[k]; ++q)
+            for (int b = 0; b < nb; ++b) jobs.push_back(Job{k, q, b});
+    std::vector<std::thread> th;
+    for (int t = 0; t < nthreads; ++t) {
+        th.emplace_back([&, t]() {
+            for (size_t j = t; j < jobs.size(); j += nthreads) {
+                const Job jb = jobs[j];
+                const int start = jb.k == 0 ? 0 : jb.k == 1 ? p.ncls[0] : p.ncls[0] + p.ncls[1];
+                const int *items = p.d_citems + start;
+                const int n = p.ncls[jb.k];
+                const QuadLine *ql = jb.k < 2 ? p.d_cquad + (int64_t)(p.cquad_off[jb.k] + jb.q) * a.maxline : nullptr;
+                if (ql && R >= 4 && p.d_cqstage[p.cquad_off[jb.k] + jb.q]) {
+                    if (jb.b == 0) g_cell_wg_staged.fetch_add(1);
+                    if (jb.k == 0) run_cell_quad<OPK, GEO, R, 2>(p, a, items, n, ql, jb.q, jb.b, WX, WY, out);
+                    else run_cell_quad<OPK, GEO, R, 3>(p, a, items, n, ql, jb.q, jb.b, WX, WY, out);
+                    continue;
+                }
+                if (ql && jb.b == 0) g_cell_wg_direct.fetch_add(1);
+                for (int idx = jb.q * kWavesPerBlock; idx < std::min(n, (jb.q + 1) * kWavesPerBlock); ++idx) {
+                    const int v = items[idx] / ncg, cg = items[idx] - v * ncg;
+                    const float wg = p.d_cgrp[items[idx]].wmax;
+                    for (int r0 = 0; r0 < a.nv; r0 += R)
+                        for (int c = cg * kWave; c < std::min(a.nu, (cg + 1) * kWave); ++c) {
+                            if (jb.k == 0)
+                                cell_body<OPK, GEO, R, 2, false>(p.d_hdr, p.d_lineC, p.d_col, WX, WY, out, a, c, r0, v,
+                                                                 jb.b, wg);
+                            else if (jb.k == 1)
+                                cell_body<OPK, GEO, R, 3, false>(p.d_hdr, p.d_lineC, p.d_col, WX, WY, out, a, c, r0, v,
+                                                                 jb.b, wg);
+                            else
+                                cell_body<OPK, GEO, R, 0, false>(p.d_hdr, p.d_lineC, p.d_col, WX, WY, out, a, c, r0, v,
+                                                                 jb.b, wg);
+                        }
+                }
+            }
+        });
+    }
+    for (auto &x : th) x.join();
+}
+
 template <int OPK, int GEO, int R>
 void run_cell_R(const Plan3d &p, const float2 *WX, const float2 *WY, float *out, int nb, int nt)
 {
-    if (g_cell_variant == 1) run_cell_RV<OPK, GEO, R, 1>(p, WX, WY, out, nb, nt);
+    if (g_cell_variant == 3 && !p.big) run_cell_staged<OPK, GEO, R>(p, WX, WY, out, nb, nt);
+    else if (g_cell_variant == 1) run_cell_RV<OPK, GEO, R, 1>(p, WX, WY, out, nb, nt);
     else if (g_cell_variant == 2) run_cell_RV<OPK, GEO, R, 2>(p, WX, WY, out, nb, nt);
     else run_cell_RV<OPK, GEO, R, 0>(p, WX, WY, out, nb, nt);
 }
@@ -253,6 +364,9 @@ int make_plan(Plan3d &p, PlanTables &t, const int *vn, const float *vf, const in
     p.d_col = t.col.data();
     p.d_vtile = t.vtile.data();
     p.d_cgrp = t.cgrp.data();
+    p.d_citems = t.citems.data();
+    p.d_cquad = t.cquad.data();
+    p.d_cqstage = t.cqstage.data();
     return XR_OK;
 }
 
@@ -265,6 +379,25 @@ const char *emu_error(void) { return g_err.c_str(); }
 void emu_set_cell_variant(int v) { g_cell_variant = v; }
 
 void emu_set_voxel_variant(int v) { g_voxel_variant = v; }
+
+// The staged cell gather's counts since the last reset: (thread, line) pairs served from
+// an LDS block [0], lookups outside their block [1] (must stay 0), workgroups of classes
+// 2 and 3 staged [2] and direct [3] (per batch item 0).
+void emu_cell_lines(long long *counts, int reset)
+{
+#if !defined(__HIP_DEVICE_COMPILE__)  // (this file's device pass compiles nothing it uses)
+    counts[0] = g_cell_lds_lines.load();
+    counts[1] = cell_box_misses.load();
+    counts[2] = g_cell_wg_staged.load();
+    counts[3] = g_cell_wg_direct.load();
+    if (reset) {
+        g_cell_lds_lines = 0;
+        cell_box_misses = 0;
+        g_cell_wg_staged = 0;
+        g_cell_wg_direct = 0;
+    }
+#endif
+}
 
 long long emu_voxel_skips(int reset)
 {
@@ -342,6 +475,59 @@ int emu_view_spans(const int *vn, const float *vf, const int *gn, const float *g
     for (int v = 0; v < nview; ++v) spans[v] = t.hdr[v].span;
     if (group_spans)
         for (size_t k = 0; k < t.cgrp.size(); ++k) group_spans[k] = t.cgrp[k].span;
+    return XR_OK;
+}
+
+// The staged cell gather's blocks for tiles of R rows (R2 for class 2, R3 for class 3),
+// over every (workgroup, row tile, line) with a column on the line: stats[0] blocks,
+// stats[1] over a slot's capacity, stats[2] workgroups staged, stats[3] workgroups
+// direct (the plan's staging flags), stats[4] sum of staged entries (pitch x slices) over the
+// staged workgroups, stats[5] the largest block (entries), stats[6] / stats[7] blocks
+// over 1024 / 1360 entries, stats[8..10] workgroups with a block over 1024 / 1360 / 1536,
+// stats[11] (workgroup, row tile) pairs, stats[12] of them with a block over a slot.
+int emu_cell_box_stats(const int *vn, const float *vf, const int *gn, const float *gf, const float *angles, int nview,
+                       int R2, int R3, long long *stats)
+{
+    Plan3d p;
+    PlanTables t;
+    const int rc = make_plan(p, t, vn, vf, gn, gf, angles, nview);
+    if (rc) return rc;
+    const CellArgs a = make_cell_args(p, 1);
+    for (int k = 0; k < 13; ++k) stats[k] = 0;
+    for (int k = 0; k < 2; ++k) {
+        const int R = k == 0 ? R2 : R3;
+        for (int q = 0; q * kWavesPerBlock < p.ncls[k]; ++q) {
+            const QuadLine *ql = t.cquad.data() + (size_t)(p.cquad_off[k] + q) * p.maxline;
+            const bool staged = t.cqstage[p.cquad_off[k] + q] != 0;
+            ++stats[staged ? 2 : 3];
+            bool over[3] = {false, false, false};
+            for (int r0 = 0; r0 < a.nv; r0 += R) {
+                bool tile_over = false;
+                for (int L = 0; L < p.maxline; ++L) {
+                    if (ql[L].nm <= 0) continue;
+                    ++stats[0];
+                    const int big = std::numeric_limits<int>::max();
+                    VoxBox bx;
+                    if (R == 4) bx = cell_box<4>(ql[L], a, r0, big);
+                    else if (R == 8) bx = cell_box<8>(ql[L], a, r0, big);
+                    else bx = cell_box<16>(ql[L], a, r0, big);
+                    const long long n = (long long)bx.pitch * bx.nr;
+                    if (n > kCellLds) ++stats[1];
+                    if (n > 1024) ++stats[6];
+                    if (n > 1360) ++stats[7];
+                    over[0] = over[0] || n > 1024;
+                    over[1] = over[1] || n > 1360;
+                    over[2] = over[2] || n > 1536;
+                    tile_over = tile_over || n > kCellLds;
+                    stats[5] = std::max(stats[5], n);
+                    if (staged && n <= kCellLds) stats[4] += n;
+                }
+                ++stats[11];
+                stats[12] += tile_over;
+            }
+            for (int i = 0; i < 3; ++i) stats[8 + i] += over[i];
+        }
+    }
     return XR_OK;
 }
 

@@ -6,8 +6,9 @@ import numpy as np
 import pytest
 
 import oracle
-from helpers import CASES2D, CASES3D, OPS, VOLUME_IN, angles_with_ties, correlate_fp64, emu_available, emu_fdk3d, emu_op3d, \
-    emu_view_spans, opts2d_as_3d, opts3d, rel_l2, shapes3d
+from helpers import CASES2D, CASES3D, OPS, VOLUME_IN, angles_with_ties, correlate_fp64, emu_available, \
+    emu_cell_box_stats, emu_cell_lines, emu_fdk3d, emu_op3d, emu_view_spans, opts2d_as_3d, opts3d, rel_l2, shapes3d, \
+    uniform_angles
 
 pytestmark = pytest.mark.skipif(not emu_available(), reason="emulator not built (run __graft_entry__.build())")
 
@@ -183,6 +184,69 @@ def test_voxel_gather_staged_and_direct_agree(geo):
         direct = emu_op3d(op, y, ang, opts, voxel_variant=1)
         assert rel_l2(staged, want) <= 1e-5
         assert np.array_equal(staged, direct)
+
+
+@pytest.mark.parametrize("geo", sorted(CASES3D))
+def test_cell_gather_staged_equals_direct(geo):
+    """The LDS-staged cell workgroups (each line's block copied into a slot whose other
+    entries are NaN, then every thread's lookups served from it) give the direct threads'
+    results bit for bit, every lookup inside its block; the workgroups mixing the two
+    orientations (around the 45-degree ties) run directly."""
+    opts = CASES3D[geo]
+    V = 90
+    ang = angles_with_ties(V)
+    vol, _ = shapes3d(opts, 2, V)
+    x = np.random.default_rng(31).random(vol).astype(np.float32)
+    emu_cell_lines()
+    for op in ("A", "BT", "BWT"):
+        want = oracle.op3d(op, x.astype(np.float64), ang, opts)
+        for R in (4, 8, 16):
+            direct = emu_op3d(op, x, ang, opts, R=R)
+            emu_cell_lines()
+            staged = emu_op3d(op, x, ang, opts, R=R, cell_variant=3)
+            lds_lines, misses, wg_staged, wg_direct = emu_cell_lines()
+            assert np.array_equal(staged, direct), (op, R)
+            assert misses == 0 and lds_lines > 0 and wg_staged > 0 and wg_direct > 0, (op, R)
+        assert rel_l2(staged, want) <= 1e-5, op
+
+
+def test_cell_gather_staged_oversize_lines():
+    """Tall detector rows over thin slices: blocks of 16-row tiles exceed a slot, and the
+    workgroup reads those lines from the table; the results still equal the direct ones
+    and the oracle's."""
+    opts = opts3d(24, 20, 120, 48, 16, 1.0, 1.0, 0.25, 1.0, 1.2, 500.0, 1000.0, 1, 0.2, -0.1, 0.05, 0.2, 0.3)
+    V = 48
+    ang = uniform_angles(V)
+    st = emu_cell_box_stats(ang, opts, 16, 8)
+    assert st[1] > 0 and st[1] < st[0]  # some blocks over a slot, not all
+    vol, _ = shapes3d(opts, 1, V)
+    x = np.random.default_rng(37).random(vol).astype(np.float32)
+    for op in ("A", "BWT"):
+        direct = emu_op3d(op, x, ang, opts, R=16)
+        emu_cell_lines()
+        staged = emu_op3d(op, x, ang, opts, R=16, cell_variant=3)
+        lds_lines, misses, _, _ = emu_cell_lines()
+        assert np.array_equal(staged, direct) and misses == 0 and lds_lines > 0, op
+        assert rel_l2(staged, oracle.op3d(op, x.astype(np.float64), ang, opts)) <= 1e-5, op
+
+
+@pytest.mark.parametrize("case", ["cone_flat_256x256x64", "cone_arc_512x512x256"])
+def test_cell_staging_blocks_of_the_workloads(case):
+    """The staged blocks of the two full-size workloads, as the slot capacity comment
+    states them: 256^2 x 64 cone-flat (720 views): every block of 8-row tiles fits a
+    slot (616 entries at most); 512^2 x 256 cone-arc (1152 views), 8 rows: a few per cent
+    of the blocks exceed a slot, 1848 entries at most; workgroups mixing orientations
+    are a few per cent."""
+    if case == "cone_flat_256x256x64":
+        opts, V = opts3d(256, 256, 64, 512, 64, 1, 1, 1, 1.0, 1.0, 500.0, 1000.0, 1), 720
+        st = emu_cell_box_stats(uniform_angles(V), opts, 8, 8)
+        assert st[1] == 0 and st[5] == 616
+    else:
+        opts = opts3d(512, 512, 256, 736, 128, .7, .7, .7, float(np.float32(1.2 / 1085.6)), 1.0, 595.0, 1085.6, 2)
+        V = 1152
+        st = emu_cell_box_stats(uniform_angles(V), opts, 8, 8)
+        assert 0.01 < st[1] / st[0] < 0.04 and st[5] == 1848
+    assert st[3] / (st[2] + st[3]) < 0.05
 
 
 @pytest.mark.parametrize("geo", ["flat", "arc", "par"])

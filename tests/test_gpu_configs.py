@@ -144,6 +144,34 @@ def test_cfg5_arc_transposes_full_size(cuda, op):
     assert err <= 1e-5
 
 
+def test_cfg5_staged_cell_gather(cuda):
+    """Full size, 1152 views: the default cell gather stages each workgroup's blocks in
+    LDS here (a z-prefix table of 539 MB per item), with some lines over a slot read from
+    the table.  Bitwise equal to the direct gather; four sampled views against the oracle."""
+    from xrecon_amd import _backend as be
+
+    C = be.native()
+    ang = uniform_angles(1152)
+    plan = C.Plan([float(v) for v in CFG5], torch.from_numpy(ang), 0, False)
+    s = torch.cuda.current_stream().cuda_stream
+    x = np.random.default_rng(5200).random((1, 256, 512, 512), dtype=np.float32)
+    xd = torch.from_numpy(x).to(cuda).unsqueeze(1)
+    ws = torch.empty(plan.workspace_bytes(OP_ID["A"], 1), dtype=torch.uint8, device=cuda)
+    outs = []
+    for rows in (0, 8 | 0x100):  # default (staged), XR_CELL_DIRECT
+        print(f"[gpu] cfg5 arc A, 1152 views, tile_rows {rows:#x}", flush=True)
+        out = torch.empty((1, 1, 1152, 128, 736), device=cuda)
+        plan.execute_into(OP_ID["A"], xd, out, ws, s, rows, 0, 0)
+        outs.append(out)
+    torch.cuda.synchronize()
+    assert torch.equal(outs[0], outs[1])
+    views = np.array([0, 144, 289, 577])
+    want = oracle.op3d("A", x.astype(np.float64), np.ascontiguousarray(ang[views]), CFG5)
+    err = rel_l2(outs[0][:, 0, views].cpu().numpy(), want)
+    print(f"[gpu] cfg5 arc A staged, sampled views: rel L2 {err:.2e}", flush=True)
+    assert err <= 1e-5
+
+
 def test_cfg5_batch4_items_equal_single_runs(cuda):
     from xrecon_amd import _backend as be
 

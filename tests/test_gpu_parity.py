@@ -141,6 +141,52 @@ def test_cell_tilings_bitwise_equal(cuda):
         assert torch.equal(o, outs[0]), f"tile_rows {R} differs from tile_rows 1"
 
 
+XR_CELL_DIRECT = 0x100
+XR_CELL_STAGED = 0x200
+
+# Staging geometries: three column groups per view (two- and three-pixel classes on the
+# flat panel), 720 views with the 45-degree ties, 4 batch items, so that every tiling
+# launches per class (the staged kernels) rather than in one mixed grid.
+STAGED = {
+    "flat": opts3d(64, 60, 16, 160, 24, 1.0, 1.0, 1.0, 1.2, 1.0, 500.0, 1000.0, 1, 0.2, -0.1, 0.1, 0.2, 0.1),
+    "arc": opts3d(64, 60, 16, 160, 24, 1.0, 1.0, 1.0, 1.2 / 1000.0, 1.0, 500.0, 1000.0, 2, 0.2, -0.1, 0.1,
+                  0.2 / 1000.0, 0.1),
+    "par": opts3d(64, 60, 16, 160, 24, 1.0, 1.0, 1.0, 0.9, 1.0, 0.0, 0.0, 0, 0.2, -0.1, 0.1, 0.2, 0.1),
+}
+
+
+@pytest.mark.parametrize("geo", sorted(STAGED))
+def test_cell_staged_equals_direct(cuda, geo):
+    """The cell gathers that stage each line's block of the z-prefix table in LDS read the
+    same entries as the direct ones (a line whose block exceeds a slot is read from the
+    table), so every operator and tiling is bitwise equal to tile_rows | XR_CELL_DIRECT.
+    The default for these small tables is the direct path."""
+    C = _native()
+    opts = STAGED[geo]
+    V, nb = 720, 4
+    ang = angles_with_ties(V)
+    plan = C.Plan([float(v) for v in opts], torch.from_numpy(ang), 0, False)
+    n2, n3, n0 = plan.cell_classes()
+    assert n2 + n3 > 0
+    vol, sino = shapes3d(opts, nb, V)
+    rng = np.random.default_rng(29)
+    s = torch.cuda.current_stream().cuda_stream
+    x = torch.from_numpy(rng.random(vol).astype(np.float32)).to(cuda).unsqueeze(1)
+    for op in ("A", "BT", "BWT"):  # the cell-gather operators: volume in, sinogram out
+        ws = torch.empty(max(1, plan.workspace_bytes(OPID[op], nb)), dtype=torch.uint8, device=cuda)
+        for R in (4, 8, 16):
+            print(f"[gpu] staged vs direct {geo} {op} rows {R}", flush=True)
+            outs = []
+            for flag in (XR_CELL_STAGED, XR_CELL_DIRECT, 0):
+                out = torch.empty((nb, 1, *sino[1:]), device=cuda)
+                plan.execute_into(OPID[op], x, out, ws, s, R | flag, 0, 0)
+                outs.append(out)
+            torch.cuda.synchronize()
+            assert torch.equal(outs[0], outs[1]), f"{geo} {op} rows {R}: staged differs from direct"
+            assert torch.equal(outs[2], outs[1])
+            assert torch.isfinite(outs[0]).all()
+
+
 def test_cell_four_pixel_lines(cuda):
     """Detector columns about two pixels wide: general-class groups whose lines take the
     fixed three- and four-pixel paths, against the oracle."""
@@ -190,6 +236,22 @@ def cfg2(cuda):
     plan = C.Plan([float(v) for v in CFG2], torch.from_numpy(ang), 0, False)
     x = torch.from_numpy(ph).to(cuda).view(1, 1, 64, 256, 256)
     return plan, ang, ph, x
+
+
+@pytest.mark.parametrize("rows", [8, 16])
+def test_cfg2_staged_equals_direct(cfg2, cuda, rows):
+    """Full-size projection: the LDS-staged cell gather equals the direct one bitwise."""
+    plan, ang, ph, x = cfg2
+    print(f"[gpu] cfg2 A staged vs direct, rows {rows}", flush=True)
+    s = torch.cuda.current_stream().cuda_stream
+    ws = torch.empty(plan.workspace_bytes(0, 1), dtype=torch.uint8, device=cuda)
+    outs = []
+    for flag in (XR_CELL_STAGED, XR_CELL_DIRECT):
+        out = torch.empty((1, 1, 720, 64, 512), device=cuda)
+        plan.execute_into(0, x, out, ws, s, rows | flag, 0, 0)
+        outs.append(out)
+    torch.cuda.synchronize()
+    assert torch.equal(outs[0], outs[1])
 
 
 def test_cfg2_projection_sampled_views(cfg2, cuda):
@@ -346,7 +408,7 @@ def test_invalid_tile_shapes_are_refused(cuda):
     for chunk in (2, 3, 32, 64, 8 | 0x100, 16 | 0x200, 0x100):
         with pytest.raises(RuntimeError, match="voxel_chunk must be"):
             plan.execute_into(2, y, out_v, ws, s, 0, chunk, 0)
-    for rows in (3, 32):
+    for rows in (3, 32, 3 | XR_CELL_DIRECT, 8 | 0x400, 8 | XR_CELL_DIRECT | XR_CELL_STAGED):
         with pytest.raises(RuntimeError, match="tile_rows must be"):
             plan.execute_into(0, x, out_s, ws, s, rows, 0, 0)
     proj = xrecon_amd.Projector3d((24, 20, 8), (40, 8), 1.0, 1.0, 500.0, 1000.0)

@@ -3,13 +3,14 @@ on the CPU emulator of the kernel bodies and on the GPU, with the default tiles 
 the halved voxel chunks.  Sizes, spacings, shifts, detector pitch and distances, view
 counts and angle sets (uniform with a random start, or sorted random angles) are drawn
 per seed, so the kernels' tile edges, span classes, staging blocks, view sets and slice
-chunks meet shapes no hand-written case has."""
+chunks meet shapes no hand-written case has.  On the emulator the cell gathers also run
+as LDS-staged workgroups, bit for bit equal to the direct threads."""
 import numpy as np
 import pytest
 import torch
 
 import oracle
-from helpers import OP_ID, OPS, VOLUME_IN, emu_available, emu_op3d, opts3d, rel_l2, shapes3d
+from helpers import OP_ID, OPS, VOLUME_IN, emu_available, emu_cell_lines, emu_op3d, opts3d, rel_l2, shapes3d
 
 SEEDS = list(range(12))
 
@@ -54,7 +55,14 @@ def test_random_geometry_bodies_vs_oracle(seed):
     for op in OPS:
         x = rng.random(vol if op in VOLUME_IN else sino).astype(np.float32)
         want = oracle.op3d(op, x.astype(np.float64), ang, opts)
-        assert rel_l2(emu_op3d(op, x, ang, opts), want) <= 1e-5, (seed, op)
+        got = emu_op3d(op, x, ang, opts)
+        assert rel_l2(got, want) <= 1e-5, (seed, op)
+        if op in VOLUME_IN:  # the cell gathers' LDS-staged workgroups too, bit for bit
+            for R in (4, 16):
+                emu_cell_lines()
+                staged = emu_op3d(op, x, ang, opts, R=R, cell_variant=3)
+                assert emu_cell_lines()[1] == 0, (seed, op, R)
+                assert np.array_equal(staged, emu_op3d(op, x, ang, opts, R=R)), (seed, op, R)
 
 
 @pytest.mark.gpu

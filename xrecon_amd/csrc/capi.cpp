@@ -103,11 +103,15 @@ void free_tables(Plan3d &p)
     if (p.d_vtile) (void)hipFree(p.d_vtile);
     if (p.d_cgrp) (void)hipFree(p.d_cgrp);
     if (p.d_citems) (void)hipFree(p.d_citems);
+    if (p.d_cquad) (void)hipFree(p.d_cquad);
+    if (p.d_cqstage) (void)hipFree(p.d_cqstage);
     p.d_hdr = nullptr;
     p.d_lineC = p.d_lineV = p.d_col = nullptr;
     p.d_vtile = nullptr;
     p.d_cgrp = nullptr;
     p.d_citems = nullptr;
+    p.d_cquad = nullptr;
+    p.d_cqstage = nullptr;
 }
 
 }  // namespace
@@ -143,7 +147,8 @@ int xr_plan3d_create(const xr_volume3d *vol, const xr_geometry3d *geo, const flo
     DeviceGuard g(device);
     if ((rc = upload(&p.d_hdr, t.hdr)) || (rc = upload(&p.d_lineC, t.lineC)) || (rc = upload(&p.d_lineV, t.lineV)) ||
         (rc = upload(&p.d_col, t.col)) || (rc = upload(&p.d_vtile, t.vtile)) ||
-        (rc = upload(&p.d_cgrp, t.cgrp)) || (rc = upload(&p.d_citems, t.citems))) {
+        (rc = upload(&p.d_cgrp, t.cgrp)) || (rc = upload(&p.d_citems, t.citems)) ||
+        (rc = upload(&p.d_cquad, t.cquad)) || (rc = upload(&p.d_cqstage, t.cqstage))) {
         free_tables(p);
         return rc;
     }

@@ -208,6 +208,8 @@ int xr_host_plan3d_create(const xr_volume3d *vol, const xr_geometry3d *geo, cons
     p.d_vtile = t.vtile.data();
     p.d_cgrp = t.cgrp.data();
     p.d_citems = t.citems.data();
+    p.d_cquad = t.cquad.data();
+    p.d_cqstage = t.cqstage.data();
     *out = plan.release();
     return XR_OK;
 }

@@ -8,6 +8,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cmath>
 #include <cstdint>
 
@@ -352,6 +353,51 @@ struct __attribute__((aligned(8))) Pair2 {
     float x0, y0, x1, y1;
 };
 
+// Block of a prefix table staged in LDS: columns c0 .. c0+nc-1 and rows r0 .. r0+nr-1,
+// stored row-major with an even pitch >= nc (whole 16-byte pairs per row).  nc = 0:
+// nothing staged.  Voxel gather: a view's row-prefix table (columns = detector columns,
+// rows = detector rows); skip = 1: no voxel of the workgroup's tile and slice chunk
+// reaches the detector on this view (bounds, not estimates: see VoxTile), so the
+// workgroup passes over the view.  Cell gather: a line of the z-prefix table (columns =
+// pixels along the line, rows = slices); skip unused.
+struct VoxBox {
+    int c0, nc, r0, nr, pitch, skip;
+};
+
+#if !defined(__HIP_DEVICE_COMPILE__)
+// Host builds: staged cell-gather lookups found outside their LDS block (cell_line).
+inline std::atomic<long long> cell_box_misses{0};
+#endif
+
+// A float2 at byte offset off of an LDS block / of a table in global memory.  On the
+// device the loads name their address space, so that the compiler keeps them apart
+// (ds_read vs global_load) instead of merging the two into generic "flat" loads through
+// a selected pointer.
+__host__ __device__ inline float2 lds_float2(const char *base, unsigned off)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return *(const __attribute__((address_space(3))) float2 *)((const __attribute__((address_space(3))) char *)base + off);
+#else
+    return *reinterpret_cast<const float2 *>(base + off);
+#endif
+}
+__host__ __device__ inline float2 global_float2(const char *base, unsigned off)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return *(const __attribute__((address_space(1))) float2 *)((const __attribute__((address_space(1))) char *)base + off);
+#else
+    return *reinterpret_cast<const float2 *>(base + off);
+#endif
+}
+__host__ __device__ inline Pair2 lds_pair(const char *base, unsigned off)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return *(const __attribute__((address_space(3))) Pair2 *)((const __attribute__((address_space(3))) char *)base + off);
+#else
+    return *reinterpret_cast<const Pair2 *>(base + off);
+#endif
+}
+
 __host__ __device__ inline int imin(int a, int b) { return a < b ? a : b; }
 __host__ __device__ inline int imax(int a, int b) { return a > b ? a : b; }
 
@@ -617,10 +663,18 @@ __host__ __device__ inline void cell_init(CellLane<R> &st, const float4 *__restr
 // overlaps at most SPAN consecutive pixels of this line (the caller's span test) and the
 // line takes a fixed SPAN-pixel path; SPAN = 0: the general pixel loop.  BIG: 64-bit
 // table offsets (entries), otherwise 32-bit byte offsets over the wave-uniform line base.
-template <int OPK, int GEO, int R, int SPAN, bool BIG>
+//
+// LDS (SPAN 2 and 3 only): the lookups come from the block bx of this line's table staged
+// at lds (bx.r0 .. bx.r0+bx.nr-1 slices of bx.pitch entries from pixel bx.c0, an exact
+// copy of the table's entries, so the results are those of the table).  The block holds
+// every lookup of the thread by construction (cell_box); the host build counts lookups
+// outside it in cell_box_misses, which the tests check stays zero.
+template <int OPK, int GEO, int R, int SPAN, bool BIG, bool LDS = false>
 __host__ __device__ inline void cell_line(CellLane<R> &st, const ViewHdr &h, const CellArgs &a, const float4 &lc,
-                                          int L, const float2 *__restrict__ W)
+                                          int L, const float2 *__restrict__ W, const char *lds = nullptr,
+                                          const VoxBox &bx = VoxBox{0, 0, 0, 0, 0, 0})
 {
+    static_assert(!LDS || ((SPAN == 2 || SPAN == 3) && !BIG), "LDS lookups: fixed two- or three-pixel paths");
     const int nalong = wave_uniform(h.nalong);
     const float fal = (float)nalong;
     // Column boundaries in pixel units along the line.
@@ -661,6 +715,13 @@ __host__ __device__ inline void cell_line(CellLane<R> &st, const ViewHdr &h, con
 
     if (SPAN == 2 || SPAN == 3 || SPAN == 4) {
         const int m0 = t0 > 0.0f ? (int)t0 : 0;
+#if !defined(__HIP_DEVICE_COMPILE__)
+        if (LDS) {
+            const int klo = imin(ki[0], ki[R]), khi = imax(ki[0], ki[R]);  // ki[] is monotone in q
+            if (!(m0 >= bx.c0 && m0 + (SPAN - 1) < bx.c0 + bx.pitch && klo >= bx.r0 && khi < bx.r0 + bx.nr))
+                cell_box_misses.fetch_add(1);
+        }
+#endif
         // Pixel weights of m0 .. m0+SPAN-1 (the column spans less than SPAN-1 pixels, so
         // it ends before m0+SPAN); entries past the line's end are weighted by zero.
         float w0, w1, w2 = 0.0f, w3 = 0.0f;
@@ -686,6 +747,12 @@ __host__ __device__ inline void cell_line(CellLane<R> &st, const ViewHdr &h, con
         Pair2 e2[R + 1];
 #pragma unroll
         for (int q = 0; q <= R; ++q) {
+            if (LDS) {
+                const unsigned off = mad24((unsigned)(ki[q] - bx.r0), (unsigned)bx.pitch * 8u, (unsigned)(m0 - bx.c0) * 8u);
+                e[q] = lds_pair(lds, off);
+                if (SPAN == 3) g[q] = lds_float2(lds, off + 16u);
+                continue;
+            }
             const float2 *pq;
             if (BIG) pq = Wl + ((int64_t)mul24((unsigned)ki[q], (unsigned)plane) + m0);
             else pq = reinterpret_cast<const float2 *>(Wb + mad24((unsigned)ki[q], a.plane8, (unsigned)m0 * 8u));
@@ -811,6 +878,74 @@ __host__ __device__ inline void cell_body(const ViewHdr *__restrict__ hdr, const
     cell_store<OPK, GEO, R>(st, h, a, out, c, r0, v, b);
 }
 
+// The cell gather's LDS staging: kCellSlots slots of kCellLds float2 entries (8 KiB
+// each, 32 KiB per workgroup), one line per slot, lines staged kCellSlots - 1 ahead of
+// the line being gathered.  The 4 groups of a workgroup read blocks of 208 entries on
+// average (616 at most) of a line of the 256^2 x 64 cone-flat workload for 8 rows, 347
+// (1056) for 16 rows; 529 (1848) of the 512^2 x 256 cone-arc one for 8 rows, where 2.7 %
+// of the blocks exceed a slot (the workgroup reads those lines from the table).
+constexpr int kCellLds = 1024;
+constexpr int kCellSlots = 4;
+static_assert(kCellLds / 2 % kBlock == 0, "staging copies whole rounds of pairs");
+
+// Copy rounds (global-to-LDS loads per thread) of a staged block (voxel_stage).
+__host__ __device__ inline int stage_rounds(const VoxBox &bx)
+{
+    return (bx.nr * (bx.pitch >> 1) + kBlock - 1) / kBlock;
+}
+
+// The block of line L's z-prefix table a cell-gather workgroup stages for its row
+// boundaries r0 .. r0+R: the QuadLine's pixels, and the slices over the QuadLine's range
+// of s = sigma * lc.z at the two extreme heights (z = s h + zoff is linear in h for each
+// column, so its extremes over the rows lie at the end rows).  A thread evaluates z as
+// fmaf(sigma * (v0 + r dvv), lc.z, zoff) and this bound as s h + zoff, both within a few
+// units in the last place of |s h| + |zoff| of the exact value; the bound is widened by
+// 1e-4 (|s h| + |zoff|) + 0.01 slices, far beyond that.  The block so holds every lookup
+// of the workgroup's threads.  nc = 0: nothing staged (no column reaches the line, or
+// the block exceeds cap: the workgroup then reads that line from the table).
+template <int R>
+__host__ __device__ inline VoxBox cell_box(const QuadLine &q, const CellArgs &a, int r0, int cap = kCellLds)
+{
+    VoxBox bx{0, 0, 0, 0, 0, 0};
+    if (q.nm <= 0) return bx;
+    const float h0 = fmaf((float)r0, a.dvv, a.v0), h1 = fmaf((float)(r0 + R), a.dvv, a.v0);
+    const float z0 = q.sg_lo * h0, z1 = q.sg_hi * h0, z2 = q.sg_lo * h1, z3 = q.sg_hi * h1;
+    const float plo = fminf(fminf(z0, z1), fminf(z2, z3)), phi = fmaxf(fmaxf(z0, z1), fmaxf(z2, z3));
+    if (!(plo <= phi)) return bx;
+    const float tol = fmaf(1e-4f, fmaxf(fabsf(plo), fabsf(phi)) + fabsf(a.zoff), 0.01f);
+    const float fnz = (float)a.nz;
+    const int s0 = (int)clampf(floorf(plo + a.zoff - tol), 0.0f, fnz);
+    const int s1 = (int)clampf(floorf(phi + a.zoff + tol), 0.0f, fnz);
+    const int pitch = (q.nm + 1) & ~1;
+    const int ns = s1 - s0 + 1;
+    if (ns * pitch > cap) return bx;
+    bx.c0 = q.m_lo;
+    bx.nc = q.nm;
+    bx.r0 = s0;
+    bx.nr = ns;
+    bx.pitch = pitch;
+    return bx;
+}
+
+// Line L of a cell-gather thread in a group of span class SPAN (2 or 3), each line on
+// the path cell_body gives it, with the lookups from the staged block bx at lds, or from
+// the table W where nothing was staged (bx.nc = 0: the block exceeds a slot, or no
+// column reaches the line; the same for the whole workgroup).
+template <int OPK, int GEO, int R, int SPAN>
+__host__ __device__ inline void cell_line_staged(CellLane<R> &st, const ViewHdr &h, const CellArgs &a,
+                                                 const float4 &lc, int L, const float2 *__restrict__ W, float wgrp,
+                                                 const char *lds, const VoxBox &bx)
+{
+    static_assert(SPAN == 2 || SPAN == 3, "staged lines: span classes 2 and 3");
+    if (SPAN == 2 || wgrp * lc.y < 0.99f) {
+        if (bx.nc > 0) cell_line<OPK, GEO, R, 2, false, true>(st, h, a, lc, L, W, lds, bx);
+        else cell_line<OPK, GEO, R, 2, false>(st, h, a, lc, L, W);
+    } else {
+        if (bx.nc > 0) cell_line<OPK, GEO, R, 3, false, true>(st, h, a, lc, L, W, lds, bx);
+        else cell_line<OPK, GEO, R, 3, false>(st, h, a, lc, L, W);
+    }
+}
+
 // ---------------------------------------------------------------- voxel gather
 
 // First column index to visit for axis point a: the closed-form inverse of the column map
@@ -823,14 +958,6 @@ __host__ __device__ inline float column_seed(const ViewHdr &h, float a)
     return h.tk0 + h.tk1 * (GEO == GEO_ARC ? atanf(T) : T);
 }
 
-// Block of one view's row-prefix table staged in LDS for a voxel-gather workgroup:
-// columns c0 .. c0+nc-1 and rows r0 .. r0+nr-1, stored row-major with an even pitch >= nc
-// (whole 16-byte pairs per row).  nc = 0: nothing staged.  skip = 1: no voxel of the
-// workgroup's tile and slice chunk reaches the detector on this view (bounds, not
-// estimates: see VoxTile), so the workgroup passes over the view.
-struct VoxBox {
-    int c0, nc, r0, nr, pitch, skip;
-};
 
 // Capacity of the staged block, in float2 entries (20 KiB of LDS per workgroup; 40 KiB
 // for chunks of 32 slices).  The blocks of 16 x 16 voxel columns x 16 slices of a 1 mm
@@ -916,11 +1043,14 @@ __host__ __device__ inline VoxBox voxel_box(const VoxTile &t, const VoxArgs &a, 
     return bx;
 }
 
-// Thread tid's share of copying the block from view v's row-prefix table Qv into LDS, in
-// 16-byte pairs of adjacent columns: pair p = tid + s * nthreads goes to lds[2p, 2p+1]
-// (at most vox_lds(KC) / 2 pairs, so s < vox_lds(KC) / 2 / nthreads).  A pair may end one entry
-// past the view's last column (the next row's first entry, or the pad after the table);
-// that entry is never looked up.
+// Thread tid's share of copying the block from the table Qv (rows nu entries apart:
+// a view's row-prefix table, or a line of a z-prefix table with nu = its plane) into
+// LDS, in 16-byte pairs of adjacent columns: pair p = tid + s * nthreads goes to lds[2p, 2p+1]
+// (at most capacity / 2 pairs, a multiple of nthreads for every buffer here: vox_lds(KC)
+// and kCellLds, so s < capacity / 2 / nthreads).  A voxel-gather pair may end one entry
+// past the view's last column (the next row's first entry, or the pad after the table),
+// an entry never looked up; a cell-gather block may reach three entries past the line's
+// end (the next line's first entries, or the table's kVolPad pad entries).
 //
 // On the device the copies are direct global-to-LDS loads (no registers held): lane l of
 // the wave writes 16 bytes at the wave's LDS base + 16 l, which is where pair p lands
@@ -971,26 +1101,6 @@ __host__ __device__ inline bool wave_any(bool p)
 #endif
 }
 
-// A float2 at byte offset off of an LDS block / of a table in global memory.  On the
-// device the loads name their address space, so that the compiler keeps them apart
-// (ds_read vs global_load) instead of merging the two into generic "flat" loads through
-// a selected pointer.
-__host__ __device__ inline float2 lds_float2(const char *base, unsigned off)
-{
-#if defined(__HIP_DEVICE_COMPILE__)
-    return *(const __attribute__((address_space(3))) float2 *)((const __attribute__((address_space(3))) char *)base + off);
-#else
-    return *reinterpret_cast<const float2 *>(base + off);
-#endif
-}
-__host__ __device__ inline float2 global_float2(const char *base, unsigned off)
-{
-#if defined(__HIP_DEVICE_COMPILE__)
-    return *(const __attribute__((address_space(1))) float2 *)((const __attribute__((address_space(1))) char *)base + off);
-#else
-    return *reinterpret_cast<const float2 *>(base + off);
-#endif
-}
 
 // Where one column's row-prefix entries are read from: the staged LDS block (when the
 // column lies in it) or the view's table in HBM.  Lb is the LDS block itself (never
@@ -1216,6 +1326,125 @@ __device__ inline void voxel_wg(const ViewHdr *__restrict__ hdr, const float4 *_
         __syncthreads();
     }
     if (active) voxel_store<KC>(out, a, i, j, k0, b, acc);
+}
+
+// Waits until at most n (0..4) of this wave's vector memory loads are outstanding (they
+// complete in order).
+__device__ inline void wait_vmem(int n)
+{
+    // s_waitcnt immediate: vmcnt in bits 3:0, expcnt (7: no wait) 6:4, lgkmcnt (15) 11:8
+    switch (n) {
+    case 0: __builtin_amdgcn_s_waitcnt(0xF70); break;
+    case 1: __builtin_amdgcn_s_waitcnt(0xF71); break;
+    case 2: __builtin_amdgcn_s_waitcnt(0xF72); break;
+    case 3: __builtin_amdgcn_s_waitcnt(0xF73); break;
+    default: __builtin_amdgcn_s_waitcnt(0xF74); break;
+    }
+}
+
+// Workgroup barrier that leaves this wave's outstanding global loads in flight (a
+// __syncthreads() fence waits for all of them).  The caller has waited for the loads
+// whose LDS data it shares across the barrier (wait_vmem).  This wave's LDS reads have
+// completed (lgkmcnt 0) before other waves may overwrite what they read, and the empty
+// asm statements keep the compiler from moving memory accesses across the barrier.
+__device__ inline void wg_barrier()
+{
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0) only
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+}
+
+// Cell gather of one workgroup with its z-prefix lookups served from LDS.  The 4 waves
+// take 4 groups of span class SPAN (2 or 3) on rows r0..r0+R-1 (see k_cell); ql: the
+// workgroup's QuadLine per line.  Line L's block (cell_box) is staged into slot L mod 4
+// three lines ahead of its gather: at line L each wave waits for its own copies of L
+// (the copies of L+1 and L+2, issued later, may stay in flight), the workgroup meets at
+// a barrier (everyone's copies of L have landed; everyone is done with line L-1, whose
+// slot L+3 then takes), and gathers L.  The 4 groups' views share one orientation (the
+// plan stages no other workgroup: cqstage), so the line count and the table are the same
+// for the 4 waves.  Every thread of the workgroup
+// takes part in staging and barriers; active: the thread has a column (its wave a group,
+// its lane a column below Nu).  The lines go four at a time so that each slot is its
+// own array at every access.
+template <int OPK, int GEO, int R, int SPAN>
+__device__ inline void cell_wg(const ViewHdr *__restrict__ hdr, const float4 *__restrict__ lineC,
+                               const float4 *__restrict__ colt, const QuadLine *__restrict__ ql,
+                               const float2 *__restrict__ WX, const float2 *__restrict__ WY, float *__restrict__ out,
+                               const CellArgs &a, int c, int r0, int v, int b, float wgrp, bool active,
+                               float2 *__restrict__ s0, float2 *__restrict__ s1, float2 *__restrict__ s2,
+                               float2 *__restrict__ s3)
+{
+    static_assert(kCellSlots == 4, "four slots, lines staged three ahead");
+    const ViewHdr h = ro_load(hdr + v);
+    CellLane<R> st;
+    cell_init<R>(st, colt + (int64_t)v * (a.nu + 1), a, active ? c : 0, r0);
+    const float2 *W = cell_table(WX, WY, h, a, b);
+    const float4 *lt = lineC + (int64_t)v * a.maxline;
+    const int nline = wave_uniform(h.nline), nalong = wave_uniform(h.nalong);
+    const int plane = nline * nalong;
+    const VoxBox none{0, 0, 0, 0, 0, 0};
+    // The blocks of 64 lines at a time, line base + l on lane l (cell_box evaluated over
+    // the wave's lanes), read back per line with readlane.  Chunk bases are multiples of 64.
+    const int lane = threadIdx.x & (kWave - 1);
+    int kc0 = 0, kr0 = 0, knr = 0, kpitch = 0;
+    auto chunk = [&](int base) {
+        const VoxBox bx = cell_box<R>(g_load(ql + imin(base + lane, nline - 1)), a, r0);
+        kc0 = bx.c0;
+        kr0 = bx.r0;
+        knr = bx.nr;
+        kpitch = bx.pitch;
+    };
+    auto box_of = [&](int j) {
+        const int l = j & (kWave - 1);
+        VoxBox bx;
+        bx.c0 = __builtin_amdgcn_readlane(kc0, l);
+        bx.r0 = __builtin_amdgcn_readlane(kr0, l);
+        bx.nr = __builtin_amdgcn_readlane(knr, l);
+        bx.pitch = __builtin_amdgcn_readlane(kpitch, l);
+        bx.nc = bx.pitch;  // > 0 exactly when the line is staged
+        bx.skip = 0;
+        return bx;
+    };
+    chunk(0);
+    VoxBox b0 = box_of(0);
+    VoxBox b1 = nline > 1 ? box_of(1) : none;
+    VoxBox b2 = nline > 2 ? box_of(2) : none;
+    VoxBox b3 = none;
+    voxel_stage(W, b0, s0, plane, threadIdx.x, kBlock);
+    voxel_stage(W + nalong, b1, s1, plane, threadIdx.x, kBlock);
+    voxel_stage(W + 2 * (int64_t)nalong, b2, s2, plane, threadIdx.x, kBlock);
+    // The line coefficients of L, read one step ahead: a step issues the next step's read
+    // after its barrier, and it lands during its gather.
+    float4 lc = ro_load(lt);
+    // line L from slot rd (block cur); n1, n2: the blocks of L+1, L+2; n3 (slot wr) is set to L+3's
+    auto step = [&](int L, const VoxBox &cur, const VoxBox &n1, const VoxBox &n2, VoxBox &n3, const float2 *rd,
+                    float2 *wr) {
+        wait_vmem(stage_rounds(n1) + stage_rounds(n2));
+        wg_barrier();
+        const float4 lc_next = ro_load(lt + imin(L + 1, nline - 1));
+        if (L + 3 < nline) {
+            if (((L + 3) & (kWave - 1)) == 0) chunk(L + 3);
+            n3 = box_of(L + 3);
+        } else {
+            n3 = none;
+        }
+        voxel_stage(W + (int64_t)(L + 3) * nalong, n3, wr, plane, threadIdx.x, kBlock);
+        __builtin_amdgcn_sched_barrier(0);  // issue the copies before the gather
+        if (active)
+            cell_line_staged<OPK, GEO, R, SPAN>(st, h, a, lc, L, W, wgrp, reinterpret_cast<const char *>(rd), cur);
+        lc = lc_next;
+    };
+    for (int L = 0; L < nline; L += 4) {
+        step(L, b0, b1, b2, b3, s0, s3);
+        if (L + 1 >= nline) break;
+        step(L + 1, b1, b2, b3, b0, s1, s0);
+        if (L + 2 >= nline) break;
+        step(L + 2, b2, b3, b0, b1, s2, s1);
+        if (L + 3 >= nline) break;
+        step(L + 3, b3, b0, b1, b2, s3, s2);
+    }
+    if (active) cell_store<OPK, GEO, R>(st, h, a, out, c, r0, v, b);
 }
 
 }  // namespace xr

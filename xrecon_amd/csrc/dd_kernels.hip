@@ -230,11 +230,14 @@ __global__ __launch_bounds__(kBlock) void k_fdk_prefix(const ViewHdr *__restrict
 // Every wave of a launch but the last quad's has work whatever Nv is.  SPAN = the class (2, 3 or 0), so each
 // class runs with the registers of its own paths; SPAN = -1: every class in one launch,
 // each workgroup reading its group's class (small problems, where launches cost more
-// than registers).
-template <int OPK, int GEO, int R, int SPAN, bool BIG>
-__global__ __launch_bounds__(kBlock) void k_cell(const ViewHdr *__restrict__ hdr, const float4 *__restrict__ lineC,
+// than registers).  STAGE (classes 2 and 3, 4 to 16 rows): a workgroup the plan marks
+// (cqstage) serves its lookups from LDS (cell_wg), with cquad the class's QuadLine table
+// (maxline entries per workgroup); the others read the table directly.
+template <int OPK, int GEO, int R, int SPAN, bool BIG, bool STAGE>
+__global__ __launch_bounds__(kBlock, STAGE && R <= 8 ? 4 : 1) void k_cell(const ViewHdr *__restrict__ hdr, const float4 *__restrict__ lineC,
                                                  const float4 *__restrict__ colt, const CellGroup *__restrict__ cgrp,
                                                  const int *__restrict__ items, int nitems,
+                                                 const QuadLine *__restrict__ cquad, const int *__restrict__ cqstage,
                                                  const float2 *__restrict__ WX, const float2 *__restrict__ WY,
                                                  float *__restrict__ out, CellArgs a)
 {
@@ -255,6 +258,24 @@ __global__ __launch_bounds__(kBlock) void k_cell(const ViewHdr *__restrict__ hdr
         b = rest / nquad;
     }
     const int idx = quad * kWavesPerBlock + wave;
+    if constexpr (STAGE) {
+        static_assert((SPAN == 2 || SPAN == 3) && !BIG && (R == 4 || R == 8 || R == 16),
+                      "staged cell gathers: span classes 2 and 3, 4 to 16 rows");
+        if (ro_load(cqstage + quad)) {  // the same for the whole workgroup
+            const QuadLine *ql = cquad + (int64_t)quad * a.maxline;
+            __shared__ __attribute__((aligned(16))) float2 s0[kCellLds];
+            __shared__ __attribute__((aligned(16))) float2 s1[kCellLds];
+            __shared__ __attribute__((aligned(16))) float2 s2[kCellLds];
+            __shared__ __attribute__((aligned(16))) float2 s3[kCellLds];
+            const bool on = idx < nitems;  // a wave past the list stages and waits with the others
+            const int item = items[on ? idx : quad * kWavesPerBlock];
+            const int v = item / ncg, cg = item - v * ncg;
+            const int c = cg * kWave + lane;
+            cell_wg<OPK, GEO, R, SPAN>(hdr, lineC, colt, ql, WX, WY, out, a, c, rt * R, v, b, ro_load(cgrp + item).wmax,
+                                       on && c < a.nu, s0, s1, s2, s3);
+            return;
+        }
+    }
     if (idx >= nitems) return;  // the whole wave
     const int item = items[idx];
     const int v = item / ncg, cg = item - v * ncg;
@@ -310,24 +331,38 @@ inline int64_t cell_blocks(const CellArgs &a, int R, int nitems, int nbatch)
     return (int64_t)ceil_div(nitems, kWavesPerBlock) * ceil_div(a.nv, R) * nbatch;
 }
 
+// cquad / cqstage: the class's QuadLine table and staging flags, or null for the direct path.
+// Staging applies to tiles of 4 rows and more (fewer rows reuse a staged line too little).
 template <int OPK, int GEO, int SPAN, bool BIG>
-void launch_cell_class(const Plan3d &p, const CellArgs &a, int R, const int *items, int nitems, const float2 *WX,
-                       const float2 *WY, float *out, int nbatch, hipStream_t s)
+void launch_cell_class(const Plan3d &p, const CellArgs &a, int R, const int *items, int nitems,
+                       const QuadLine *cquad, const int *cqstage, const float2 *WX, const float2 *WY, float *out,
+                       int nbatch, hipStream_t s)
 {
     if (nitems == 0) return;
     const dim3 grid((unsigned)cell_blocks(a, R, nitems, nbatch));
-#define XR_CELL(RR)                                                                                                  \
-    hipLaunchKernelGGL((k_cell<OPK, GEO, RR, SPAN, BIG>), grid, dim3(kBlock), 0, s, p.d_hdr, p.d_lineC, p.d_col,    \
-                       p.d_cgrp, items, nitems, WX, WY, out, a)
+#define XR_CELL(RR, ST)                                                                                              \
+    hipLaunchKernelGGL((k_cell<OPK, GEO, RR, SPAN, BIG, ST>), grid, dim3(kBlock), 0, s, p.d_hdr, p.d_lineC,         \
+                       p.d_col, p.d_cgrp, items, nitems, cquad, cqstage, WX, WY, out, a)
+    constexpr bool can_stage = (SPAN == 2 || SPAN == 3) && !BIG;
+    const bool st = can_stage && cquad != nullptr;
     switch (R) {
-    case 1: XR_CELL(1); break;
-    case 2: XR_CELL(2); break;
-    case 4: XR_CELL(4); break;
-    case 16:
-        if constexpr (SPAN == 2 || SPAN == -1) XR_CELL(16);  // the wider classes run at most 8 rows
-        else XR_CELL(8);
+    case 1: XR_CELL(1, false); break;
+    case 2: XR_CELL(2, false); break;
+    case 4:
+        if (st) XR_CELL(4, can_stage);
+        else XR_CELL(4, false);
         break;
-    default: XR_CELL(8); break;
+    case 16:
+        if constexpr (SPAN == 2 || SPAN == -1) {  // the wider classes run at most 8 rows
+            if (st) XR_CELL(16, can_stage);
+            else XR_CELL(16, false);
+            break;
+        }
+        [[fallthrough]];
+    default:
+        if (st) XR_CELL(8, can_stage);
+        else XR_CELL(8, false);
+        break;
     }
 #undef XR_CELL
 }
@@ -339,24 +374,31 @@ void launch_cell_class(const Plan3d &p, const CellArgs &a, int R, const int *ite
 constexpr int64_t kCellSplitMin = 4096;
 
 template <int OPK, int GEO>
-void launch_cell(const Plan3d &p, int R, const float2 *WX, const float2 *WY, float *out, int nbatch, hipStream_t s)
+void launch_cell(const Plan3d &p, int R, bool stage, const float2 *WX, const float2 *WY, float *out, int nbatch,
+                 hipStream_t s)
 {
     const CellArgs a = make_cell_args(p, nbatch);
     const int ntot = p.ncls[0] + p.ncls[1] + p.ncls[2];
     if (p.big) {
         // tables of 4 GiB and more: one 64-bit general variant over every group
-        launch_cell_class<OPK, GEO, 0, true>(p, a, 8, p.d_citems, ntot, WX, WY, out, nbatch, s);
+        launch_cell_class<OPK, GEO, 0, true>(p, a, 8, p.d_citems, ntot, nullptr, nullptr, WX, WY, out, nbatch, s);
         return;
     }
     if (cell_blocks(a, R, ntot, nbatch) < kCellSplitMin) {
-        launch_cell_class<OPK, GEO, -1, false>(p, a, R, p.d_citems, ntot, WX, WY, out, nbatch, s);
+        launch_cell_class<OPK, GEO, -1, false>(p, a, R, p.d_citems, ntot, nullptr, nullptr, WX, WY, out, nbatch, s);
         return;
     }
     const int Rw = R > 8 ? 8 : R;
-    launch_cell_class<OPK, GEO, 2, false>(p, a, R, p.d_citems, p.ncls[0], WX, WY, out, nbatch, s);
-    launch_cell_class<OPK, GEO, 3, false>(p, a, Rw, p.d_citems + p.ncls[0], p.ncls[1], WX, WY, out, nbatch, s);
-    launch_cell_class<OPK, GEO, 0, false>(p, a, Rw, p.d_citems + p.ncls[0] + p.ncls[1], p.ncls[2], WX, WY, out,
-                                          nbatch, s);
+    const bool st = stage && p.d_cquad;
+    const QuadLine *q2 = st ? p.d_cquad + (int64_t)p.cquad_off[0] * p.maxline : nullptr;
+    const QuadLine *q3 = st ? p.d_cquad + (int64_t)p.cquad_off[1] * p.maxline : nullptr;
+    const int *f2 = st ? p.d_cqstage + p.cquad_off[0] : nullptr;
+    const int *f3 = st ? p.d_cqstage + p.cquad_off[1] : nullptr;
+    launch_cell_class<OPK, GEO, 2, false>(p, a, R, p.d_citems, p.ncls[0], q2, f2, WX, WY, out, nbatch, s);
+    launch_cell_class<OPK, GEO, 3, false>(p, a, Rw, p.d_citems + p.ncls[0], p.ncls[1], q3, f3, WX, WY, out, nbatch,
+                                          s);
+    launch_cell_class<OPK, GEO, 0, false>(p, a, Rw, p.d_citems + p.ncls[0] + p.ncls[1], p.ncls[2], nullptr, nullptr,
+                                          WX, WY, out, nbatch, s);
 }
 
 template <int OPK, int GEO>
@@ -376,12 +418,12 @@ void launch_voxel(const Plan3d &p, int KC, const float2 *Q, float *out, float *p
 }
 
 template <int GEO>
-void launch_cell_geo(const Plan3d &p, int kop, int R, const float2 *WX, const float2 *WY, float *out, int nb,
-                     hipStream_t s)
+void launch_cell_geo(const Plan3d &p, int kop, int R, bool stage, const float2 *WX, const float2 *WY, float *out,
+                     int nb, hipStream_t s)
 {
-    if (kop == CELL_A) launch_cell<CELL_A, GEO>(p, R, WX, WY, out, nb, s);
-    else if (kop == CELL_BT) launch_cell<CELL_BT, GEO>(p, R, WX, WY, out, nb, s);
-    else launch_cell<CELL_BWT, GEO>(p, R, WX, WY, out, nb, s);
+    if (kop == CELL_A) launch_cell<CELL_A, GEO>(p, R, stage, WX, WY, out, nb, s);
+    else if (kop == CELL_BT) launch_cell<CELL_BT, GEO>(p, R, stage, WX, WY, out, nb, s);
+    else launch_cell<CELL_BWT, GEO>(p, R, stage, WX, WY, out, nb, s);
 }
 
 template <int GEO>
@@ -529,9 +571,13 @@ int execute_phase(const Plan3d &p, int op, int phase, const float *in, float *ou
     const size_t need = workspace_bytes(p, op, nbatch);
     if (ws_bytes < need || (need && !ws))
         return fail(XR_EWORKSPACE, "workspace of " + std::to_string(ws_bytes) + " bytes, need " + std::to_string(need));
+    const int cell_mode = tile_rows & (XR_CELL_DIRECT | XR_CELL_STAGED);
+    tile_rows &= ~(XR_CELL_DIRECT | XR_CELL_STAGED);
     if (tile_rows == 0) tile_rows = default_rows(p);
     if (voxel_chunk == 0) voxel_chunk = default_chunk(p);
-    if (!valid_rows(tile_rows)) return fail(XR_EINVAL, "tile_rows must be 1, 2, 4, 8 or 16");
+    if (!valid_rows(tile_rows) || cell_mode == (XR_CELL_DIRECT | XR_CELL_STAGED))
+        return fail(XR_EINVAL, "tile_rows must be 1, 2, 4, 8 or 16, with at most one of XR_CELL_DIRECT, XR_CELL_STAGED");
+    const bool stage = cell_mode == XR_CELL_STAGED || (cell_mode == 0 && make_cell_args(p, 1).band);
     if (!valid_chunk(voxel_chunk)) return fail(XR_EINVAL, "voxel_chunk must be 1, 4, 8, 16, 16 | XR_VOXEL_HALVES or 32 | XR_VOXEL_HALVES");
 
     if (rt.cell && cell_blocks(make_cell_args(p, nbatch), 1, p.ncls[0] + p.ncls[1] + p.ncls[2], nbatch) >=
@@ -552,9 +598,9 @@ int execute_phase(const Plan3d &p, int op, int phase, const float *in, float *ou
             }
         }
         if (do_gather) {
-            if (p.kind == GEO_FLAT) launch_cell_geo<GEO_FLAT>(p, rt.kop, tile_rows, WX, WY, out, nbatch, s);
-            else if (p.kind == GEO_ARC) launch_cell_geo<GEO_ARC>(p, rt.kop, tile_rows, WX, WY, out, nbatch, s);
-            else launch_cell_geo<GEO_PAR>(p, rt.kop, tile_rows, WX, WY, out, nbatch, s);
+            if (p.kind == GEO_FLAT) launch_cell_geo<GEO_FLAT>(p, rt.kop, tile_rows, stage, WX, WY, out, nbatch, s);
+            else if (p.kind == GEO_ARC) launch_cell_geo<GEO_ARC>(p, rt.kop, tile_rows, stage, WX, WY, out, nbatch, s);
+            else launch_cell_geo<GEO_PAR>(p, rt.kop, tile_rows, stage, WX, WY, out, nbatch, s);
         }
     } else {
         float2 *Q = static_cast<float2 *>(ws);

@@ -310,6 +310,56 @@ int build_tables(Plan3d &p, const float *angles, PlanTables &t, std::string &err
         p.ncls[k] = t.ncls[k];
     }
 
+    // LDS staging bounds of the cell gather's workgroups of classes 2 and 3 (QuadLine):
+    // workgroup q of a class takes the class's entries 4q .. 4q+3.  A column's pixel range
+    // on a line comes from its boundaries t = d lc.y + lc.x, evaluated as the kernel
+    // evaluates them; the boundaries d are monotone in the column, so the group's first and
+    // last boundaries bound all of its columns.
+    {
+        const int ML = p.maxline;
+        int nq = 0;
+        for (int k = 0; k < 2; ++k) {
+            p.cquad_off[k] = nq;
+            nq += (int)ceil_div(t.ncls[k], kWavesPerBlock);
+        }
+        t.cquad.assign((size_t)nq * ML, QuadLine{0, 0, 0.0f, 0.0f});
+        t.cqstage.assign((size_t)nq, 0);
+        int start = 0;
+        for (int k = 0; k < 2; ++k) {
+            const int extra = k == 0 ? 1 : 2;  // entries a fixed-span read takes past its first pixel
+            for (int q = 0; q * kWavesPerBlock < t.ncls[k]; ++q) {
+                QuadLine *ql = t.cquad.data() + (size_t)(p.cquad_off[k] + q) * ML;
+                const int i0 = start + q * kWavesPerBlock, i1 = std::min(start + t.ncls[k], i0 + kWavesPerBlock);
+                const ViewHdr &h0 = t.hdr[t.citems[i0] / ncg];
+                bool mixed = false;
+                for (int i = i0; i < i1; ++i) mixed = mixed || t.hdr[t.citems[i] / ncg].xdrv != h0.xdrv;
+                if (mixed) continue;  // not staged: the waves' line counts and tables may differ
+                const float fal = (float)h0.nalong;
+                for (int L = 0; L < h0.nline; ++L) {
+                    int mlo = h0.nalong, mhi = -1;
+                    float slo = 3.0e38f, shi = -3.0e38f;
+                    for (int i = i0; i < i1; ++i) {
+                        const int v = t.citems[i] / ncg, g = t.citems[i] - v * ncg;
+                        const float4 lc = t.lineC[(size_t)v * ML + L];
+                        const float4 *ct = t.col.data() + (size_t)v * (nu + 1);
+                        const float ta = std::fmaf(ct[g * kWave].x, lc.y, lc.x);
+                        const float tb = std::fmaf(ct[std::min(nu, (g + 1) * kWave)].x, lc.y, lc.x);
+                        const float tlo = std::min(ta, tb), thi = std::max(ta, tb);
+                        if (!(thi > 0.0f && tlo < fal)) continue;
+                        mlo = std::min(mlo, std::max((int)std::floor(std::max(tlo, -1.0f)) - 1, 0));
+                        mhi = std::max(mhi, std::min((int)std::floor(std::min(thi, fal)) + 1, h0.nalong - 1) + extra);
+                        const CellGroup &cg = t.cgrp[(size_t)v * ncg + g];
+                        slo = std::min(slo, std::min(cg.sig_lo * lc.z, cg.sig_hi * lc.z));
+                        shi = std::max(shi, std::max(cg.sig_lo * lc.z, cg.sig_hi * lc.z));
+                    }
+                    if (mhi >= mlo) ql[L] = QuadLine{mlo, mhi - mlo + 1, slo, shi};
+                }
+                t.cqstage[(size_t)p.cquad_off[k] + q] = 1;
+            }
+            start += t.ncls[k];
+        }
+    }
+
     // Voxel-gather staging estimates per (view, 16 x 16 tile of voxel columns).
     t.vtile.assign((size_t)V * gx * gy, VoxTile{0, 0, 0.0f, 0.0f});
     for (int v = 0; v < V; ++v) {

@@ -54,7 +54,7 @@ struct VoxTile {
 static_assert(sizeof(VoxTile) == 16, "VoxTile layout");
 
 // Per (view, 64-column group of the cell gather): the range of the depth scale sigma over
-// the group's columns (descriptive; no kernel reads it), the group's span class -- 2 / 3: each of its
+// the group's columns (the plan's staging bounds use it; no kernel reads it), the group's span class -- 2 / 3: each of its
 // columns overlaps at most 2 / 3 consecutive pixels of any line, 0: no bound -- and its
 // widest column on the axis, wmax.  On a diagonal view the edge columns of a flat panel
 // are wider on the axis than the central ones, so most groups of a three-pixel view
@@ -66,6 +66,20 @@ struct CellGroup {
     float wmax;
 };
 static_assert(sizeof(CellGroup) == 16, "CellGroup layout");
+
+// Per (cell-gather workgroup, line): what the workgroup's 4 groups can look up in the
+// z-prefix table on that line, for staging it in LDS.  Pixels m_lo .. m_lo+nm-1 of the
+// line: the union over the groups of the pixels their columns overlap, one pixel of
+// margin on each side, plus the one (two-pixel class) or two (three-pixel class)
+// entries a fixed-span read takes past its first pixel.  Slices: z = s h + zoff for a
+// row boundary at height h on the axis, with s = sigma * lc.z in [sg_lo, sg_hi] over the
+// groups' columns (sigma: the column's depth scale; lc.z: the line's z factor).
+// nm = 0: no column of the workgroup overlaps the line.
+struct QuadLine {
+    int m_lo, nm;
+    float sg_lo, sg_hi;
+};
+static_assert(sizeof(QuadLine) == 16, "QuadLine layout");
 
 // Geometry family of a plan; selects the column inverse g() and the weights.
 enum GeoKind : int { GEO_PAR = 0, GEO_FLAT = 1, GEO_ARC = 2 };
@@ -100,6 +114,11 @@ struct Plan3d {
     int *d_citems = nullptr;     // the groups v * ceil(Nu/64) + g listed by span class (2, then 3, then 0),
                                  // within a class by g, then v
     int ncls[3] = {0, 0, 0};     // number of groups of span class 2, 3 and 0
+    QuadLine *d_cquad = nullptr; // [workgroups of classes 2 and 3][maxline]: the cell gather's LDS staging
+                                 // bounds; workgroup q of class k is entry cquad_off[k] + q
+    int *d_cqstage = nullptr;      // [workgroups of classes 2 and 3]: 1 for the workgroups the cell gather
+                                 // stages (their 4 views share one orientation), else 0
+    int cquad_off[2] = {0, 0};
 };
 
 // Host-side tables before upload (kept separate so tests can inspect the builder).
@@ -110,6 +129,8 @@ struct PlanTables {
     std::vector<CellGroup> cgrp;
     std::vector<int> citems;  // as Plan3d::d_citems
     int ncls[3] = {0, 0, 0};
+    std::vector<QuadLine> cquad;  // as Plan3d::d_cquad
+    std::vector<int> cqstage;       // as Plan3d::d_cqstage
 };
 
 // capi.cpp: the geometry checks of xr_plan3d_create (sizes, spacings, index limits).
