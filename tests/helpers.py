@@ -143,14 +143,13 @@ def emu_cell_lines(reset=True):
 
 def emu_cell_box_stats(angles, opts, R2, R3):
     """Block statistics of the staged cell gather over a plan (tiles of R2 rows for the
-    two-pixel class, R3 for the three-pixel one): [blocks, blocks over a slot, workgroups
-    staged, workgroups direct, entries staged, largest block, blocks over 1024, over 1360,
-    workgroups with a block over 1024 / 1360 / 1536, (workgroup, row tile) pairs, pairs
-    with a block over a slot]."""
+    two-pixel class, R3 for the three-pixel one): [blocks, blocks over a slot, (workgroup,
+    row tile) pairs staged, pairs direct, entries staged, largest block, blocks over 1024
+    entries]."""
     lib = _load_emu()
     vn, vf, gn, gf = oracle.split3d(opts)
     angles = np.ascontiguousarray(angles, dtype=np.float32)
-    st = np.zeros(13, dtype=np.int64)
+    st = np.zeros(7, dtype=np.int64)
     rc = lib.emu_cell_box_stats(_p(vn), _p(vf), _p(gn), _p(gf), _p(angles), angles.shape[0], int(R2), int(R3),
                                 st.ctypes.data_as(ctypes.c_void_p))
     if rc != 0:

@@ -73,66 +73,67 @@ void run_cell_RV(const Plan3d &p, const float2 *WX, const float2 *WY, float *out
 }
 
 // One staged workgroup of k_cell (class SPAN 2 or 3, quad q of the class's list `items`
-// of n entries, batch item b), row tile by row tile, line by line as cell_wg runs it: the
-// line's block copied into a one-buffer "LDS" by every thread's share of voxel_stage, the
-// rest of the buffer NaN (a lookup outside the staged entries would show in the output),
-// then each thread's line.
+// of n entries, row tile r0, batch item b), line by line as cell_wg runs it: the line's
+// block (at its 64-line chunk's pitch) copied into a one-slot "LDS" by every thread's
+// share of voxel_stage, the rest of the slot NaN (a lookup outside the staged entries
+// would show in the output), then each thread's line.
 template <int OPK, int GEO, int R, int SPAN>
-void run_cell_quad(const Plan3d &p, const CellArgs &a, const int *items, int n, const QuadLine *ql, int q, int b,
-                   const float2 *WX, const float2 *WY, float *out)
+void run_cell_quad(const Plan3d &p, const CellArgs &a, const int *items, int n, const QuadLine *ql, int q, int r0,
+                   int b, const float2 *WX, const float2 *WY, float *out)
 {
     const int ncg = ceil_div(a.nu, kWave);
     const float nan = std::numeric_limits<float>::quiet_NaN();
     std::vector<float2> buf(kCellLds);
     std::vector<CellLane<R>> st(kBlock);
-    for (int r0 = 0; r0 < a.nv; r0 += R) {
-        int v[kWavesPerBlock];
-        bool on[kBlock];
-        for (int w = 0; w < kWavesPerBlock; ++w) {
-            const int idx = q * kWavesPerBlock + w;
-            const int item = items[idx < n ? idx : q * kWavesPerBlock];
-            v[w] = item / ncg;
-            for (int l = 0; l < kWave; ++l) {
-                const int c = (item - v[w] * ncg) * kWave + l;
-                on[w * kWave + l] = idx < n && c < a.nu;
-                if (on[w * kWave + l]) cell_init<R>(st[w * kWave + l], p.d_col + (int64_t)v[w] * (a.nu + 1), a, c, r0);
-            }
+    int v[kWavesPerBlock];
+    bool on[kBlock];
+    for (int w = 0; w < kWavesPerBlock; ++w) {
+        const int idx = q * kWavesPerBlock + w;
+        const int item = items[idx < n ? idx : q * kWavesPerBlock];
+        v[w] = item / ncg;
+        for (int l = 0; l < kWave; ++l) {
+            const int c = (item - v[w] * ncg) * kWave + l;
+            on[w * kWave + l] = idx < n && c < a.nu;
+            if (on[w * kWave + l]) cell_init<R>(st[w * kWave + l], p.d_col + (int64_t)v[w] * (a.nu + 1), a, c, r0);
         }
-        const ViewHdr &h0 = p.d_hdr[v[0]];
-        const float2 *W = cell_table(WX, WY, h0, a, b);
-        const int plane = h0.nline * h0.nalong;
-        for (int L = 0; L < h0.nline; ++L) {
-            const VoxBox bx = cell_box<R>(ql[L], a, r0);
-            std::fill(buf.begin(), buf.end(), float2{nan, nan});
-            for (int tid = 0; tid < kBlock; ++tid) voxel_stage(W + (int64_t)L * h0.nalong, bx, buf.data(), plane, tid, kBlock);
-            for (int t = 0; t < kBlock; ++t) {
-                if (!on[t]) continue;
-                const int w = t / kWave;
-                const int item = items[q * kWavesPerBlock + w];
-                const ViewHdr &h = p.d_hdr[v[w]];
-                const float4 lc = p.d_lineC[(int64_t)v[w] * a.maxline + L];
-                cell_line_staged<OPK, GEO, R, SPAN>(st[t], h, a, lc, L, W, p.d_cgrp[item].wmax,
-                                                    reinterpret_cast<const char *>(buf.data()), bx);
-                if (bx.nc > 0) g_cell_lds_lines.fetch_add(1);
-            }
-        }
-        for (int t = 0; t < kBlock; ++t)
-            if (on[t]) {
-                const int w = t / kWave;
-                const int item = items[q * kWavesPerBlock + w];
-                cell_store<OPK, GEO, R>(st[t], p.d_hdr[v[w]], a, out, (item - v[w] * ncg) * kWave + t % kWave, r0, v[w],
-                                        b);
-            }
     }
+    const ViewHdr &h0 = p.d_hdr[v[0]];
+    const float2 *W = cell_table(WX, WY, h0, a, b);
+    const int plane = h0.nline * h0.nalong;
+    std::vector<int> pitch((size_t)ceil_div(h0.nline, kCellChunk));
+    cell_chunk_pitches(ql, h0.nline, pitch.data());
+    for (int L = 0; L < h0.nline; ++L) {
+        const VoxBox bx = cell_box<R>(ql[L], a, r0, pitch[L / kCellChunk], h0.nalong);
+        std::fill(buf.begin(), buf.end(), float2{nan, nan});
+        for (int tid = 0; tid < kBlock; ++tid) voxel_stage(W + (int64_t)L * h0.nalong, bx, buf.data(), plane, tid, kBlock);
+        for (int t = 0; t < kBlock; ++t) {
+            if (!on[t]) continue;
+            const int w = t / kWave;
+            const int item = items[q * kWavesPerBlock + w];
+            const ViewHdr &h = p.d_hdr[v[w]];
+            const float4 lc = p.d_lineC[(int64_t)v[w] * a.maxline + L];
+            cell_line_staged<OPK, GEO, R, SPAN>(st[t], h, a, lc, L, W, p.d_cgrp[item].wmax,
+                                                reinterpret_cast<const char *>(buf.data()), bx);
+            if (bx.nc > 0) g_cell_lds_lines.fetch_add(1);
+        }
+    }
+    for (int t = 0; t < kBlock; ++t)
+        if (on[t]) {
+            const int w = t / kWave;
+            const int item = items[q * kWavesPerBlock + w];
+            cell_store<OPK, GEO, R>(st[t], p.d_hdr[v[w]], a, out, (item - v[w] * ncg) * kWave + t % kWave, r0, v[w], b);
+        }
 }
 
-// The cell gather as launched with staging (tiles of 4 rows and more): the workgroups of
-// classes 2 and 3 that the plan marks staged, the others and class 0 directly (k_cell).
+// The cell gather as launched with staging (tiles of 4, 8 or 16 rows): the (group quad,
+// row tile) pairs of classes 2 and 3 that the plan marks staged, the others and class 0
+// directly (k_cell).
 template <int OPK, int GEO, int R>
 void run_cell_staged(const Plan3d &p, const float2 *WX, const float2 *WY, float *out, int nb, int nthreads)
 {
     const CellArgs a = make_cell_args(p, nb);
     const int ncg = ceil_div(a.nu, kWave);
+    const int ntile = cell_tile_count(a.nv);
     struct Job {
         int k, q, b;
     };
@@ -148,18 +149,21 @@ void run_cell_staged(const Plan3d &p, const float2 *WX, const float2 *WY, float 
                 const int start = jb.k == 0 ? 0 : jb.k == 1 ? p.ncls[0] : p.ncls[0] + p.ncls[1];
                 const int *items = p.d_citems + start;
                 const int n = p.ncls[jb.k];
-                const QuadLine *ql = jb.k < 2 ? p.d_cquad + (int64_t)(p.cquad_off[jb.k] + jb.q) * a.maxline : nullptr;
-                if (ql && R >= 4 && p.d_cqstage[p.cquad_off[jb.k] + jb.q]) {
-                    if (jb.b == 0) g_cell_wg_staged.fetch_add(1);
-                    if (jb.k == 0) run_cell_quad<OPK, GEO, R, 2>(p, a, items, n, ql, jb.q, jb.b, WX, WY, out);
-                    else run_cell_quad<OPK, GEO, R, 3>(p, a, items, n, ql, jb.q, jb.b, WX, WY, out);
-                    continue;
-                }
-                if (ql && jb.b == 0) g_cell_wg_direct.fetch_add(1);
-                for (int idx = jb.q * kWavesPerBlock; idx < std::min(n, (jb.q + 1) * kWavesPerBlock); ++idx) {
-                    const int v = items[idx] / ncg, cg = items[idx] - v * ncg;
-                    const float wg = p.d_cgrp[items[idx]].wmax;
-                    for (int r0 = 0; r0 < a.nv; r0 += R)
+                const bool can = jb.k < 2 && (R == 4 || R == 8 || R == 16);
+                const QuadLine *ql = can ? p.d_cquad + (int64_t)(p.cquad_off[jb.k] + jb.q) * a.maxline : nullptr;
+                const int *flag = can ? p.d_cqstage + (int64_t)(p.cquad_off[jb.k] + jb.q) * ntile + cell_tile_offset(a.nv, R)
+                                      : nullptr;
+                for (int r0 = 0; r0 < a.nv; r0 += R) {
+                    if (flag && flag[r0 / R]) {
+                        if (jb.b == 0) g_cell_wg_staged.fetch_add(1);
+                        if (jb.k == 0) run_cell_quad<OPK, GEO, R, 2>(p, a, items, n, ql, jb.q, r0, jb.b, WX, WY, out);
+                        else run_cell_quad<OPK, GEO, R, 3>(p, a, items, n, ql, jb.q, r0, jb.b, WX, WY, out);
+                        continue;
+                    }
+                    if (flag && jb.b == 0) g_cell_wg_direct.fetch_add(1);
+                    for (int idx = jb.q * kWavesPerBlock; idx < std::min(n, (jb.q + 1) * kWavesPerBlock); ++idx) {
+                        const int v = items[idx] / ncg, cg = items[idx] - v * ncg;
+                        const float wg = p.d_cgrp[items[idx]].wmax;
                         for (int c = cg * kWave; c < std::min(a.nu, (cg + 1) * kWave); ++c) {
                             if (jb.k == 0)
                                 cell_body<OPK, GEO, R, 2, false>(p.d_hdr, p.d_lineC, p.d_col, WX, WY, out, a, c, r0, v,
@@ -171,6 +175,7 @@ void run_cell_staged(const Plan3d &p, const float2 *WX, const float2 *WY, float 
                                 cell_body<OPK, GEO, R, 0, false>(p.d_hdr, p.d_lineC, p.d_col, WX, WY, out, a, c, r0, v,
                                                                  jb.b, wg);
                         }
+                    }
                 }
             }
         });
@@ -479,12 +484,10 @@ int emu_view_spans(const int *vn, const float *vf, const int *gn, const float *g
 }
 
 // The staged cell gather's blocks for tiles of R rows (R2 for class 2, R3 for class 3),
-// over every (workgroup, row tile, line) with a column on the line: stats[0] blocks,
-// stats[1] over a slot's capacity, stats[2] workgroups staged, stats[3] workgroups
-// direct (the plan's staging flags), stats[4] sum of staged entries (pitch x slices) over the
-// staged workgroups, stats[5] the largest block (entries), stats[6] / stats[7] blocks
-// over 1024 / 1360 entries, stats[8..10] workgroups with a block over 1024 / 1360 / 1536,
-// stats[11] (workgroup, row tile) pairs, stats[12] of them with a block over a slot.
+// over every (workgroup, row tile, line) with a column on the line, at the chunk pitches:
+// stats[0] blocks, stats[1] over a slot, stats[2] (workgroup, row tile) pairs staged,
+// stats[3] pairs direct (the plan's flags), stats[4] entries staged (pitch x slices) in the
+// staged pairs, stats[5] the largest block (entries), stats[6] blocks over 1024 entries.
 int emu_cell_box_stats(const int *vn, const float *vf, const int *gn, const float *gf, const float *angles, int nview,
                        int R2, int R3, long long *stats)
 {
@@ -493,39 +496,45 @@ int emu_cell_box_stats(const int *vn, const float *vf, const int *gn, const floa
     const int rc = make_plan(p, t, vn, vf, gn, gf, angles, nview);
     if (rc) return rc;
     const CellArgs a = make_cell_args(p, 1);
-    for (int k = 0; k < 13; ++k) stats[k] = 0;
+    const int ntile = cell_tile_count(a.nv);
+    for (int k = 0; k < 7; ++k) stats[k] = 0;
+    std::vector<int> pitch((size_t)ceil_div(p.maxline, kCellChunk));
     for (int k = 0; k < 2; ++k) {
         const int R = k == 0 ? R2 : R3;
         for (int q = 0; q * kWavesPerBlock < p.ncls[k]; ++q) {
             const QuadLine *ql = t.cquad.data() + (size_t)(p.cquad_off[k] + q) * p.maxline;
-            const bool staged = t.cqstage[p.cquad_off[k] + q] != 0;
-            ++stats[staged ? 2 : 3];
-            bool over[3] = {false, false, false};
+            const int *flag = t.cqstage.data() + (size_t)(p.cquad_off[k] + q) * ntile + cell_tile_offset(a.nv, R);
+            const int v0 = t.citems[(k == 0 ? 0 : p.ncls[0]) + q * kWavesPerBlock] / ceil_div(a.nu, kWave);
+            const int nline = t.hdr[v0].nline, nalong = t.hdr[v0].nalong;
+            cell_chunk_pitches(ql, nline, pitch.data());
             for (int r0 = 0; r0 < a.nv; r0 += R) {
-                bool tile_over = false;
-                for (int L = 0; L < p.maxline; ++L) {
+                const bool staged = flag[r0 / R] != 0;
+                ++stats[staged ? 2 : 3];
+                for (int L = 0; L < nline; ++L) {
                     if (ql[L].nm <= 0) continue;
                     ++stats[0];
-                    const int big = std::numeric_limits<int>::max();
+                    const int pt = pitch[L / kCellChunk];
                     VoxBox bx;
-                    if (R == 4) bx = cell_box<4>(ql[L], a, r0, big);
-                    else if (R == 8) bx = cell_box<8>(ql[L], a, r0, big);
-                    else bx = cell_box<16>(ql[L], a, r0, big);
-                    const long long n = (long long)bx.pitch * bx.nr;
-                    if (n > kCellLds) ++stats[1];
-                    if (n > 1024) ++stats[6];
-                    if (n > 1360) ++stats[7];
-                    over[0] = over[0] || n > 1024;
-                    over[1] = over[1] || n > 1360;
-                    over[2] = over[2] || n > 1536;
-                    tile_over = tile_over || n > kCellLds;
+                    if (R == 4) bx = cell_box<4>(ql[L], a, r0, pt, nalong);
+                    else if (R == 8) bx = cell_box<8>(ql[L], a, r0, pt, nalong);
+                    else bx = cell_box<16>(ql[L], a, r0, pt, nalong);
+                    long long n = (long long)bx.pitch * bx.nr;
+                    if (bx.nc == 0) {  // over a slot: its size without the rounding margin
+                        const float h0 = fmaf((float)r0, a.dvv, a.v0), h1 = fmaf((float)(r0 + R), a.dvv, a.v0);
+                        const QuadLine &qq = ql[L];
+                        const float lo = std::min(std::min(qq.sg_lo * h0, qq.sg_hi * h0), std::min(qq.sg_lo * h1, qq.sg_hi * h1));
+                        const float hi = std::max(std::max(qq.sg_lo * h0, qq.sg_hi * h0), std::max(qq.sg_lo * h1, qq.sg_hi * h1));
+                        const int ns = (int)std::min(std::floor(hi + a.zoff), (float)a.nz) -
+                                       (int)std::max(std::floor(lo + a.zoff), 0.0f) + 1;
+                        n = (long long)ns * pt;
+                        ++stats[1];
+                    } else if (staged) {
+                        stats[4] += n;
+                    }
                     stats[5] = std::max(stats[5], n);
-                    if (staged && n <= kCellLds) stats[4] += n;
+                    if (n > 1024) ++stats[6];
                 }
-                ++stats[11];
-                stats[12] += tile_over;
             }
-            for (int i = 0; i < 3; ++i) stats[8 + i] += over[i];
         }
     }
     return XR_OK;

@@ -748,7 +748,10 @@ __host__ __device__ inline void cell_line(CellLane<R> &st, const ViewHdr &h, con
 #pragma unroll
         for (int q = 0; q <= R; ++q) {
             if (LDS) {
-                const unsigned off = mad24((unsigned)(ki[q] - bx.r0), (unsigned)bx.pitch * 8u, (unsigned)(m0 - bx.c0) * 8u);
+                // (ki - r0) pitch 8 + (m0 - c0) 8, the row base folded into the addend
+                // (modulo 2^32; the offset itself lies in the block)
+                const unsigned off = mad24((unsigned)ki[q], (unsigned)bx.pitch * 8u,
+                                           (unsigned)(m0 - bx.c0) * 8u - (unsigned)(bx.r0 * bx.pitch) * 8u);
                 e[q] = lds_pair(lds, off);
                 if (SPAN == 3) g[q] = lds_float2(lds, off + 16u);
                 continue;
@@ -878,15 +881,16 @@ __host__ __device__ inline void cell_body(const ViewHdr *__restrict__ hdr, const
     cell_store<OPK, GEO, R>(st, h, a, out, c, r0, v, b);
 }
 
-// The cell gather's LDS staging: kCellSlots slots of kCellLds float2 entries (8 KiB
-// each, 32 KiB per workgroup), one line per slot, lines staged kCellSlots - 1 ahead of
-// the line being gathered.  The 4 groups of a workgroup read blocks of 208 entries on
-// average (616 at most) of a line of the 256^2 x 64 cone-flat workload for 8 rows, 347
-// (1056) for 16 rows; 529 (1848) of the 512^2 x 256 cone-arc one for 8 rows, where 2.7 %
-// of the blocks exceed a slot (the workgroup reads those lines from the table).
-constexpr int kCellLds = 1024;
-constexpr int kCellSlots = 4;
+// The cell gather's LDS staging: kCellSlots slots of kCellLds float2 entries (12 KiB
+// each, 36 KiB per workgroup), one line per slot, lines staged kCellSlots - 1 ahead of
+// the line being gathered.  The 4 groups of a workgroup stage blocks of 230 entries on
+// average (616 at most) per line of the 256^2 x 64 cone-flat workload for 8 rows; 555
+// (1860) of the 512^2 x 256 cone-arc one, where 0.06 % of the blocks exceed a slot (their
+// (workgroup, row tile) pairs, 2 % with those mixing orientations, read the table).
+constexpr int kCellLds = 1536;
+constexpr int kCellSlots = 3;
 static_assert(kCellLds / 2 % kBlock == 0, "staging copies whole rounds of pairs");
+constexpr int kCellChunk = 64;  // lines per staging-pitch chunk (one per lane of a wave)
 
 // Copy rounds (global-to-LDS loads per thread) of a staged block (voxel_stage).
 __host__ __device__ inline int stage_rounds(const VoxBox &bx)
@@ -894,17 +898,26 @@ __host__ __device__ inline int stage_rounds(const VoxBox &bx)
     return (bx.nr * (bx.pitch >> 1) + kBlock - 1) / kBlock;
 }
 
+// Staging pitch of line L's block: its QuadLine's pixel count in whole pairs (0: nothing
+// to stage).  A workgroup stages every line of a 64-line chunk at the chunk's largest
+// pitch, so that each thread's share of the copies sits at the same place in every block
+// of the chunk.
+__host__ __device__ inline int cell_line_pitch(const QuadLine &q) { return q.nm > 0 ? (q.nm + 1) & ~1 : 0; }
+
 // The block of line L's z-prefix table a cell-gather workgroup stages for its row
-// boundaries r0 .. r0+R: the QuadLine's pixels, and the slices over the QuadLine's range
-// of s = sigma * lc.z at the two extreme heights (z = s h + zoff is linear in h for each
-// column, so its extremes over the rows lie at the end rows).  A thread evaluates z as
-// fmaf(sigma * (v0 + r dvv), lc.z, zoff) and this bound as s h + zoff, both within a few
-// units in the last place of |s h| + |zoff| of the exact value; the bound is widened by
-// 1e-4 (|s h| + |zoff|) + 0.01 slices, far beyond that.  The block so holds every lookup
-// of the workgroup's threads.  nc = 0: nothing staged (no column reaches the line, or
-// the block exceeds cap: the workgroup then reads that line from the table).
+// boundaries r0 .. r0+R at pitch (>= the line's own pitch): pixels from the QuadLine's
+// first, moved left where the block would reach more than three entries past the line's
+// end (the table's last line is followed by kVolPad pad entries; a QuadLine covers at most
+// Nalong + 2 pixels, so the block stays at or after the line's start); slices over the
+// QuadLine's range of s = sigma * lc.z at the two extreme heights (z = s h + zoff is
+// linear in h for each column, so its extremes over the rows lie at the end rows).  A
+// thread evaluates z as fmaf(sigma * (v0 + r dvv), lc.z, zoff) and this bound as s h +
+// zoff, both within a few units in the last place of |s h| + |zoff| of the exact value;
+// the bound is widened by 1e-4 (|s h| + |zoff|) + 0.01 slices, far beyond that.  The
+// block so holds every lookup of the workgroup's threads.  nc = 0: nothing staged (no
+// column reaches the line, or the block exceeds a slot).
 template <int R>
-__host__ __device__ inline VoxBox cell_box(const QuadLine &q, const CellArgs &a, int r0, int cap = kCellLds)
+__host__ __device__ inline VoxBox cell_box(const QuadLine &q, const CellArgs &a, int r0, int pitch, int nalong)
 {
     VoxBox bx{0, 0, 0, 0, 0, 0};
     if (q.nm <= 0) return bx;
@@ -916,34 +929,45 @@ __host__ __device__ inline VoxBox cell_box(const QuadLine &q, const CellArgs &a,
     const float fnz = (float)a.nz;
     const int s0 = (int)clampf(floorf(plo + a.zoff - tol), 0.0f, fnz);
     const int s1 = (int)clampf(floorf(phi + a.zoff + tol), 0.0f, fnz);
-    const int pitch = (q.nm + 1) & ~1;
     const int ns = s1 - s0 + 1;
-    if (ns * pitch > cap) return bx;
-    bx.c0 = q.m_lo;
-    bx.nc = q.nm;
+    if (ns * pitch > kCellLds) return bx;
+    bx.c0 = imin(q.m_lo, nalong + 3 - pitch);
+    bx.nc = pitch;
     bx.r0 = s0;
     bx.nr = ns;
     bx.pitch = pitch;
     return bx;
 }
 
+// Chunk pitches of a workgroup's lines (host): pitch[L / 64] for lines 0 .. nline-1.
+inline void cell_chunk_pitches(const QuadLine *ql, int nline, int *pitch)
+{
+    for (int c = 0; c * kCellChunk < nline; ++c) {
+        int p = 0;
+        for (int L = c * kCellChunk; L < imin(nline, (c + 1) * kCellChunk); ++L) p = imax(p, cell_line_pitch(ql[L]));
+        pitch[c] = p;
+    }
+}
+
+// Row tiles of R rows (4, 8, 16) of all three sizes, as the plan lists the cell gather's
+// staging flags per workgroup: tiles of 4 rows first, then 8, then 16.
+__host__ __device__ inline int cell_tile_count(int nv) { return ceil_div(nv, 4) + ceil_div(nv, 8) + ceil_div(nv, 16); }
+__host__ __device__ inline int cell_tile_offset(int nv, int R)
+{
+    return R == 4 ? 0 : R == 8 ? ceil_div(nv, 4) : ceil_div(nv, 4) + ceil_div(nv, 8);
+}
+
 // Line L of a cell-gather thread in a group of span class SPAN (2 or 3), each line on
-// the path cell_body gives it, with the lookups from the staged block bx at lds, or from
-// the table W where nothing was staged (bx.nc = 0: the block exceeds a slot, or no
-// column reaches the line; the same for the whole workgroup).
+// the path cell_body gives it, with the lookups from the staged block bx at lds (in a
+// staged workgroup every line with a column on it has its block: the plan's flags).
 template <int OPK, int GEO, int R, int SPAN>
 __host__ __device__ inline void cell_line_staged(CellLane<R> &st, const ViewHdr &h, const CellArgs &a,
                                                  const float4 &lc, int L, const float2 *__restrict__ W, float wgrp,
                                                  const char *lds, const VoxBox &bx)
 {
     static_assert(SPAN == 2 || SPAN == 3, "staged lines: span classes 2 and 3");
-    if (SPAN == 2 || wgrp * lc.y < 0.99f) {
-        if (bx.nc > 0) cell_line<OPK, GEO, R, 2, false, true>(st, h, a, lc, L, W, lds, bx);
-        else cell_line<OPK, GEO, R, 2, false>(st, h, a, lc, L, W);
-    } else {
-        if (bx.nc > 0) cell_line<OPK, GEO, R, 3, false, true>(st, h, a, lc, L, W, lds, bx);
-        else cell_line<OPK, GEO, R, 3, false>(st, h, a, lc, L, W);
-    }
+    if (SPAN == 2 || wgrp * lc.y < 0.99f) cell_line<OPK, GEO, R, 2, false, true>(st, h, a, lc, L, W, lds, bx);
+    else cell_line<OPK, GEO, R, 3, false, true>(st, h, a, lc, L, W, lds, bx);
 }
 
 // ---------------------------------------------------------------- voxel gather
@@ -1329,7 +1353,7 @@ __device__ inline void voxel_wg(const ViewHdr *__restrict__ hdr, const float4 *_
 }
 
 // Waits until at most n (0..4) of this wave's vector memory loads are outstanding (they
-// complete in order).
+// complete in order); n above 4 waits as for 4 (more than needed, never less).
 __device__ inline void wait_vmem(int n)
 {
     // s_waitcnt immediate: vmcnt in bits 3:0, expcnt (7: no wait) 6:4, lgkmcnt (15) 11:8
@@ -1357,43 +1381,56 @@ __device__ inline void wg_barrier()
 
 // Cell gather of one workgroup with its z-prefix lookups served from LDS.  The 4 waves
 // take 4 groups of span class SPAN (2 or 3) on rows r0..r0+R-1 (see k_cell); ql: the
-// workgroup's QuadLine per line.  Line L's block (cell_box) is staged into slot L mod 4
-// three lines ahead of its gather: at line L each wave waits for its own copies of L
-// (the copies of L+1 and L+2, issued later, may stay in flight), the workgroup meets at
-// a barrier (everyone's copies of L have landed; everyone is done with line L-1, whose
-// slot L+3 then takes), and gathers L.  The 4 groups' views share one orientation (the
-// plan stages no other workgroup: cqstage), so the line count and the table are the same
-// for the 4 waves.  Every thread of the workgroup
-// takes part in staging and barriers; active: the thread has a column (its wave a group,
-// its lane a column below Nu).  The lines go four at a time so that each slot is its
-// own array at every access.
+// workgroup's QuadLine per line.  Line L's block (cell_box) is staged into slot L mod 3
+// two lines ahead of its gather: at line L each wave waits for its own copies of L (the
+// copies of L+1, issued later, may stay in flight), the workgroup meets at a barrier
+// (everyone's copies of L have landed; everyone is done with line L-1, whose slot L+2
+// then takes), and gathers L.  The plan stages a workgroup only where its 4 views share
+// one orientation (so the line count and the table are the same for the 4 waves) and
+// every block of its row tile fits a slot.  Every thread of the workgroup takes part in
+// staging and barriers; active: the thread has a column (its wave a group, its lane a
+// column below Nu).  The lines go three at a time so that each slot is its own array at
+// every access.
+//
+// Per 64-line chunk each wave evaluates the chunk's blocks over its lanes (line base + l
+// on lane l, read back per line with readlane) at the chunk's pitch, and each thread the
+// places of its copies in a block of that pitch: pair p = tid + 256 s at row p / (pitch /
+// 2), pair p mod (pitch / 2), as voxel_stage lays them out.
 template <int OPK, int GEO, int R, int SPAN>
 __device__ inline void cell_wg(const ViewHdr *__restrict__ hdr, const float4 *__restrict__ lineC,
                                const float4 *__restrict__ colt, const QuadLine *__restrict__ ql,
                                const float2 *__restrict__ WX, const float2 *__restrict__ WY, float *__restrict__ out,
                                const CellArgs &a, int c, int r0, int v, int b, float wgrp, bool active,
-                               float2 *__restrict__ s0, float2 *__restrict__ s1, float2 *__restrict__ s2,
-                               float2 *__restrict__ s3)
+                               float2 *__restrict__ s0, float2 *__restrict__ s1, float2 *__restrict__ s2)
 {
-    static_assert(kCellSlots == 4, "four slots, lines staged three ahead");
+    static_assert(kCellSlots == 3 && kCellChunk == kWave, "three slots, lines staged two ahead");
+    constexpr int kRounds = kCellLds / 2 / kBlock;  // copy rounds of a full slot
     const ViewHdr h = ro_load(hdr + v);
     CellLane<R> st;
     cell_init<R>(st, colt + (int64_t)v * (a.nu + 1), a, active ? c : 0, r0);
     const float2 *W = cell_table(WX, WY, h, a, b);
     const float4 *lt = lineC + (int64_t)v * a.maxline;
     const int nline = wave_uniform(h.nline), nalong = wave_uniform(h.nalong);
-    const int plane = nline * nalong;
+    const int tid = threadIdx.x, lane = tid & (kWave - 1);
     const VoxBox none{0, 0, 0, 0, 0, 0};
-    // The blocks of 64 lines at a time, line base + l on lane l (cell_box evaluated over
-    // the wave's lanes), read back per line with readlane.  Chunk bases are multiples of 64.
-    const int lane = threadIdx.x & (kWave - 1);
-    int kc0 = 0, kr0 = 0, knr = 0, kpitch = 0;
+    int kc0 = 0, kr0 = 0, knr = 0, kpitch = 0;  // lane l: line base + l's block; the chunk's pitch
+    unsigned cplace[kRounds];  // this thread's copies: row << 16 | pair offset in bytes (< 2^14 each)
     auto chunk = [&](int base) {
-        const VoxBox bx = cell_box<R>(g_load(ql + imin(base + lane, nline - 1)), a, r0);
+        const QuadLine q = g_load(ql + imin(base + lane, nline - 1));
+        int pm = base + lane < nline ? cell_line_pitch(q) : 0;
+#pragma unroll
+        for (int m = kWave / 2; m > 0; m >>= 1) pm = imax(pm, __shfl_xor(pm, m));
+        kpitch = wave_uniform(pm);
+        const VoxBox bx = cell_box<R>(q, a, r0, kpitch, nalong);
         kc0 = bx.c0;
         kr0 = bx.r0;
         knr = bx.nr;
-        kpitch = bx.pitch;
+        const int half = imax(kpitch >> 1, 1);
+#pragma unroll
+        for (int k = 0; k < kRounds; ++k) {
+            const int p = tid + k * kBlock, r = p / half;
+            cplace[k] = (unsigned)r << 16 | (unsigned)(p - r * half) * 16u;
+        }
     };
     auto box_of = [&](int j) {
         const int l = j & (kWave - 1);
@@ -1401,48 +1438,57 @@ __device__ inline void cell_wg(const ViewHdr *__restrict__ hdr, const float4 *__
         bx.c0 = __builtin_amdgcn_readlane(kc0, l);
         bx.r0 = __builtin_amdgcn_readlane(kr0, l);
         bx.nr = __builtin_amdgcn_readlane(knr, l);
-        bx.pitch = __builtin_amdgcn_readlane(kpitch, l);
-        bx.nc = bx.pitch;  // > 0 exactly when the line is staged
+        bx.pitch = bx.nr > 0 ? kpitch : 0;
+        bx.nc = bx.pitch;
         bx.skip = 0;
         return bx;
+    };
+    // the copies of line L's block: as voxel_stage, with this thread's places from chunk()
+    auto stage = [&](int L, const VoxBox &bx, float2 *slot) {
+        const int E2 = bx.nr * (bx.pitch >> 1);
+        const char *Wl = reinterpret_cast<const char *>(W + (int64_t)L * nalong);
+        const unsigned base = (unsigned)bx.r0 * a.plane8 + (unsigned)bx.c0 * 8u;
+#pragma unroll
+        for (int k = 0; k < kRounds; ++k) {
+            if (k * kBlock >= E2) break;
+            const unsigned off =
+                tid + k * kBlock < E2 ? mad24(cplace[k] >> 16, a.plane8, base + (cplace[k] & 0xFFFFu)) : 0u;
+            float2 *wave_base = slot + 2 * (k * kBlock + (tid & ~(kWave - 1)));
+            __builtin_amdgcn_global_load_lds(Wl + off, (__attribute__((address_space(3))) void *)wave_base, 16, 0, 0);
+        }
     };
     chunk(0);
     VoxBox b0 = box_of(0);
     VoxBox b1 = nline > 1 ? box_of(1) : none;
-    VoxBox b2 = nline > 2 ? box_of(2) : none;
-    VoxBox b3 = none;
-    voxel_stage(W, b0, s0, plane, threadIdx.x, kBlock);
-    voxel_stage(W + nalong, b1, s1, plane, threadIdx.x, kBlock);
-    voxel_stage(W + 2 * (int64_t)nalong, b2, s2, plane, threadIdx.x, kBlock);
+    VoxBox b2 = none;
+    stage(0, b0, s0);
+    stage(1, b1, s1);
     // The line coefficients of L, read one step ahead: a step issues the next step's read
     // after its barrier, and it lands during its gather.
     float4 lc = ro_load(lt);
-    // line L from slot rd (block cur); n1, n2: the blocks of L+1, L+2; n3 (slot wr) is set to L+3's
-    auto step = [&](int L, const VoxBox &cur, const VoxBox &n1, const VoxBox &n2, VoxBox &n3, const float2 *rd,
-                    float2 *wr) {
-        wait_vmem(stage_rounds(n1) + stage_rounds(n2));
+    // line L from slot rd (block cur); n1: the block of L+1; n2 (slot wr) is set to L+2's
+    auto step = [&](int L, const VoxBox &cur, const VoxBox &n1, VoxBox &n2, const float2 *rd, float2 *wr) {
+        wait_vmem(stage_rounds(n1));
         wg_barrier();
         const float4 lc_next = ro_load(lt + imin(L + 1, nline - 1));
-        if (L + 3 < nline) {
-            if (((L + 3) & (kWave - 1)) == 0) chunk(L + 3);
-            n3 = box_of(L + 3);
+        if (L + 2 < nline) {
+            if (((L + 2) & (kCellChunk - 1)) == 0) chunk(L + 2);
+            n2 = box_of(L + 2);
         } else {
-            n3 = none;
+            n2 = none;
         }
-        voxel_stage(W + (int64_t)(L + 3) * nalong, n3, wr, plane, threadIdx.x, kBlock);
+        stage(L + 2, n2, wr);
         __builtin_amdgcn_sched_barrier(0);  // issue the copies before the gather
         if (active)
             cell_line_staged<OPK, GEO, R, SPAN>(st, h, a, lc, L, W, wgrp, reinterpret_cast<const char *>(rd), cur);
         lc = lc_next;
     };
-    for (int L = 0; L < nline; L += 4) {
-        step(L, b0, b1, b2, b3, s0, s3);
+    for (int L = 0; L < nline; L += 3) {
+        step(L, b0, b1, b2, s0, s2);
         if (L + 1 >= nline) break;
-        step(L + 1, b1, b2, b3, b0, s1, s0);
+        step(L + 1, b1, b2, b0, s1, s0);
         if (L + 2 >= nline) break;
-        step(L + 2, b2, b3, b0, b1, s2, s1);
-        if (L + 3 >= nline) break;
-        step(L + 3, b3, b0, b1, b2, s3, s2);
+        step(L + 2, b2, b0, b1, s2, s1);
     }
     if (active) cell_store<OPK, GEO, R>(st, h, a, out, c, r0, v, b);
 }

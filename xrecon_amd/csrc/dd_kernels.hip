@@ -230,9 +230,10 @@ __global__ __launch_bounds__(kBlock) void k_fdk_prefix(const ViewHdr *__restrict
 // Every wave of a launch but the last quad's has work whatever Nv is.  SPAN = the class (2, 3 or 0), so each
 // class runs with the registers of its own paths; SPAN = -1: every class in one launch,
 // each workgroup reading its group's class (small problems, where launches cost more
-// than registers).  STAGE (classes 2 and 3, 4 to 16 rows): a workgroup the plan marks
-// (cqstage) serves its lookups from LDS (cell_wg), with cquad the class's QuadLine table
-// (maxline entries per workgroup); the others read the table directly.
+// than registers).  STAGE (classes 2 and 3, 4 to 16 rows): a workgroup whose (group
+// quad, row tile) the plan marks (cqstage, cell_tile_count entries per quad) serves its
+// lookups from LDS (cell_wg), with cquad the class's QuadLine table (maxline entries per
+// quad); the others read the table directly.
 template <int OPK, int GEO, int R, int SPAN, bool BIG, bool STAGE>
 __global__ __launch_bounds__(kBlock, STAGE && R <= 8 ? 4 : 1) void k_cell(const ViewHdr *__restrict__ hdr, const float4 *__restrict__ lineC,
                                                  const float4 *__restrict__ colt, const CellGroup *__restrict__ cgrp,
@@ -261,18 +262,18 @@ __global__ __launch_bounds__(kBlock, STAGE && R <= 8 ? 4 : 1) void k_cell(const 
     if constexpr (STAGE) {
         static_assert((SPAN == 2 || SPAN == 3) && !BIG && (R == 4 || R == 8 || R == 16),
                       "staged cell gathers: span classes 2 and 3, 4 to 16 rows");
-        if (ro_load(cqstage + quad)) {  // the same for the whole workgroup
+        if (ro_load(cqstage + (int64_t)quad * cell_tile_count(a.nv) + cell_tile_offset(a.nv, R) + rt)) {
+            // (the same for the whole workgroup)
             const QuadLine *ql = cquad + (int64_t)quad * a.maxline;
             __shared__ __attribute__((aligned(16))) float2 s0[kCellLds];
             __shared__ __attribute__((aligned(16))) float2 s1[kCellLds];
             __shared__ __attribute__((aligned(16))) float2 s2[kCellLds];
-            __shared__ __attribute__((aligned(16))) float2 s3[kCellLds];
             const bool on = idx < nitems;  // a wave past the list stages and waits with the others
             const int item = items[on ? idx : quad * kWavesPerBlock];
             const int v = item / ncg, cg = item - v * ncg;
             const int c = cg * kWave + lane;
             cell_wg<OPK, GEO, R, SPAN>(hdr, lineC, colt, ql, WX, WY, out, a, c, rt * R, v, b, ro_load(cgrp + item).wmax,
-                                       on && c < a.nu, s0, s1, s2, s3);
+                                       on && c < a.nu, s0, s1, s2);
             return;
         }
     }
@@ -392,8 +393,9 @@ void launch_cell(const Plan3d &p, int R, bool stage, const float2 *WX, const flo
     const bool st = stage && p.d_cquad;
     const QuadLine *q2 = st ? p.d_cquad + (int64_t)p.cquad_off[0] * p.maxline : nullptr;
     const QuadLine *q3 = st ? p.d_cquad + (int64_t)p.cquad_off[1] * p.maxline : nullptr;
-    const int *f2 = st ? p.d_cqstage + p.cquad_off[0] : nullptr;
-    const int *f3 = st ? p.d_cqstage + p.cquad_off[1] : nullptr;
+    const int nt = cell_tile_count(p.geo.nv);
+    const int *f2 = st ? p.d_cqstage + (int64_t)p.cquad_off[0] * nt : nullptr;
+    const int *f3 = st ? p.d_cqstage + (int64_t)p.cquad_off[1] * nt : nullptr;
     launch_cell_class<OPK, GEO, 2, false>(p, a, R, p.d_citems, p.ncls[0], q2, f2, WX, WY, out, nbatch, s);
     launch_cell_class<OPK, GEO, 3, false>(p, a, Rw, p.d_citems + p.ncls[0], p.ncls[1], q3, f3, WX, WY, out, nbatch,
                                           s);

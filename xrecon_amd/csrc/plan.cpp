@@ -323,7 +323,10 @@ int build_tables(Plan3d &p, const float *angles, PlanTables &t, std::string &err
             nq += (int)ceil_div(t.ncls[k], kWavesPerBlock);
         }
         t.cquad.assign((size_t)nq * ML, QuadLine{0, 0, 0.0f, 0.0f});
-        t.cqstage.assign((size_t)nq, 0);
+        const int ntile = cell_tile_count(nv);
+        t.cqstage.assign((size_t)nq * ntile, 0);
+        const CellArgs ca = make_cell_args(p, 1);
+        std::vector<int> pitch((size_t)ceil_div(ML, kCellChunk));
         int start = 0;
         for (int k = 0; k < 2; ++k) {
             const int extra = k == 0 ? 1 : 2;  // entries a fixed-span read takes past its first pixel
@@ -354,7 +357,23 @@ int build_tables(Plan3d &p, const float *angles, PlanTables &t, std::string &err
                     }
                     if (mhi >= mlo) ql[L] = QuadLine{mlo, mhi - mlo + 1, slo, shi};
                 }
-                t.cqstage[(size_t)p.cquad_off[k] + q] = 1;
+                // Staged (group quad, row tile) pairs for tiles of 4, 8 and 16 rows: those
+                // whose every block fits a slot, evaluated as the kernel evaluates them.
+                cell_chunk_pitches(ql, h0.nline, pitch.data());
+                int *flag = t.cqstage.data() + (size_t)(p.cquad_off[k] + q) * ntile;
+                for (int R = 4; R <= 16; R *= 2)
+                    for (int rt = 0; rt * R < nv; ++rt) {
+                        bool fits = true;
+                        for (int L = 0; fits && L < h0.nline; ++L) {
+                            if (ql[L].nm <= 0) continue;
+                            const int pt = pitch[L / kCellChunk];
+                            const VoxBox bx = R == 4    ? cell_box<4>(ql[L], ca, rt * R, pt, h0.nalong)
+                                              : R == 8 ? cell_box<8>(ql[L], ca, rt * R, pt, h0.nalong)
+                                                       : cell_box<16>(ql[L], ca, rt * R, pt, h0.nalong);
+                            fits = bx.nc > 0;
+                        }
+                        flag[cell_tile_offset(nv, R) + rt] = fits ? 1 : 0;
+                    }
             }
             start += t.ncls[k];
         }

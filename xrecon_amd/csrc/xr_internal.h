@@ -116,8 +116,9 @@ struct Plan3d {
     int ncls[3] = {0, 0, 0};     // number of groups of span class 2, 3 and 0
     QuadLine *d_cquad = nullptr; // [workgroups of classes 2 and 3][maxline]: the cell gather's LDS staging
                                  // bounds; workgroup q of class k is entry cquad_off[k] + q
-    int *d_cqstage = nullptr;      // [workgroups of classes 2 and 3]: 1 for the workgroups the cell gather
-                                 // stages (their 4 views share one orientation), else 0
+    int *d_cqstage = nullptr;    // [workgroups of classes 2 and 3][cell_tile_count(Nv)]: 1 for the
+                                 // (group quad, row tile) pairs the cell gather stages (the 4 views share
+                                 // one orientation, every block fits a slot), per tile size 4, 8, 16
     int cquad_off[2] = {0, 0};
 };
 
