@@ -234,8 +234,12 @@ __global__ __launch_bounds__(kBlock) void k_fdk_prefix(const ViewHdr *__restrict
 // quad, row tile) the plan marks (cqstage, cell_tile_count entries per quad) serves its
 // lookups from LDS (cell_wg), with cquad the class's QuadLine table (maxline entries per
 // quad); the others read the table directly.
+// Staged kernels run at least 4 waves per SIMD at 8 rows and fewer (128 registers), 3 at
+// 16 rows (168): uncapped they take 130 and 177 (the 256^2 x 64 cone-flat A at 16 rows:
+// 3.31 ms uncapped, 2.72 ms capped; the 512^2 x 256 cone-arc A at 8 rows: 34.7 against
+// 30.4 ms per item).
 template <int OPK, int GEO, int R, int SPAN, bool BIG, bool STAGE>
-__global__ __launch_bounds__(kBlock, STAGE && R <= 8 ? 4 : 1) void k_cell(const ViewHdr *__restrict__ hdr, const float4 *__restrict__ lineC,
+__global__ __launch_bounds__(kBlock, STAGE ? (R <= 8 ? 4 : 3) : 1) void k_cell(const ViewHdr *__restrict__ hdr, const float4 *__restrict__ lineC,
                                                  const float4 *__restrict__ colt, const CellGroup *__restrict__ cgrp,
                                                  const int *__restrict__ items, int nitems,
                                                  const QuadLine *__restrict__ cquad, const int *__restrict__ cqstage,
