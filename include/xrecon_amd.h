@@ -122,11 +122,12 @@ int xr_plan3d_execute(const xr_plan3d *plan, int op, const float *in, float *out
 #define XR_VOXEL_HALVES 0x100
 /* tile_rows: detector rows per cell-gather thread, 1, 2, 4, 8 or 16 (0 = the plan's
  * default).  With 4 rows and more a cell gather may stage, per line, the block of the
- * z-prefix table each workgroup looks up in LDS (three lines ahead of its use) instead of
+ * z-prefix table each workgroup looks up in LDS (two lines ahead of its use) instead of
  * reading every lookup from device memory; the results are the same bit for bit.  By
  * default it stages where the z-prefix table of one batch item exceeds 128 MiB (its
  * lookups then miss the last-level caches most); tile_rows | XR_CELL_DIRECT never
- * stages, tile_rows | XR_CELL_STAGED always does (the autotuner tries both). */
+ * stages, tile_rows | XR_CELL_STAGED stages wherever it applies (the autotuner tries
+ * both).  Staging applies to the kernel launches of at least 4096 workgroups. */
 #define XR_CELL_DIRECT 0x100
 #define XR_CELL_STAGED 0x200
 int xr_plan3d_execute_phase(const xr_plan3d *plan, int op, int phase, const float *in, float *out,

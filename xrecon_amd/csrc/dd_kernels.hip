@@ -377,6 +377,7 @@ void launch_cell_class(const Plan3d &p, const CellArgs &a, int R, const int *ite
 // fewer than kCellSplitMin workgroups (2D images, small volumes) take one launch for
 // every class instead.
 constexpr int64_t kCellSplitMin = 4096;
+constexpr int64_t kCellStageMin = 4096;
 
 template <int OPK, int GEO>
 void launch_cell(const Plan3d &p, int R, bool stage, const float2 *WX, const float2 *WY, float *out, int nbatch,
@@ -394,12 +395,16 @@ void launch_cell(const Plan3d &p, int R, bool stage, const float2 *WX, const flo
         return;
     }
     const int Rw = R > 8 ? 8 : R;
-    const bool st = stage && p.d_cquad;
-    const QuadLine *q2 = st ? p.d_cquad + (int64_t)p.cquad_off[0] * p.maxline : nullptr;
-    const QuadLine *q3 = st ? p.d_cquad + (int64_t)p.cquad_off[1] * p.maxline : nullptr;
+    // A class launch of fewer than kCellStageMin workgroups reads directly: too few
+    // workgroups per CU to hide the staging's per-line barrier (the 256^2 x 64 cone-flat
+    // three-pixel class, 1344 workgroups: 0.40 ms staged against 0.31 ms direct).
+    const bool st2 = stage && p.d_cquad && cell_blocks(a, R, p.ncls[0], nbatch) >= kCellStageMin;
+    const bool st3 = stage && p.d_cquad && cell_blocks(a, Rw, p.ncls[1], nbatch) >= kCellStageMin;
+    const QuadLine *q2 = st2 ? p.d_cquad + (int64_t)p.cquad_off[0] * p.maxline : nullptr;
+    const QuadLine *q3 = st3 ? p.d_cquad + (int64_t)p.cquad_off[1] * p.maxline : nullptr;
     const int nt = cell_tile_count(p.geo.nv);
-    const int *f2 = st ? p.d_cqstage + (int64_t)p.cquad_off[0] * nt : nullptr;
-    const int *f3 = st ? p.d_cqstage + (int64_t)p.cquad_off[1] * nt : nullptr;
+    const int *f2 = st2 ? p.d_cqstage + (int64_t)p.cquad_off[0] * nt : nullptr;
+    const int *f3 = st3 ? p.d_cqstage + (int64_t)p.cquad_off[1] * nt : nullptr;
     launch_cell_class<OPK, GEO, 2, false>(p, a, R, p.d_citems, p.ncls[0], q2, f2, WX, WY, out, nbatch, s);
     launch_cell_class<OPK, GEO, 3, false>(p, a, Rw, p.d_citems + p.ncls[0], p.ncls[1], q3, f3, WX, WY, out, nbatch,
                                           s);
