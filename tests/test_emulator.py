@@ -249,6 +249,21 @@ def test_cell_staging_blocks_of_the_workloads(case):
     assert st[3] / (st[2] + st[3]) < 0.05
 
 
+def test_prefix_passes_over_several_chunks():
+    """Columns of 70 entries (three load chunks of the prefix passes, above the register
+    path's 64): the z-prefix (A, B_W^T) and row-prefix (B, A^T) tables give the oracle's
+    results."""
+    opts = opts3d(12, 10, 70, 20, 70, 1.0, 1.0, 0.5, 1.0, 0.5, 500.0, 1000.0, 1, 0.1, -0.1, 0.05, 0.1, 0.2)
+    V = 6
+    ang = angles_with_ties(V)
+    vol, sino = shapes3d(opts, 1, V)
+    rng = np.random.default_rng(41)
+    for op in ("A", "BWT", "B", "AT"):
+        x = rng.random(vol if op in VOLUME_IN else sino).astype(np.float32)
+        want = oracle.op3d(op, x.astype(np.float64), ang, opts)
+        assert rel_l2(emu_op3d(op, x, ang, opts), want) <= 1e-5, op
+
+
 @pytest.mark.parametrize("geo", ["flat", "arc", "par"])
 def test_voxel_gather_skips_views_off_the_detector(geo):
     """A tall, wide volume seen by a short, narrow detector: most (tile, slice chunk)

@@ -453,6 +453,10 @@ __host__ __device__ inline float fdk_weight(float sx, float sy, float px, float 
 
 // ---------------------------------------------------------------- prefix tables
 
+// The prefix passes read a column kPrefixChunk entries at a time into registers, so that
+// a chunk's loads are in flight together; the sums still run over the entries in order.
+constexpr int kPrefixChunk = 32;
+
 // z-prefix of one volume column (b, e = j*nx + i) in the x-driven line orientation:
 // W[b][k][j][i] = (J_k, f_k), J_k = sum_{k'<k} f[k'] - total/2 (FP64 accumulation).
 __host__ __device__ inline void vol_prefix_body(const float *__restrict__ f, float2 *__restrict__ W, int64_t plane,
@@ -461,12 +465,25 @@ __host__ __device__ inline void vol_prefix_body(const float *__restrict__ f, flo
     const float *src = f + b * plane * nz + e;
     float2 *dst = W + b * plane * (nz + 1) + e;
     double total = 0.0;
-    for (int k = 0; k < nz; ++k) total += (double)src[(int64_t)k * plane];
+    for (int k0 = 0; k0 < nz; k0 += kPrefixChunk) {
+        float v[kPrefixChunk];
+#pragma unroll
+        for (int k = 0; k < kPrefixChunk; ++k) v[k] = k0 + k < nz ? src[(int64_t)(k0 + k) * plane] : 0.0f;
+#pragma unroll
+        for (int k = 0; k < kPrefixChunk; ++k)
+            if (k0 + k < nz) total += (double)v[k];
+    }
     double run = -0.5 * total;
-    for (int k = 0; k < nz; ++k) {
-        const float v = src[(int64_t)k * plane];
-        dst[(int64_t)k * plane] = make_float2((float)run, v);
-        run += (double)v;
+    for (int k0 = 0; k0 < nz; k0 += kPrefixChunk) {
+        float v[kPrefixChunk];
+#pragma unroll
+        for (int k = 0; k < kPrefixChunk; ++k) v[k] = k0 + k < nz ? src[(int64_t)(k0 + k) * plane] : 0.0f;
+#pragma unroll
+        for (int k = 0; k < kPrefixChunk; ++k)
+            if (k0 + k < nz) {
+                dst[(int64_t)(k0 + k) * plane] = make_float2((float)run, v[k]);
+                run += (double)v[k];
+            }
     }
     dst[(int64_t)nz * plane] = make_float2((float)run, 0.0f);
 }
@@ -504,17 +521,30 @@ __host__ __device__ inline void prefix_column(const ViewHdr &h, const float4 *__
         sg = cc.y;
     }
     double total = 0.0;
-    for (int r = 0; r < a.nv; ++r) {
-        float q = qof(r);
-        if (MODE == SINO_AT) q *= at_scale<GEO>(h, scale0, dbp, sg, a.v0, a.dvv, r);
-        total += (double)q;
+    for (int r0 = 0; r0 < a.nv; r0 += kPrefixChunk) {
+        float q[kPrefixChunk];
+#pragma unroll
+        for (int k = 0; k < kPrefixChunk; ++k) q[k] = r0 + k < a.nv ? qof(r0 + k) : 0.0f;
+#pragma unroll
+        for (int k = 0; k < kPrefixChunk; ++k)
+            if (r0 + k < a.nv) {
+                if (MODE == SINO_AT) q[k] *= at_scale<GEO>(h, scale0, dbp, sg, a.v0, a.dvv, r0 + k);
+                total += (double)q[k];
+            }
     }
     double run = -0.5 * total;
-    for (int r = 0; r < a.nv; ++r) {
-        float q = qof(r);
-        if (MODE == SINO_AT) q *= at_scale<GEO>(h, scale0, dbp, sg, a.v0, a.dvv, r);
-        dst[(int64_t)r * a.nu] = make_float2((float)(run - (double)r * (double)q), q);
-        run += (double)q;
+    for (int r0 = 0; r0 < a.nv; r0 += kPrefixChunk) {
+        float q[kPrefixChunk];
+#pragma unroll
+        for (int k = 0; k < kPrefixChunk; ++k) q[k] = r0 + k < a.nv ? qof(r0 + k) : 0.0f;
+#pragma unroll
+        for (int k = 0; k < kPrefixChunk; ++k)
+            if (r0 + k < a.nv) {
+                const int r = r0 + k;
+                if (MODE == SINO_AT) q[k] *= at_scale<GEO>(h, scale0, dbp, sg, a.v0, a.dvv, r);
+                dst[(int64_t)r * a.nu] = make_float2((float)(run - (double)r * (double)q[k]), q[k]);
+                run += (double)q[k];
+            }
     }
     dst[(int64_t)a.nv * a.nu] = make_float2((float)run, 0.0f);
 }
