@@ -86,3 +86,35 @@ def test_random_geometry_gpu_vs_oracle(cuda, seed):
             print(f"[gpu] random geometry {seed} {op} chunk {chunk:#x}", flush=True)
             plan.execute_into(k, xd, out, ws, s, rows, chunk, 0)
             assert rel_l2(out.cpu().numpy()[:, 0], want) <= 1e-5, (seed, op, chunk)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("seed", SEEDS)
+def test_random_geometry_gpu_staged_equals_direct(cuda, seed):
+    """The LDS-staged cell gather on the GPU against the direct one, bit for bit: a batch
+    large enough that the two-pixel class launches at least 4096 workgroups (staging
+    applies to such launches only), 8-row tiles."""
+    from xrecon_amd import _backend as be
+
+    opts, ang, rng = random_case(seed)
+    plan = be.native().Plan(opts, torch.from_numpy(ang), 0, False)
+    n2 = plan.cell_classes()[0]
+    nv = int(opts[4])
+    if n2 == 0:
+        pytest.skip("no two-pixel column groups in this geometry")
+    per_item = -(-n2 // 4) * -(-nv // 8)
+    nb = min(1024, -(-4096 // per_item))
+    vol, _ = shapes3d(opts, nb, ang.shape[0])
+    x = torch.from_numpy(rng.random(vol).astype(np.float32)).to(cuda).unsqueeze(1)
+    s = torch.cuda.current_stream().cuda_stream
+    for op in VOLUME_IN:
+        k = OP_ID[op]
+        ws = torch.empty(max(plan.workspace_bytes(k, nb), 1), dtype=torch.uint8, device=cuda)
+        print(f"[gpu] random geometry {seed} {op} staged vs direct, batch {nb}", flush=True)
+        outs = []
+        for rows in (8 | 0x200, 8 | 0x100):  # XR_CELL_STAGED, XR_CELL_DIRECT
+            out = plan.execute(k, x, s)
+            plan.execute_into(k, x, out, ws, s, rows, 0, 0)
+            outs.append(out)
+        torch.cuda.synchronize()
+        assert torch.equal(outs[0], outs[1]), (seed, op)
