@@ -627,7 +627,7 @@ def main():
         rows = (r_a, min(r_a, 8), min(r_a, 8))
         # LDS-staged classes 2 and 3 at 4 rows and more: as tuned (this table is below the
         # 128 MiB the library stages by default)
-        staged = bool(tune_a[0] & autotune.CELL_STAGED)
+        staged = bool(tune_a[0] & autotune.CELL_LDS)
         if -(-sum(ncls) // 4) * -(-wl["Nv"] // r_a) < 4096:  # small grids: one launch for all classes
             symbol = [f"k_cell<0, 1, {r_a}, -1, false, false>"]
         else:
@@ -692,7 +692,7 @@ def main():
                      "note": "FP32 vector-ALU roof (packed FP32 peak); no dense contraction, so no MFMA; "
                              "F = 4*nnz per launch; traffic = HBM bytes per launch (rocprofv3 FETCH_SIZE x2.0 "
                              "+ WRITE_SIZE x1.0, copy-kernel calibrated)"},
-        "tiles": {"A_tile_rows": tune_a[0] & 0xFF, "A_cell_staged": bool(tune_a[0] & autotune.CELL_STAGED),
+        "tiles": {"A_tile_rows": tune_a[0] & 0xFF, "A_cell_staged": bool(tune_a[0] & autotune.CELL_LDS),
                   "B_voxel_chunk": tune_b[1]},
         "kernels_ms": {"cell_gather_A": t_cell, "voxel_gather_B": t_vox,
                        "prefix_passes": ms_step - t_cell - t_vox},

@@ -126,10 +126,10 @@ int xr_plan3d_execute(const xr_plan3d *plan, int op, const float *in, float *out
  * reading every lookup from device memory; the results are the same bit for bit.  By
  * default it stages where the z-prefix table of one batch item exceeds 128 MiB (its
  * lookups then miss the last-level caches most); tile_rows | XR_CELL_DIRECT never
- * stages, tile_rows | XR_CELL_STAGED stages wherever it applies (the autotuner tries
+ * stages, tile_rows | XR_CELL_LDS stages wherever it applies (the autotuner tries
  * both).  Staging applies to the kernel launches of at least 4096 workgroups. */
 #define XR_CELL_DIRECT 0x100
-#define XR_CELL_STAGED 0x200
+#define XR_CELL_LDS 0x200
 int xr_plan3d_execute_phase(const xr_plan3d *plan, int op, int phase, const float *in, float *out,
                             int num_batch, void *workspace, size_t workspace_bytes, int tile_rows,
                             int voxel_chunk, void *stream);
@@ -184,7 +184,7 @@ size_t xr_batch3d_workspace_bytes(const xr_batch3d *batch, int op);
 /* in[i] / out[i]: instance i's device buffers in the layouts above.  `tile_rows`
  * and `voxel_chunk` (values as for xr_plan3d_execute_phase; 0 = default) apply to all
  * instances of the launch; the batched cell gather reads its lookups from device memory
- * (XR_CELL_DIRECT and XR_CELL_STAGED are accepted and change nothing). */
+ * (XR_CELL_DIRECT and XR_CELL_LDS are accepted and change nothing). */
 int xr_batch3d_execute(const xr_batch3d *batch, int op, const float *const *in, float *const *out,
                        void *workspace, size_t workspace_bytes, int tile_rows, int voxel_chunk,
                        void *stream);

@@ -142,7 +142,7 @@ def test_cell_tilings_bitwise_equal(cuda):
 
 
 XR_CELL_DIRECT = 0x100
-XR_CELL_STAGED = 0x200
+XR_CELL_LDS = 0x200
 
 # Staging geometries: three column groups per view (two- and three-pixel classes on the
 # flat panel), 720 views with the 45-degree ties, 4 batch items, so that every tiling
@@ -177,7 +177,7 @@ def test_cell_staged_equals_direct(cuda, geo):
         for R in (4, 8, 16):
             print(f"[gpu] staged vs direct {geo} {op} rows {R}", flush=True)
             outs = []
-            for flag in (XR_CELL_STAGED, XR_CELL_DIRECT, 0):
+            for flag in (XR_CELL_LDS, XR_CELL_DIRECT, 0):
                 out = torch.empty((nb, 1, *sino[1:]), device=cuda)
                 plan.execute_into(OPID[op], x, out, ws, s, R | flag, 0, 0)
                 outs.append(out)
@@ -246,7 +246,7 @@ def test_cfg2_staged_equals_direct(cfg2, cuda, rows):
     s = torch.cuda.current_stream().cuda_stream
     ws = torch.empty(plan.workspace_bytes(0, 1), dtype=torch.uint8, device=cuda)
     outs = []
-    for flag in (XR_CELL_STAGED, XR_CELL_DIRECT):
+    for flag in (XR_CELL_LDS, XR_CELL_DIRECT):
         out = torch.empty((1, 1, 720, 64, 512), device=cuda)
         plan.execute_into(0, x, out, ws, s, rows | flag, 0, 0)
         outs.append(out)
@@ -408,7 +408,7 @@ def test_invalid_tile_shapes_are_refused(cuda):
     for chunk in (2, 3, 32, 64, 8 | 0x100, 16 | 0x200, 0x100):
         with pytest.raises(RuntimeError, match="voxel_chunk must be"):
             plan.execute_into(2, y, out_v, ws, s, 0, chunk, 0)
-    for rows in (3, 32, 3 | XR_CELL_DIRECT, 8 | 0x400, 8 | XR_CELL_DIRECT | XR_CELL_STAGED):
+    for rows in (3, 32, 3 | XR_CELL_DIRECT, 8 | 0x400, 8 | XR_CELL_DIRECT | XR_CELL_LDS):
         with pytest.raises(RuntimeError, match="tile_rows must be"):
             plan.execute_into(0, x, out_s, ws, s, rows, 0, 0)
     proj = xrecon_amd.Projector3d((24, 20, 8), (40, 8), 1.0, 1.0, 500.0, 1000.0)

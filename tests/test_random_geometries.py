@@ -112,7 +112,7 @@ def test_random_geometry_gpu_staged_equals_direct(cuda, seed):
         ws = torch.empty(max(plan.workspace_bytes(k, nb), 1), dtype=torch.uint8, device=cuda)
         print(f"[gpu] random geometry {seed} {op} staged vs direct, batch {nb}", flush=True)
         outs = []
-        for rows in (8 | 0x200, 8 | 0x100):  # XR_CELL_STAGED, XR_CELL_DIRECT
+        for rows in (8 | 0x200, 8 | 0x100):  # XR_CELL_LDS, XR_CELL_DIRECT
             out = plan.execute(k, x, s)
             plan.execute_into(k, x, out, ws, s, rows, 0, 0)
             outs.append(out)

@@ -5,7 +5,7 @@ Two knobs exist, one per kernel family (``Plan.kernel_kind(op)``):
 * cell gathers (A, B^T, B_W^T): ``tile_rows`` in {1, 2, 4, 8, 16} -- detector rows per
   thread.  More rows amortise each line's column overlaps and share row boundaries,
   but cost registers and therefore resident waves.  Tiles of 4 rows and more are also
-  tried with CELL_STAGED and CELL_DIRECT: each workgroup's table lookups served from
+  tried with CELL_LDS and CELL_DIRECT: each workgroup's table lookups served from
   blocks staged in LDS, or read from device memory (the same results either way).
 * voxel gathers (B, B_W, A^T): ``voxel_chunk`` in {1, 4, 8, 16} -- slices per thread --
   and 16 | VOXEL_HALVES, 32 | VOXEL_HALVES: 16 or 32 slices run as two halves that a wave
@@ -29,7 +29,7 @@ CELL = 1
 VOXEL = 2
 TILE_ROWS = (1, 2, 4, 8, 16)
 CELL_DIRECT = 0x100  # XR_CELL_DIRECT
-CELL_STAGED = 0x200  # XR_CELL_STAGED
+CELL_LDS = 0x200  # XR_CELL_LDS
 VOXEL_HALVES = 0x100  # XR_VOXEL_HALVES
 VOXEL_CHUNKS = (1, 4, 8, 16, 16 | VOXEL_HALVES, 32 | VOXEL_HALVES)
 
@@ -48,7 +48,7 @@ def cached(plan, op: int, nbatch: int):
 
 def candidates(plan, op: int):
     if plan.kernel_kind(op) == CELL:
-        return [(r | m, 0) for r in TILE_ROWS for m in ((CELL_DIRECT, CELL_STAGED) if r >= 4 else (0,))]
+        return [(r | m, 0) for r in TILE_ROWS for m in ((CELL_DIRECT, CELL_LDS) if r >= 4 else (0,))]
     return [(0, k) for k in VOXEL_CHUNKS]
 
 

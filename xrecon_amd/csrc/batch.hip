@@ -467,9 +467,9 @@ int xr_batch3d_execute(const xr_batch3d *b, int op, const float *const *in, floa
         io.out[i] = out[i];
     }
     // the batched cell gather reads the tables directly: the staging flags change nothing
-    if ((tile_rows & (XR_CELL_DIRECT | XR_CELL_STAGED)) == (XR_CELL_DIRECT | XR_CELL_STAGED))
-        return fail(XR_EINVAL, "tile_rows must be 1, 2, 4, 8 or 16, with at most one of XR_CELL_DIRECT, XR_CELL_STAGED");
-    int R = tile_rows & ~(XR_CELL_DIRECT | XR_CELL_STAGED), KC = voxel_chunk;
+    if ((tile_rows & (XR_CELL_DIRECT | XR_CELL_LDS)) == (XR_CELL_DIRECT | XR_CELL_LDS))
+        return fail(XR_EINVAL, "tile_rows must be 1, 2, 4, 8 or 16, with at most one of XR_CELL_DIRECT, XR_CELL_LDS");
+    int R = tile_rows & ~(XR_CELL_DIRECT | XR_CELL_LDS), KC = voxel_chunk;
     if (R == 0) {
         R = 1;
         for (const xr::Plan3d *p : b->plans) R = std::max(R, xr::default_rows(*p));
@@ -479,7 +479,7 @@ int xr_batch3d_execute(const xr_batch3d *b, int op, const float *const *in, floa
         for (const xr::Plan3d *p : b->plans) KC = std::max(KC, xr::default_chunk(*p));
     }
     if (!xr::valid_rows(R))
-        return fail(XR_EINVAL, "tile_rows must be 1, 2, 4, 8 or 16, with at most one of XR_CELL_DIRECT, XR_CELL_STAGED");
+        return fail(XR_EINVAL, "tile_rows must be 1, 2, 4, 8 or 16, with at most one of XR_CELL_DIRECT, XR_CELL_LDS");
     if (!xr::valid_chunk(KC)) return fail(XR_EINVAL, "voxel_chunk must be 1, 4, 8, 16, 16 | XR_VOXEL_HALVES or 32 | XR_VOXEL_HALVES");
     bool big = false;
     for (const xr::BatchDesc &d : b->desc) big = big || d.big;

@@ -582,13 +582,13 @@ int execute_phase(const Plan3d &p, int op, int phase, const float *in, float *ou
     const size_t need = workspace_bytes(p, op, nbatch);
     if (ws_bytes < need || (need && !ws))
         return fail(XR_EWORKSPACE, "workspace of " + std::to_string(ws_bytes) + " bytes, need " + std::to_string(need));
-    const int cell_mode = tile_rows & (XR_CELL_DIRECT | XR_CELL_STAGED);
-    tile_rows &= ~(XR_CELL_DIRECT | XR_CELL_STAGED);
+    const int cell_mode = tile_rows & (XR_CELL_DIRECT | XR_CELL_LDS);
+    tile_rows &= ~(XR_CELL_DIRECT | XR_CELL_LDS);
     if (tile_rows == 0) tile_rows = default_rows(p);
     if (voxel_chunk == 0) voxel_chunk = default_chunk(p);
-    if (!valid_rows(tile_rows) || cell_mode == (XR_CELL_DIRECT | XR_CELL_STAGED))
-        return fail(XR_EINVAL, "tile_rows must be 1, 2, 4, 8 or 16, with at most one of XR_CELL_DIRECT, XR_CELL_STAGED");
-    const bool stage = cell_mode == XR_CELL_STAGED || (cell_mode == 0 && make_cell_args(p, 1).band);
+    if (!valid_rows(tile_rows) || cell_mode == (XR_CELL_DIRECT | XR_CELL_LDS))
+        return fail(XR_EINVAL, "tile_rows must be 1, 2, 4, 8 or 16, with at most one of XR_CELL_DIRECT, XR_CELL_LDS");
+    const bool stage = cell_mode == XR_CELL_LDS || (cell_mode == 0 && make_cell_args(p, 1).band);
     if (!valid_chunk(voxel_chunk)) return fail(XR_EINVAL, "voxel_chunk must be 1, 4, 8, 16, 16 | XR_VOXEL_HALVES or 32 | XR_VOXEL_HALVES");
 
     if (rt.cell && cell_blocks(make_cell_args(p, nbatch), 1, p.ncls[0] + p.ncls[1] + p.ncls[2], nbatch) >=
