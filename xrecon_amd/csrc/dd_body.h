@@ -458,7 +458,9 @@ __host__ __device__ inline float fdk_weight(float sx, float sy, float px, float 
 constexpr int kPrefixChunk = 32;
 
 // z-prefix of one volume column (b, e = j*nx + i) in the x-driven line orientation:
-// W[b][k][j][i] = (J_k, f_k), J_k = sum_{k'<k} f[k'] - total/2 (FP64 accumulation).
+// W[b][k][j][i] = (J_k - k f_k, f_k), J_k = sum_{k'<k} f[k'] - total/2 (FP64 accumulation,
+// the entry rounded once), so that the depth integral up to z in [k, k+1] is
+// J_k + (z - k) f_k = fmaf(z, f_k, W.x): one fmaf, no fract.  Entry Nz is (J_Nz, 0).
 __host__ __device__ inline void vol_prefix_body(const float *__restrict__ f, float2 *__restrict__ W, int64_t plane,
                                                 int nz, int64_t b, int64_t e)
 {
@@ -481,7 +483,7 @@ __host__ __device__ inline void vol_prefix_body(const float *__restrict__ f, flo
 #pragma unroll
         for (int k = 0; k < kPrefixChunk; ++k)
             if (k0 + k < nz) {
-                dst[(int64_t)(k0 + k) * plane] = make_float2((float)run, v[k]);
+                dst[(int64_t)(k0 + k) * plane] = make_float2((float)(run - (double)(k0 + k) * (double)v[k]), v[k]);
                 run += (double)v[k];
             }
     }
@@ -585,7 +587,7 @@ __host__ __device__ inline void vol_prefix_reg(const float *__restrict__ f, floa
 #pragma unroll
     for (int k = 0; k < kPrefixRegRows; ++k)
         if (k < nz) {
-            dst[(int64_t)k * plane] = make_float2((float)run, v[k]);
+            dst[(int64_t)k * plane] = make_float2((float)(run - (double)k * (double)v[k]), v[k]);
             run += (double)v[k];
         }
     dst[(int64_t)nz * plane] = make_float2((float)run, 0.0f);
@@ -729,15 +731,15 @@ __host__ __device__ inline void cell_line(CellLane<R> &st, const ViewHdr &h, con
         return ou;
     };
 
-    // Row boundaries in voxel units along z at this line's magnification.
+    // Row boundaries in voxel units along z at this line's magnification: entry ki of the
+    // z-prefix table, interpolated at z (J = fmaf(z, f, W.x), see vol_prefix_body).
     const float fnz = (float)a.nz;
     int ki[R + 1];
     float fr[R + 1];
 #pragma unroll
     for (int q = 0; q <= R; ++q) {
-        const float z = clampf(fmaf(st.hv[q], lc.z, a.zoff), 0.0f, fnz);
-        ki[q] = (int)z;
-        fr[q] = fractf(z);
+        fr[q] = clampf(fmaf(st.hv[q], lc.z, a.zoff), 0.0f, fnz);
+        ki[q] = (int)fr[q];
     }
     const int plane = h.nline * nalong;
     const float2 *Wl = W + (int64_t)L * nalong;

@@ -12,7 +12,8 @@
 //    y/x-transposed copy so both orientations are coalesced); on each line the column's
 //    axis interval [d_c, d_c+1] covers a few pixels m, and each row's height interval
 //    covers a fraction of each pixel's z column.  That z overlap is read from a z-prefix
-//    table of the volume, J(q) = J[k] + (q - k) f[k], so one row costs one table lookup
+//    table of the volume, J(q) = J[k] + (q - k) f[k] (stored as J[k] - k f[k] beside f[k],
+//    read as one fmaf), so one row costs one table lookup
 //    per boundary instead of a loop over the voxels it overlaps, and the R rows share
 //    the line's column overlaps and their common row boundaries.  The 64-column groups
 //    of every view are classed by how many pixels their widest column can cover (two,
