@@ -26,6 +26,9 @@ int g_cell_variant = 0;
 // Staged cell gather: (thread, line) pairs served from an LDS block; workgroups
 // staged and not staged (the plan's flags).
 std::atomic<long long> g_cell_lds_lines{0}, g_cell_wg_staged{0}, g_cell_wg_direct{0};
+// Staged blocks whose copies would start before their line or end more than three
+// entries past it.
+std::atomic<long long> g_cell_block_out{0};
 
 // Number of (workgroup, view) pairs the voxel gather passed over (VoxBox::skip).
 std::atomic<long long> g_voxel_skips{0};
@@ -104,6 +107,8 @@ void run_cell_quad(const Plan3d &p, const CellArgs &a, const int *items, int n, 
     cell_chunk_pitches(ql, h0.nline, pitch.data());
     for (int L = 0; L < h0.nline; ++L) {
         const VoxBox bx = cell_box<R>(ql[L], a, r0, pitch[L / kCellChunk], h0.nalong);
+        // the copies stay within the line and three entries past it (the pad after the table)
+        if (bx.nc > 0 && (bx.c0 < 0 || bx.c0 + bx.pitch > h0.nalong + 3)) g_cell_block_out.fetch_add(1);
         std::fill(buf.begin(), buf.end(), float2{nan, nan});
         for (int tid = 0; tid < kBlock; ++tid) voxel_stage(W + (int64_t)L * h0.nalong, bx, buf.data(), plane, tid, kBlock);
         for (int t = 0; t < kBlock; ++t) {
@@ -386,18 +391,20 @@ void emu_set_cell_variant(int v) { g_cell_variant = v; }
 void emu_set_voxel_variant(int v) { g_voxel_variant = v; }
 
 // The staged cell gather's counts since the last reset: (thread, line) pairs served from
-// an LDS block [0], lookups outside their block [1] (must stay 0), workgroups of classes
-// 2 and 3 staged [2] and direct [3] (per batch item 0).
+// an LDS block [0], lookups outside their block and blocks reaching outside their line
+// and its three-entry margin [1] (must stay 0), (workgroup, row tile) pairs staged [2]
+// and direct [3] (per batch item 0).
 void emu_cell_lines(long long *counts, int reset)
 {
 #if !defined(__HIP_DEVICE_COMPILE__)  // (this file's device pass compiles nothing it uses)
     counts[0] = g_cell_lds_lines.load();
-    counts[1] = cell_box_misses.load();
+    counts[1] = cell_box_misses.load() + g_cell_block_out.load();
     counts[2] = g_cell_wg_staged.load();
     counts[3] = g_cell_wg_direct.load();
     if (reset) {
         g_cell_lds_lines = 0;
         cell_box_misses = 0;
+        g_cell_block_out = 0;
         g_cell_wg_staged = 0;
         g_cell_wg_direct = 0;
     }

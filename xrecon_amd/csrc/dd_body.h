@@ -964,6 +964,7 @@ __host__ __device__ inline VoxBox cell_box(const QuadLine &q, const CellArgs &a,
     const int ns = s1 - s0 + 1;
     if (ns * pitch > kCellLds) return bx;
     bx.c0 = imin(q.m_lo, nalong + 3 - pitch);
+    if (bx.c0 < 0) return VoxBox{0, 0, 0, 0, 0, 0};  // (pitch above Nalong + 3: not a QuadLine's)
     bx.nc = pitch;
     bx.r0 = s0;
     bx.nr = ns;
