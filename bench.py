@@ -628,11 +628,13 @@ def main():
         # LDS-staged classes 2 and 3 at 4 rows and more: as tuned (this table is below the
         # 128 MiB the library stages by default)
         staged = bool(tune_a[0] & autotune.CELL_LDS)
+        blocks = [-(-n // 4) * -(-wl["Nv"] // r) for r, n in zip(rows, ncls)]
         if -(-sum(ncls) // 4) * -(-wl["Nv"] // r_a) < 4096:  # small grids: one launch for all classes
             symbol = [f"k_cell<0, 1, {r_a}, -1, false, false>"]
-        else:
-            symbol = [f"k_cell<0, 1, {r}, {span}, false, {'true' if staged and span and r >= 4 else 'false'}>"
-                      for r, span, n in zip(rows, (2, 3, 0), ncls) if n > 0]
+        else:  # a class launch stages only with 4096 workgroups or more
+            symbol = [f"k_cell<0, 1, {r}, {span}, false, "
+                      f"{'true' if staged and span and r >= 4 and nb >= 4096 else 'false'}>"
+                      for r, span, n, nb in zip(rows, (2, 3, 0), ncls, blocks) if n > 0]
         dominant, t_dom = "cell_gather(A)", t_cell
     else:
         kc = tune_b[1]
